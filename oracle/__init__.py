@@ -1,0 +1,20 @@
+"""CPU oracle of the pGM electrostatics hot path: TEST INFRASTRUCTURE ONLY.
+
+This package is a float64 numpy restatement of the reference's algorithm (pGM-JAX: md/kernels.py,
+md/pme.py, md/forcefield.py, ewald.py, solver.py, lj.py) and the checker against which the HIP
+library is tested.  Only `tests/`, `__graft_entry__.smoke()` and `bench.py`'s CPU-baseline leg may
+import it, and there only as the checker or as the timed CPU baseline; the product package
+(`pgm-jax_amd/`) never imports, links or executes anything from here.
+
+Modules
+-------
+systems     test systems: toy water/methanol box, pGM3P-25 water boxes, prmtop/rst7 readers
+kernels     radial kernels B_n of erf(a r)/r (series + closed form)
+pme         smooth PME for Gaussian charges and point dipoles (splines, moduli, spread, FFT, gather)
+pairs       minimum image and neighbour pairs (brute force and cell list)
+forcefield  the MD force field: field, CG induction solver, energies, forces, strain derivative
+ewald       exact Ewald reference (direct + structure-factor sums, dense induction solve)
+
+Every function cites the reference file:line it restates.  Parity status of each function (how it
+is pinned) is listed in the header of the module that holds it.
+"""

@@ -1,0 +1,312 @@
+"""The pGM + LJ MD force field in float64 (restates pgm_jax/md/forcefield.py, the hot path).
+
+E = KE [U_dir + U_rec + U_self + U_bg + sum |mu|^2/(2 alpha)] + E_LJ,   d = p(R) + mu   (forcefield.py:8-16)
+U_dir = sum_{pairs < rc} q_i q_j G0 + (q_i d_j.x - q_j d_i.x) G1 - G2 (d_i.x)(d_j.x) + G1 d_i.d_j,
+G_n = B_n[erf(a_ij r)/r] - B_n[erf(b0 r)/r], a_ij = 1/sqrt(2 (R_i^2 + R_j^2)).
+
+Restated functions (reference file:line):
+  perm_dipoles        forcefield.py:1032-1042        p_i = sum c_m unit(min_image(r_j - r_i))
+  perm_dipoles_vjp    the jax.vjp of forcefield.py:2412 written out
+  row_field           forcefield.py:1395-1402        sum_k de_ik/dd_i (minus the direct field)
+  field               forcefield.py:1428-1433        -(row + PME - c_self d)
+  operator            forcefield.py:1694-1712        A v = v/alpha - field(0, v)
+  cg                  forcefield.py:1714-1821        Jacobi PCG, Polak-Ribiere beta, max|alpha r|/norm, peek
+  solve               forcefield.py:1917-2008        mu4 / mu3 predictor, fused residual, history push
+  pair_terms          forcefield.py:2294-2360        pair energies and analytic row gradients, LJ rows
+  nonpair             forcefield.py:2011-2045        PME + self + background + polarisation energies
+  energy_forces       forcefield.py:2362-2430        forces = -(KE g_el + g_lj + grad_np + VJP_p(dE/dd))
+  lj_long_range       lj.py:87-114                   continuum tail -2 pi/(3 V rc^3) sum_ij 2 eps rmin^6
+  compute             forcefield.py:2489-2560        one force evaluation -> Result
+  strain_derivative   forcefield.py:2628-2695        molecular dE/d eps at fixed mu (+ tail impulse)
+
+Every pair is kept once (i < j); the reference keeps it in both rows and halves the sums, which is
+the same arithmetic in float64.  Only the "qpi" electrostatics with "lj" van der Waals, the PME
+long-range treatment, the Jacobi preconditioner and nonzero polarizabilities are restated (the
+settings of every benchmark and golden case here).
+
+Parity: pinned by the golden ff_small_box (double: energies, forces, induced dipoles, CG iterations,
+residual, strain) and ff_pgm3p25_512 (double) single points of the reference, by the exact Ewald
+oracle (dense induction solve) as the PME grid grows, and by finite differences of the energy for
+the forces and the strain derivative.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .kernels import pair_kernels
+from .pairs import min_image, pairs
+from .pme import PME, grid_size
+
+KE = 138.935458  # Coulomb constant, kJ/mol nm / e^2 (pgm_jax/units.py:31)
+SQRT_PI = math.sqrt(math.pi)
+PRED = {"mu3": (3.0, -3.0, 1.0), "mu4": (4.0, -6.0, 4.0, -1.0)}  # forcefield.py:605
+
+
+@dataclasses.dataclass(frozen=True)
+class Settings:
+    """The MDSettings fields of the hot path (forcefield.py:142-454 defaults)."""
+
+    cutoff: float = 0.9
+    ewald_beta: float = 4.0
+    pme_grid: tuple | None = None
+    pme_spacing: float = 0.08
+    pme_order: int = 6
+    lj_lrc: bool = True
+    dipole_tol: float = 1e-5
+    max_iter: int = 50
+    predictor: str = "mu4"
+    fused: bool = True
+    norm_refresh: int = 1000
+    peek: float = 0.65
+
+
+@dataclasses.dataclass
+class InductionState:
+    """mu (N, 3), hist (4, N, 3) newest first, count, norm (forcefield.py:679-713)."""
+
+    mu: np.ndarray
+    hist: np.ndarray
+    count: int
+    norm: float
+
+
+@dataclasses.dataclass
+class Result:
+    """energy {"elec", "vdw", "total"} [kJ/mol], forces (N, 3), mu, iterations, residual (forcefield.py:716-746)."""
+
+    energy: dict
+    forces: np.ndarray
+    mu: np.ndarray
+    iterations: int
+    residual: float
+    induction: InductionState
+
+
+def lj_long_range(rh: np.ndarray, se: np.ndarray, volume: float, rc: float) -> float:
+    """Return E_lrc = -2 pi/(3 V rc^3) sum_{i,j} 2 eps_ij rmin_ij^6 in O(N) by moments (lj.py:87-114)."""
+    M = [np.sum(se * rh**k) for k in range(7)]
+    SB = 2.0 * sum(math.comb(6, k) * M[k] * M[6 - k] for k in range(7))
+    return -2.0 * math.pi * SB / (3.0 * volume * rc**3)
+
+
+class ForceField:
+    """The force field of one system (PGMForceField, forcefield.py:810-1000), float64."""
+
+    def __init__(self, top, H: np.ndarray, settings: Settings = Settings()) -> None:
+        """Set up PME (grid from settings or grid_size(H, spacing)), b0 and c_self (forcefield.py:940-960)."""
+        self.top, self.s = top, settings
+        self.b0 = float(settings.ewald_beta)
+        self.c_self = 4.0 * self.b0**3 / (3.0 * SQRT_PI)
+        grid = settings.pme_grid or grid_size(H, settings.pme_spacing)
+        self.pme = PME(grid, settings.pme_order, self.b0)
+        self.rc = float(settings.cutoff)
+
+    # ------------------------------------------------------------------ geometry
+    def geometry(self, pos: np.ndarray, H: np.ndarray) -> dict:
+        """Return the pairs within rc with their kernels G0..G3 and LJ parameters (forcefield.py:1340-1380)."""
+        t = self.top
+        g = pairs(pos, H, self.rc, t.mol, t.first)
+        i, j = g["i"], g["j"]
+        a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
+        g["G"] = pair_kernels(a, self.b0, g["r"], 4)
+        g["wv"] = np.where(g["intra"], 0.0, 1.0)  # rigid topology: intramolecular vdW weight 0
+        g["rmin"] = t.lj_rmin_half[i] + t.lj_rmin_half[j]
+        g["eps"] = t.lj_sqrt_eps[i] * t.lj_sqrt_eps[j]
+        return g
+
+    def perm_dipoles(self, pos: np.ndarray, H: np.ndarray) -> np.ndarray:
+        """Return p_i = sum_m c_m unit(min_image(r_j - r_i)) (forcefield.py:1032-1042)."""
+        t = self.top
+        v = min_image(pos[t.cov_j] - pos[t.cov_i], H)
+        u = v / np.linalg.norm(v, axis=-1, keepdims=True)
+        p = np.zeros((t.n, 3))
+        np.add.at(p, t.cov_i, t.cov_c[:, None] * u)
+        return p
+
+    def perm_dipoles_vjp(self, pos: np.ndarray, H: np.ndarray, gp: np.ndarray) -> np.ndarray:
+        """Return sum_i gp_i . dp_i/dr (N, 3): the pull-back through the covalent frames."""
+        t = self.top
+        v = min_image(pos[t.cov_j] - pos[t.cov_i], H)
+        L = np.linalg.norm(v, axis=-1, keepdims=True)
+        u = v / L
+        gi = gp[t.cov_i]
+        tv = t.cov_c[:, None] * (gi - np.sum(gi * u, -1, keepdims=True) * u) / L
+        out = np.zeros((t.n, 3))
+        np.add.at(out, t.cov_j, tv)
+        np.add.at(out, t.cov_i, -tv)
+        return out
+
+    # ------------------------------------------------------------------ fields and operator
+    def row_field(self, g: dict, q: np.ndarray | None, d: np.ndarray) -> np.ndarray:
+        """Return sum_k de_ik/dd_i (N, 3), minus the direct-space field (forcefield.py:1395-1402)."""
+        i, j, x = g["i"], g["j"], g["x"]
+        G1, G2 = g["G"][1], g["G"][2]
+        di, dj = d[i], d[j]
+        dix = np.sum(di * x, -1)
+        djx = np.sum(dj * x, -1)
+        ci = -G2 * djx
+        cj = -G2 * dix
+        if q is not None:
+            ci = ci - q[j] * G1
+            cj = cj + q[i] * G1
+        out = np.zeros_like(d, dtype=float)
+        np.add.at(out, i, ci[:, None] * x + G1[:, None] * dj)
+        np.add.at(out, j, cj[:, None] * x + G1[:, None] * di)
+        return out
+
+    def field(self, g: dict, S: dict, Gk: np.ndarray, q: np.ndarray | None, d: np.ndarray) -> np.ndarray:
+        """Return the total field -dU/dd = -(row + PME - c_self d) [e/nm^2] (forcefield.py:1428-1433)."""
+        qq = np.zeros(self.top.n) if q is None else q
+        rec, _ = self.pme.gradients(S, Gk, qq, d)
+        return -(self.row_field(g, q, d) + rec - self.c_self * d)
+
+    def operator(self, g: dict, S: dict, Gk: np.ndarray):
+        """Return A: v -> v/alpha - field(0, v) (forcefield.py:1694-1712)."""
+        inv_a = 1.0 / self.top.alpha[:, None]
+        return lambda v: v * inv_a - self.field(g, S, Gk, None, v)
+
+    def cg(self, A, x: np.ndarray, r: np.ndarray, norm: float, tol=None, peek=None, max_iter=None) -> tuple:
+        """Jacobi PCG from (x0, r0 = b - A x0) (forcefield.py:1714-1821); returns (mu, iterations, residual)."""
+        s = self.s
+        tol = s.dipole_tol if tol is None else tol
+        peek = s.peek if peek is None else peek
+        max_iter = s.max_iter if max_iter is None else max_iter
+        a_c = self.top.alpha[:, None]
+        z = r * a_c
+        p = z.copy()
+        rz = float(np.sum(r * z))
+        it = 0
+        err = float(np.max(np.abs(r * a_c))) / norm
+        while err > tol and it < max_iter:
+            Ap = A(p)
+            al = rz / max(float(np.sum(p * Ap)), 1e-300)
+            x = x + al * p
+            r_new = r - al * Ap
+            z_new = r_new * a_c
+            beta = float(np.sum(z_new * (r_new - r))) / max(rz, 1e-300)
+            p = z_new + beta * p
+            rz = float(np.sum(r_new * z_new))
+            r = r_new
+            it += 1
+            err = float(np.max(np.abs(r * a_c))) / norm
+        if peek:
+            x = x + peek * r * a_c
+        return x, it, err
+
+    def init_induction(self) -> InductionState:
+        """Return a fresh solver state: zero dipoles and history, normaliser 1 (forcefield.py:1436-1450)."""
+        n = self.top.n
+        return InductionState(np.zeros((n, 3)), np.zeros((4, n, 3)), 0, 1.0)
+
+    def solve(self, g: dict, S: dict, Gk: np.ndarray, p: np.ndarray, ind: InductionState) -> tuple:
+        """Predictor + (fused) initial residual + CG; returns (mu, it, err, new state) (forcefield.py:1917-2008)."""
+        t, s = self.top, self.s
+        a64 = t.alpha[:, None]
+        A = self.operator(g, S, Gk)
+        q = t.q
+
+        def plain(x0h, have):
+            b = self.field(g, S, Gk, q, p)
+            ab = a64 * b
+            x0 = x0h if have else ab
+            return x0, b - A(x0), float(np.mean(np.abs(ab))) + 1e-300
+
+        if s.predictor in PRED:
+            c = PRED[s.predictor]
+            have = ind.count >= len(c)
+            x0h = sum(ci * ind.hist[k] for k, ci in enumerate(c))
+            if s.fused and have and ind.count % s.norm_refresh != 0:
+                x0 = x0h
+                r0 = self.field(g, S, Gk, q, p + x0h) - x0h / a64
+                norm = ind.norm
+            else:
+                x0, r0, norm = plain(x0h, have)
+        else:
+            x0, r0, norm = plain(None, False)
+        mu, it, err = self.cg(A, x0, r0, norm)
+        hist = np.concatenate([mu[None], ind.hist[:-1]], 0)
+        return mu, it, err, InductionState(mu, hist, ind.count + 1, norm)
+
+    # ------------------------------------------------------------------ energies and forces
+    def pair_terms(self, g: dict, q: np.ndarray, d: np.ndarray) -> tuple:
+        """Return (sum e_elec, sum e_lj, g_el (N,3), g_lj (N,3)) over unique pairs (forcefield.py:2294-2360, :2116-2146)."""
+        i, j, x, r = g["i"], g["j"], g["x"], g["r"]
+        G0, G1, G2, G3 = g["G"]
+        qi, qk = q[i], q[j]
+        di, dk = d[i], d[j]
+        dix = np.sum(di * x, -1)
+        dkx = np.sum(dk * x, -1)
+        didk = np.sum(di * dk, -1)
+        tt = qi * dkx - qk * dix
+        e = qi * qk * G0 + tt * G1 - G2 * dix * dkx + G1 * didk
+        radial = -qi * qk * G1 - tt * G2 + G3 * dix * dkx - G2 * didk
+        cross = di * dkx[:, None] + dk * dix[:, None]
+        gx = radial[:, None] * x + (qi * G1)[:, None] * dk - (qk * G1)[:, None] * di - G2[:, None] * cross
+        s6 = (g["rmin"] / r) ** 6
+        on = (g["wv"] != 0) & (g["eps"] != 0)
+        elj = np.where(on, g["eps"] * (s6 * s6 - 2.0 * s6), 0.0)
+        dlj = np.where(on, g["eps"] * 12.0 * (s6 - s6 * s6) / (r * r), 0.0)
+        glj = dlj[:, None] * x
+        n = len(q)
+        g_el = np.zeros((n, 3))
+        g_lj = np.zeros((n, 3))
+        np.add.at(g_el, i, gx)
+        np.add.at(g_el, j, -gx)
+        np.add.at(g_lj, i, glj)
+        np.add.at(g_lj, j, -glj)
+        return float(np.sum(e)), float(np.sum(elj)), g_el, g_lj
+
+    def vdw_tail(self, H: np.ndarray) -> float:
+        """Return the LJ long-range correction [kJ/mol] (forcefield.py:2148-2168 -> lj.py:87-114)."""
+        if not self.s.lj_lrc:
+            return 0.0
+        t = self.top
+        return lj_long_range(t.lj_rmin_half, t.lj_sqrt_eps, abs(np.linalg.det(H)), self.rc)
+
+    def nonpair(self, S: dict, Gk: np.ndarray, d: np.ndarray, mu: np.ndarray, H: np.ndarray) -> float:
+        """Return KE [U_rec + U_self + U_bg + U_pol] [kJ/mol] (forcefield.py:2011-2045)."""
+        t = self.top
+        u_rec = self.pme.energy(S, Gk, t.q, d)
+        u_self = -(self.b0 / SQRT_PI) * np.sum(t.q**2) - 0.5 * self.c_self * np.sum(d * d)
+        u_bg = -math.pi * np.sum(t.q) ** 2 / (2.0 * abs(np.linalg.det(H)) * self.b0**2)
+        u_pol = np.sum(mu * mu / (2.0 * t.alpha[:, None]))
+        return KE * (u_rec + u_self + u_bg + u_pol)
+
+    def energy_forces(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, g: dict, S: dict, Gk: np.ndarray) -> tuple:
+        """Return ({"elec", "vdw", "total"}, forces (N, 3)) at fixed mu (forcefield.py:2362-2430)."""
+        t = self.top
+        p = self.perm_dipoles(pos, H)
+        d = p + mu
+        se, sl, g_el, g_lj = self.pair_terms(g, t.q, d)
+        dEdd = KE * self.row_field(g, t.q, d)
+        rec_dd, rec_dr = self.pme.gradients(S, Gk, t.q, d)
+        gd_np = KE * (rec_dd - self.c_self * d)
+        gpos_np = KE * rec_dr
+        forces = -(KE * g_el + g_lj + gpos_np + self.perm_dipoles_vjp(pos, H, dEdd + gd_np))
+        e_elec = KE * se + self.nonpair(S, Gk, d, mu, H)
+        e_lj = sl + self.vdw_tail(H)
+        return {"elec": e_elec, "vdw": e_lj, "total": e_elec + e_lj}, forces
+
+    def compute(self, pos: np.ndarray, H: np.ndarray, ind: InductionState | None = None) -> Result:
+        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560)."""
+        pos = np.asarray(pos, float)
+        H = np.asarray(H, float)
+        ind = self.init_induction() if ind is None else ind
+        g = self.geometry(pos, H)
+        p = self.perm_dipoles(pos, H)
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        mu, it, err, ind = self.solve(g, S, Gk, p, ind)
+        energy, forces = self.energy_forces(pos, H, mu, g, S, Gk)
+        return Result(energy, forces, mu, it, err, ind)
+
+    def energy_fixed_mu(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> float:
+        """Return the total energy [kJ/mol] at fixed induced dipoles (forcefield.py:2249-2292)."""
+        g = self.geometry(pos, H)
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        return self.energy_forces(pos, H, mu, g, S, Gk)[0]["total"]

@@ -1,0 +1,141 @@
+"""Write the committed test fixtures from data files that the reference repository holds.
+
+Run in the build container only (the reference is not present on GPU machines):
+
+    python tests/golden/make_fixtures.py [/root/reference]
+
+Every output is data (inputs and expected outputs), never reference source:
+
+* ``water512.npz``   -- the 512-water pGM3P-25 box of ``examples/water512`` (prmtop + NetCDF rst7):
+  coordinates and velocities (nm, nm/ps), cell lengths/angles, and the per-water pGM parameters
+  read from the POL_GAUSS_* / LENNARD_JONES_* / MASS sections of the prmtop (converted to nm, e,
+  e nm, nm^3, R*/2 nm, sqrt(kJ/mol) the way the reference's ``param.read_prmtop_pgm``
+  (pgm_jax/param.py:113-233, :281-307) defines them).
+* ``ff_small_box.npz``, ``ff_pgm3p25_512.npz``, ``periodic_model.npz`` -- copies of the golden
+  outputs of the reference's regression harness (tests/regression/golden/), recorded with jax 0.11.2.
+* ``pmemd_pbc.npz`` -- the independent Fortran (pmemd-pgm) single point of
+  data/validation/amber_ref/pmemd_pbc: EELEC / VDWAALS (kcal/mol), forces (kcal/mol/A) and induced
+  dipoles (e A).
+
+Only numpy / scipy text and NetCDF readers are used (no pickles, nothing executed from the files).
+"""
+
+from __future__ import annotations
+
+import os
+import re
+import sys
+
+import numpy as np
+from scipy.io import netcdf_file
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KCAL = 4.184
+ANG_NM = 0.1
+
+
+def read_prmtop_sections(path: str) -> dict:
+    """Return {flag: list of str tokens} of an Amber prmtop (fixed-width FORMAT fields)."""
+    out, flag, fmt = {}, None, None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("%FLAG"):
+                flag = line.split()[1]
+                out[flag] = []
+                fmt = None
+            elif line.startswith("%FORMAT"):
+                fmt = re.search(r"\((\d+)([aIE])(\d+)", line)
+            elif line.startswith("%COMMENT") or line.startswith("%VERSION"):
+                continue
+            elif flag is not None and fmt is not None:
+                width = int(fmt.group(3))
+                s = line.rstrip("\n")
+                out[flag] += [s[k : k + width].strip() for k in range(0, len(s), width) if s[k : k + width].strip()]
+    return out
+
+
+def water512(ref: str) -> dict:
+    """Return the water512 box: coordinates, velocities, cell and one water's pGM parameters."""
+    top = read_prmtop_sections(os.path.join(ref, "examples/water512/pgm3p25_512.prmtop"))
+    q = np.array(top["POL_GAUSS_MONOPOLES_LIST"], float)
+    rad = np.array(top["POL_GAUSS_RADII_LIST"], float) * ANG_NM
+    alp = np.array(top["POL_GAUSS_POLARIZABILITY_LIST"], float) * ANG_NM**3
+    nptr = np.array(top["POL_GAUSS_COVALENT_POINTERS_LIST"], int)
+    catm = np.array(top["POL_GAUSS_COVALENT_ATOMS_LIST"], int) - 1
+    cdip = np.array(top["POL_GAUSS_COVALENT_DIPOLES_LIST"], float) * ANG_NM
+    mass = np.array(top["MASS"], float)
+    ntypes = int(top["POINTERS"][1])
+    ti = np.array(top["ATOM_TYPE_INDEX"], int) - 1
+    nbi = np.array(top["NONBONDED_PARM_INDEX"], int).reshape(ntypes, ntypes) - 1
+    A = np.array(top["LENNARD_JONES_ACOEF"], float)[nbi]
+    B = np.array(top["LENNARD_JONES_BCOEF"], float)[nbi]
+    a, b = np.diag(A), np.diag(B)
+    ok = (a > 0) & (b > 0)
+    rmin = np.where(ok, (2 * np.where(ok, a, 1) / np.where(ok, b, 1)) ** (1 / 6), 0.0)
+    eps = np.where(ok, b**2 / (4 * np.where(ok, a, 1)), 0.0)
+    rh = (rmin / 2 * ANG_NM)[ti]
+    se = np.sqrt(eps * KCAL)[ti]
+    start = np.concatenate([[0], np.cumsum(nptr)])
+    cov = [(i, catm[k], cdip[k]) for i in range(3) for k in range(start[i], start[i + 1])]
+    nc = netcdf_file(os.path.join(ref, "examples/water512/pgm3p25_512.rst7"), "r", mmap=False)
+    xyz = np.array(nc.variables["coordinates"].data, float)
+    vel = np.array(nc.variables["velocities"].data, float)
+    lengths = np.array(nc.variables["cell_lengths"].data, float)
+    angles = np.array(nc.variables["cell_angles"].data, float)
+    nc.close()
+    return {
+        "pos": xyz * ANG_NM,
+        "vel": vel * ANG_NM,  # A/ps -> nm/ps
+        "cell_lengths": lengths * ANG_NM,
+        "cell_angles": angles,
+        "q": q[:3],
+        "radius": rad[:3],
+        "alpha": alp[:3],
+        "lj_rmin_half": rh[:3],
+        "lj_sqrt_eps": se[:3],
+        "masses": mass[:3],
+        "cov_i": np.array([c[0] for c in cov], np.int32),
+        "cov_j": np.array([c[1] for c in cov], np.int32),
+        "cov_c": np.array([c[2] for c in cov], float),
+        "all_q": q,
+        "all_radius": rad,
+        "all_alpha": alp,
+    }
+
+
+def pmemd_pbc(ref: str) -> dict:
+    """Return pmemd-pgm's single point of the 512-water box (data/validation/amber_ref/pmemd_pbc)."""
+    d = os.path.join(ref, "data/validation/amber_ref/pmemd_pbc")
+    text = open(os.path.join(d, "mdout")).read()
+    eelec = float(re.findall(r"EELEC\s*=\s*(-?\d+\.\d+)", text)[0])
+    vdw = float(re.findall(r"VDWAALS\s*=\s*(-?\d+\.\d+)", text)[0])
+    nc = netcdf_file(os.path.join(d, "mdfrc"), "r", mmap=False)
+    frc = np.array(nc.variables["forces"].data, float)[0]
+    nc.close()
+    mu = []
+    for line in open(os.path.join(d, "cpu_cg_iter_041.dat")):
+        parts = line.split()
+        try:
+            vals = [float(v) for v in parts]
+        except ValueError:
+            continue
+        if len(vals) == 5:  # atom, x, y, z, |x|
+            mu.append(vals[1:4])
+    return {"eelec_kcal": np.array(eelec), "vdw_kcal": np.array(vdw), "forces_kcal_A": frc, "mu_eA": np.array(mu)}
+
+
+def main(ref: str) -> None:
+    """Write every fixture into tests/golden/."""
+    np.savez_compressed(os.path.join(HERE, "water512.npz"), **water512(ref))
+    gold = os.path.join(ref, "tests/regression/golden")
+    for name in ("ff_small_box", "ff_pgm3p25_512", "periodic_model"):
+        with np.load(os.path.join(gold, name + ".npz"), allow_pickle=False) as z:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **{k: z[k] for k in z.files})
+    try:
+        np.savez_compressed(os.path.join(HERE, "pmemd_pbc.npz"), **pmemd_pbc(ref))
+    except (OSError, IndexError, KeyError) as exc:  # optional independent check
+        print("pmemd_pbc fixture skipped:", exc)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
