@@ -1,0 +1,125 @@
+/*
+ * pgm_hip.h -- C ABI of the MI355X (gfx950) pGM electrostatics library (libpgm_hip.so).
+ *
+ * One context holds the static topology of one system (per-atom charges, Gaussian radii,
+ * polarizabilities, LJ parameters, molecules, covalent dipoles), the settings, the PME grid with
+ * its rocFFT plans, the neighbour rows and every scratch buffer.  Positions, forces, dipoles and
+ * the induction state are caller-owned DEVICE buffers (float64, row-major (N, 3)).  Every call is
+ * issued on the context's stream and returns without synchronising unless stated.  No C++
+ * exception crosses this boundary; every function returns a status code:
+ *
+ *   PGM_OK 0, PGM_ROW_OVERFLOW 1 (neighbour-row capacity exceeded: results invalid, call
+ *   pgm_build_rows again -- it grows the capacity), PGM_NOT_CONVERGED 3 (CG stopped at max_iter;
+ *   results valid, residual reported), negative values are errors (pgm_last_error() describes them).
+ *
+ * Units: nm, e, e nm, nm^3, kJ/mol, kJ/mol/nm.  Boxes are reduced lower-triangular with the
+ * lattice vectors as rows (H[3*a + b] = component b of vector a).
+ *
+ * Each entry point names the reference interface (pgm-jax) it replaces.
+ */
+#ifndef PGM_HIP_H
+#define PGM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGM_OK 0
+#define PGM_ROW_OVERFLOW 1
+#define PGM_NOT_CONVERGED 3
+#define PGM_ERR_ARG -1
+#define PGM_ERR_HIP -2
+#define PGM_ERR_FFT -3
+#define PGM_ERR_STATE -4
+
+typedef struct pgm_ctx pgm_ctx;
+
+/* Per-atom parameters (host arrays, float64 / int32), the expand() output of the reference's
+ * System (pgm_jax/system.py:561-577); covalent dipole m puts c[m] e nm on atom cov_i[m] along
+ * unit(r_{cov_j[m]} - r_{cov_i[m]}) (pgm_jax/md/forcefield.py:1032-1042). */
+typedef struct {
+    int n_atoms;
+    const double *q, *radius, *alpha, *lj_rmin_half, *lj_sqrt_eps;
+    const int *mol;
+    int n_cov;
+    const int *cov_i, *cov_j;
+    const double *cov_c;
+} pgm_topology;
+
+/* The MDSettings fields of the hot path (pgm_jax/md/forcefield.py:142-454). */
+typedef struct {
+    double cutoff;      /* pair cutoff rc [nm] (Cutoffs.cutoff) */
+    double skin;        /* neighbour-row skin [nm] (NeighborList.skin) */
+    double ewald_beta;  /* [1/nm] (PMESettings.ewald_beta) */
+    int grid[3];        /* PME grid K1, K2, K3 (PMESettings.grid) */
+    int order;          /* B-spline order (PMESettings.order), 4..8 */
+    int lj_lrc;         /* LJ long-range correction (Terms.lj_lrc) */
+    double dipole_tol;  /* max|alpha r| / mean|alpha b| (Induction.tol) */
+    int max_iter;       /* Induction.max_iter */
+    int predictor;      /* 0 none, 1 mu3, 2 mu4, 3 previous step (Induction.predictor) */
+    int fused;          /* fused initial residual (Induction.fused) */
+    int norm_refresh;   /* Induction.norm_refresh */
+    double peek;        /* Induction.peek */
+    int precision;      /* 0 double, 1 mixed (MDSettings.precision) */
+} pgm_settings;
+
+/* Induction state (pgm_jax/md/forcefield.py:679-713 InductionState), caller-owned:
+ * mu (N,3) f64 device, hist (4,N,3) f64 device (newest first), count and norm on the host. */
+typedef struct {
+    double *mu;
+    double *hist;
+    int count;
+    double norm;
+} pgm_induction;
+
+/* Energy parts [kJ/mol] (Result.energy of forcefield.py:716-746). */
+typedef struct {
+    double elec, vdw, total;
+} pgm_energy;
+
+const char *pgm_last_error(void);
+int pgm_version(void);
+
+/* PGMForceField.__init__ (pgm_jax/md/forcefield.py:868-1000): build the context. `stream` is a
+ * hipStream_t (NULL: the null stream).  H is the box used for the cell grid and influence function. */
+int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_settings *s, const double *H, void *stream);
+int pgm_ctx_destroy(pgm_ctx *ctx);
+int pgm_set_stream(pgm_ctx *ctx, void *stream);
+
+/* PME.influence + box change (pgm_jax/md/pme.py:243-267): new box, influence function, cells. */
+int pgm_set_box(pgm_ctx *ctx, const double *H);
+
+/* Neighbour rows (replaces the neighbour list + PGMForceField._rows/_compact_parts,
+ * pgm_jax/md/neighbors.py:525-558, forcefield.py:1140-1186): every pair within cutoff + skin of
+ * the device positions pos (N,3).  Returns PGM_ROW_OVERFLOW after growing the capacity. */
+int pgm_build_rows(pgm_ctx *ctx, const double *pos);
+
+/* Number of stored row entries (pairs counted twice) within cutoff at the last compute. */
+long long pgm_row_entries(pgm_ctx *ctx);
+
+/* PGMForceField.perm_dipoles (forcefield.py:1032-1042): p (N,3) f64 device. */
+int pgm_perm_dipoles(pgm_ctx *ctx, const double *pos, double *p);
+
+/* PGMForceField._field (forcefield.py:1428-1433): the total field -dU/dd (N,3) f64 device of
+ * charges (q_scale * topology charges, q_scale 0 or 1) and dipoles d (N,3) f64 device. */
+int pgm_field(pgm_ctx *ctx, const double *pos, int with_charges, const double *d, double *field);
+
+/* PME.energy / PME.grad_dipoles (pgm_jax/md/pme.py:339-411) of the topology charges times
+ * q_scale and dipoles d: E_rec [e^2/nm] (host) and dU_rec/dd (N,3) device (may be NULL). */
+int pgm_pme_energy(pgm_ctx *ctx, const double *pos, int with_charges, const double *d, double *e_rec,
+                   double *grad_d);
+
+/* PGMForceField.compute (forcefield.py:2489-2560): rows compaction, permanent dipoles, PME setup,
+ * predictor + CG induced dipoles (updates ind), energies and forces (N,3) f64 device.
+ * iterations and residual as Result.iterations / Result.residual. */
+int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces,
+                int *iterations, double *residual);
+
+/* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy
+ * and forces at fixed induced dipoles mu (N,3) f64 device. */
+int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_energy *energy, double *forces);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGM_HIP_H */

@@ -1,0 +1,128 @@
+"""ctypes binding of libpgm_hip.so (include/pgm_hip.h).
+
+The library is loaded from this package directory.  There is no fallback: if it is missing or a
+GPU is absent, calls raise.  Device pointers come from torch CUDA (HIP) tensors; torch is only the
+device-memory and stream plumbing.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgm_hip.so")
+
+PGM_OK = 0
+PGM_ROW_OVERFLOW = 1
+PGM_NOT_CONVERGED = 3
+
+EXPORTS = (
+    "pgm_last_error",
+    "pgm_version",
+    "pgm_ctx_create",
+    "pgm_ctx_destroy",
+    "pgm_set_stream",
+    "pgm_set_box",
+    "pgm_build_rows",
+    "pgm_row_entries",
+    "pgm_perm_dipoles",
+    "pgm_field",
+    "pgm_pme_energy",
+    "pgm_compute",
+    "pgm_energy_forces",
+)
+
+
+class Topology(C.Structure):
+    """pgm_topology."""
+
+    _fields_ = [
+        ("n_atoms", C.c_int),
+        ("q", C.POINTER(C.c_double)),
+        ("radius", C.POINTER(C.c_double)),
+        ("alpha", C.POINTER(C.c_double)),
+        ("lj_rmin_half", C.POINTER(C.c_double)),
+        ("lj_sqrt_eps", C.POINTER(C.c_double)),
+        ("mol", C.POINTER(C.c_int)),
+        ("n_cov", C.c_int),
+        ("cov_i", C.POINTER(C.c_int)),
+        ("cov_j", C.POINTER(C.c_int)),
+        ("cov_c", C.POINTER(C.c_double)),
+    ]
+
+
+class Settings(C.Structure):
+    """pgm_settings."""
+
+    _fields_ = [
+        ("cutoff", C.c_double),
+        ("skin", C.c_double),
+        ("ewald_beta", C.c_double),
+        ("grid", C.c_int * 3),
+        ("order", C.c_int),
+        ("lj_lrc", C.c_int),
+        ("dipole_tol", C.c_double),
+        ("max_iter", C.c_int),
+        ("predictor", C.c_int),
+        ("fused", C.c_int),
+        ("norm_refresh", C.c_int),
+        ("peek", C.c_double),
+        ("precision", C.c_int),
+    ]
+
+
+class Induction(C.Structure):
+    """pgm_induction."""
+
+    _fields_ = [("mu", C.c_void_p), ("hist", C.c_void_p), ("count", C.c_int), ("norm", C.c_double)]
+
+
+class Energy(C.Structure):
+    """pgm_energy."""
+
+    _fields_ = [("elec", C.c_double), ("vdw", C.c_double), ("total", C.c_double)]
+
+
+class PGMError(RuntimeError):
+    """An error status of the HIP library."""
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Return the loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PGMError(f"{LIB_PATH} not found: build it first (__graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    L.pgm_last_error.restype = C.c_char_p
+    L.pgm_version.restype = I
+    L.pgm_ctx_create.argtypes = [C.POINTER(P), C.POINTER(Topology), C.POINTER(Settings), C.POINTER(D), P]
+    L.pgm_ctx_destroy.argtypes = [P]
+    L.pgm_set_stream.argtypes = [P, P]
+    L.pgm_set_box.argtypes = [P, C.POINTER(D)]
+    L.pgm_build_rows.argtypes = [P, P]
+    L.pgm_row_entries.argtypes = [P]
+    L.pgm_row_entries.restype = C.c_longlong
+    L.pgm_perm_dipoles.argtypes = [P, P, P]
+    L.pgm_field.argtypes = [P, P, I, P, P]
+    L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
+    L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
+    L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
+    for name in EXPORTS:
+        if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries"):
+            getattr(L, name).restype = I
+    _lib = L
+    return L
+
+
+def check(status: int, what: str) -> int:
+    """Raise PGMError for a negative status; return the status otherwise."""
+    if status < 0:
+        raise PGMError(f"{what} failed ({status}): {lib().pgm_last_error().decode(errors='replace')}")
+    return status

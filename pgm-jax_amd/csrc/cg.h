@@ -1,0 +1,394 @@
+// cg.h -- induced-dipole conjugate gradients on the device, covalent dipole frames, force and
+// energy assembly.
+//
+// CG (pgm_jax/md/forcefield.py:1714-1821): Jacobi preconditioner z = alpha r, flexible
+// Polak-Ribiere beta, converged when max|alpha r| / norm <= tol or after max_iter iterations, then
+// the peek step mu = x + peek alpha r.  Vectors are real4 (x, y, z, 0) in the compute precision;
+// every inner product and the residual maximum are FP64 block partials, reduced in a fixed order
+// by every block of the next kernel (deterministic, no atomics).  Iteration k's kernels run only
+// while k < *conv_it; the update kernel of iteration k sets *conv_it = k + 1 when it converges, so
+// the host can queue iterations ahead of the convergence test.
+#pragma once
+
+#include "common.h"
+
+namespace pgm {
+
+struct CGState {
+    int conv_it;  // first iteration index that must not run (INT_MAX while unconverged)
+    int pad;
+    double norm;  // mean |alpha b| (convergence normaliser)
+};
+
+// Sum of np partials (stride `stride`, component c) by the whole block, fixed order; result in all threads.
+PGM_DEV double block_reduce_partials(const double *__restrict__ part, int np, int stride, int c, double *lds, bool is_max) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < np; b += blockDim.x) {
+        const double x = part[b * stride + c];
+        v = is_max ? fmax(v, x) : v + x;
+    }
+    double r;
+    if (is_max) {
+        r = block_max(v, lds);
+    } else {
+        double a[1] = {v};
+        block_sum<1>(a, lds);
+        r = a[0];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) lds[31] = r;
+    __syncthreads();
+    r = lds[31];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------- small vector kernels
+// x0 = sum_k c_k hist[k] (predictor polynomial over the converged dipoles, newest first).
+template <typename real>
+__global__ void k_predict(int n, const double *__restrict__ hist, double c0, double c1, double c2, double c3,
+                          real4<real> *__restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t s = (size_t)n * 3;
+    real v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)3 * i + c;
+        v[c] = (real)(c0 * hist[o] + c1 * hist[s + o] + c2 * hist[2 * s + o] + c3 * hist[3 * s + o]);
+    }
+    real4<real> r;
+    r.x = v[0];
+    r.y = v[1];
+    r.z = v[2];
+    r.w = 0;
+    x0[i] = r;
+}
+
+// dq = (p + x0, q or 0): the dipoles and charges a field evaluation spreads.
+template <typename real>
+__global__ void k_pack_dq(int n, const double *__restrict__ p, const real4<real> *__restrict__ x0, const double *__restrict__ q,
+                          real4<real> *__restrict__ dq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    real4<real> r;
+    r.x = (real)p[3 * i];
+    r.y = (real)p[3 * i + 1];
+    r.z = (real)p[3 * i + 2];
+    if (x0) {
+        const real4<real> a = x0[i];
+        r.x += a.x;
+        r.y += a.y;
+        r.z += a.z;
+    }
+    r.w = q ? (real)q[i] : real(0);
+    dq[i] = r;
+}
+
+// x = alpha * b
+template <typename real>
+__global__ void k_scale_alpha(int n, const real4<real> *__restrict__ b, const real *__restrict__ alpha, real4<real> *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const real a = alpha[i];
+    real4<real> v = b[i];
+    v.x *= a;
+    v.y *= a;
+    v.z *= a;
+    v.w = 0;
+    x[i] = v;
+}
+
+// real4 (N) -> double (N, 3)
+template <typename real>
+__global__ void k_to_double3(int n, const real4<real> *__restrict__ v, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const real4<real> a = v[i];
+    out[3 * i] = a.x;
+    out[3 * i + 1] = a.y;
+    out[3 * i + 2] = a.z;
+}
+
+// norm = sum(partials) / (3 N) + 1e-300 (mean |alpha b|)
+__global__ void k_set_norm(const double *__restrict__ part, int np, double inv3n, CGState *st) {
+    __shared__ double lds[32];
+    const double s = block_reduce_partials(part, np, 1, 0, lds, false);
+    if (threadIdx.x == 0) st->norm = s * inv3n + 1e-300;
+}
+
+__global__ void k_set_norm_value(CGState *st, double norm) {
+    if (threadIdx.x == 0) st->norm = norm;
+}
+
+// ---------------------------------------------------------------------------- CG
+// Start: z = alpha r, p = z, partials (r . z, max |alpha r|).
+template <typename real>
+__global__ __launch_bounds__(256) void k_cg_start(int n, const real4<real> *__restrict__ r, const real *__restrict__ alpha,
+                                                  real4<real> *__restrict__ z, real4<real> *__restrict__ p,
+                                                  double *__restrict__ part) {
+    __shared__ double lds[32];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[1] = {0.0};
+    double m = 0.0;
+    if (i < n) {
+        const real a = alpha[i];
+        const real4<real> rr = r[i];
+        real4<real> zz;
+        zz.x = rr.x * a;
+        zz.y = rr.y * a;
+        zz.z = rr.z * a;
+        zz.w = 0;
+        z[i] = zz;
+        p[i] = zz;
+        s[0] = (double)(rr.x * zz.x) + (double)(rr.y * zz.y) + (double)(rr.z * zz.z);
+        m = fmax(fmax(fabs((double)zz.x), fabs((double)zz.y)), fabs((double)zz.z));
+    }
+    block_sum<1>(s, lds);
+    m = block_max(m, lds + 16);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s[0];
+        part[2 * blockIdx.x + 1] = m;
+    }
+}
+
+// rz[0], err[0], conv_it (one block).
+__global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGState *st, double *__restrict__ rz,
+                                  double *__restrict__ err, double tol, int max_iter) {
+    __shared__ double lds[32];
+    const double s = block_reduce_partials(part, np, 2, 0, lds, false);
+    const double m = block_reduce_partials(part, np, 2, 1, lds, true);
+    if (threadIdx.x == 0) {
+        rz[0] = s;
+        const double e = m / st->norm;
+        err[0] = e;
+        st->conv_it = (e > tol && 0 < max_iter) ? 0x7fffffff : 0;
+    }
+}
+
+// Iteration k, first half: al = rz_k / max(p . Ap, 1e-300) (rounded to real); x += al p;
+// r' = r - al Ap; z' = alpha r'; partials (z' . (r' - r), r' . z', max |alpha r'|).
+template <typename real>
+__global__ __launch_bounds__(256) void k_cg_update1(int n, const double *__restrict__ pap_part, int np,
+                                                    const double *__restrict__ rz, int k, real4<real> *__restrict__ x,
+                                                    real4<real> *__restrict__ r, real4<real> *__restrict__ z,
+                                                    const real4<real> *__restrict__ p, const real4<real> *__restrict__ Ap,
+                                                    const real *__restrict__ alpha, double *__restrict__ part,
+                                                    const CGState *__restrict__ st) {
+    if (k >= st->conv_it) return;
+    __shared__ double lds[32];
+    const double pap = block_reduce_partials(pap_part, np, 1, 0, lds, false);
+    const real al = (real)(rz[k] / fmax(pap, 1e-300));
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[2] = {0.0, 0.0};
+    double m = 0.0;
+    if (i < n) {
+        const real4<real> pp = p[i], ap = Ap[i], r0 = r[i];
+        real4<real> xx = x[i];
+        xx.x += al * pp.x;
+        xx.y += al * pp.y;
+        xx.z += al * pp.z;
+        x[i] = xx;
+        real4<real> r1;
+        r1.x = r0.x - al * ap.x;
+        r1.y = r0.y - al * ap.y;
+        r1.z = r0.z - al * ap.z;
+        r1.w = 0;
+        r[i] = r1;
+        const real a = alpha[i];
+        real4<real> z1;
+        z1.x = r1.x * a;
+        z1.y = r1.y * a;
+        z1.z = r1.z * a;
+        z1.w = 0;
+        z[i] = z1;
+        s[0] = (double)(z1.x * (r1.x - r0.x)) + (double)(z1.y * (r1.y - r0.y)) + (double)(z1.z * (r1.z - r0.z));
+        s[1] = (double)(r1.x * z1.x) + (double)(r1.y * z1.y) + (double)(r1.z * z1.z);
+        m = fmax(fmax(fabs((double)z1.x), fabs((double)z1.y)), fabs((double)z1.z));
+    }
+    block_sum<2>(s, lds);
+    m = block_max(m, lds + 16);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s[0];
+        part[3 * blockIdx.x + 1] = s[1];
+        part[3 * blockIdx.x + 2] = m;
+    }
+}
+
+// Iteration k, second half: beta = z'.(r' - r) / max(rz_k, 1e-300) (rounded to real); p = z' + beta p.
+// Block 0 records rz[k+1], err[k+1] and, when converged or out of iterations, conv_it = k + 1.
+template <typename real>
+__global__ __launch_bounds__(256) void k_cg_update2(int n, const double *__restrict__ part, int np, double *__restrict__ rz,
+                                                    double *__restrict__ err, int k, const real4<real> *__restrict__ z,
+                                                    real4<real> *__restrict__ p, CGState *st, double tol, int max_iter) {
+    if (k >= st->conv_it) return;
+    __shared__ double lds[32];
+    const double s0 = block_reduce_partials(part, np, 3, 0, lds, false);
+    const double s1 = block_reduce_partials(part, np, 3, 1, lds, false);
+    const double mx = block_reduce_partials(part, np, 3, 2, lds, true);
+    const real beta = (real)(s0 / fmax(rz[k], 1e-300));
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const real4<real> zz = z[i];
+        real4<real> pp = p[i];
+        pp.x = zz.x + beta * pp.x;
+        pp.y = zz.y + beta * pp.y;
+        pp.z = zz.z + beta * pp.z;
+        pp.w = 0;
+        p[i] = pp;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rz[k + 1] = s1;
+        const double e = mx / st->norm;
+        err[k + 1] = e;
+        if (!(e > tol && k + 1 < max_iter)) st->conv_it = k + 1;
+    }
+}
+
+// mu = x + peek alpha r (FP64 out).
+template <typename real>
+__global__ void k_cg_finish(int n, const real4<real> *__restrict__ x, const real4<real> *__restrict__ r,
+                            const real *__restrict__ alpha, double peek, double *__restrict__ mu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const real4<real> xx = x[i], rr = r[i];
+    const real a = alpha[i];
+    const real pk = (real)peek;
+    if (peek != 0.0) {
+        mu[3 * i] = (double)(xx.x + pk * rr.x * a);
+        mu[3 * i + 1] = (double)(xx.y + pk * rr.y * a);
+        mu[3 * i + 2] = (double)(xx.z + pk * rr.z * a);
+    } else {
+        mu[3 * i] = (double)xx.x;
+        mu[3 * i + 1] = (double)xx.y;
+        mu[3 * i + 2] = (double)xx.z;
+    }
+}
+
+// hist = [mu, hist[0], hist[1], hist[2]] (in place, newest first).
+__global__ void k_hist_push(int n3, const double *__restrict__ mu, double *__restrict__ hist) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n3) return;
+    const size_t s = (size_t)n3;
+    hist[3 * s + o] = hist[2 * s + o];
+    hist[2 * s + o] = hist[s + o];
+    hist[s + o] = hist[o];
+    hist[o] = mu[o];
+}
+
+// ---------------------------------------------------------------------------- covalent frames
+// p_i = sum over i's covalent dipoles m of c_m unit(min_image(r_j - r_i))  (forcefield.py:1032-1042)
+__global__ void k_perm_dipoles(int n, const double *__restrict__ pos, Box box, const int *__restrict__ cstart,
+                               const int *__restrict__ cj, const double *__restrict__ cc, double *__restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double px = 0, py = 0, pz = 0;
+    for (int m = cstart[i]; m < cstart[i + 1]; ++m) {
+        const int j = cj[m];
+        double vx = pos[3 * j] - pos[3 * i], vy = pos[3 * j + 1] - pos[3 * i + 1], vz = pos[3 * j + 2] - pos[3 * i + 2];
+        min_image(vx, vy, vz, box);
+        const double il = 1.0 / sqrt(vx * vx + vy * vy + vz * vz);
+        px += cc[m] * vx * il;
+        py += cc[m] * vy * il;
+        pz += cc[m] * vz * il;
+    }
+    p[3 * i] = px;
+    p[3 * i + 1] = py;
+    p[3 * i + 2] = pz;
+}
+
+// d = p + mu (FP64) and dq = (d, q) in real; partials (sum |d|^2, sum |mu|^2 / (2 alpha)).
+template <typename real>
+__global__ __launch_bounds__(256) void k_energy_pack(int n, const double *__restrict__ p, const double *__restrict__ mu,
+                                                     const double *__restrict__ q, const double *__restrict__ alpha,
+                                                     double *__restrict__ dd, real4<real> *__restrict__ dq,
+                                                     double *__restrict__ part) {
+    __shared__ double lds[32];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[2] = {0.0, 0.0};
+    if (i < n) {
+        double d[3];
+        double m2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double m = mu ? mu[3 * i + c] : 0.0;
+            d[c] = p[3 * i + c] + m;
+            dd[3 * i + c] = d[c];
+            m2 += m * m;
+        }
+        real4<real> v;
+        v.x = (real)d[0];
+        v.y = (real)d[1];
+        v.z = (real)d[2];
+        v.w = (real)q[i];
+        dq[i] = v;
+        s[0] = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        s[1] = m2 / (2.0 * alpha[i]);
+    }
+    block_sum<2>(s, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s[0];
+        part[2 * blockIdx.x + 1] = s[1];
+    }
+}
+
+// forces = -(KE g_el + g_lj + KE rec_dr + VJP_p(gp)); the pull-back through the covalent frames of
+// p_i = sum c unit(v), v = r_j - r_i: t_m = c_m (gp_i - (gp_i . u) u) / |v|, +t_m on j, -t_m on i,
+// gathered per atom from the covalent dipoles it owns (cstart) and those pointing at it (jstart).
+__global__ void k_forces_final(int n, const double *__restrict__ pos, Box box, const double *__restrict__ g_el,
+                               const double *__restrict__ g_lj, const double *__restrict__ rec_dr,
+                               const double *__restrict__ gp, const int *__restrict__ cstart, const int *__restrict__ ci,
+                               const int *__restrict__ cj, const double *__restrict__ cc, const int *__restrict__ jstart,
+                               const int *__restrict__ jlist, double *__restrict__ forces) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = KE * g_el[3 * a + c] + g_lj[3 * a + c] + KE * rec_dr[3 * a + c];
+    for (int pass = 0; pass < 2; ++pass) {
+        const int m0 = pass == 0 ? cstart[a] : jstart[a];
+        const int m1 = pass == 0 ? cstart[a + 1] : jstart[a + 1];
+        for (int s = m0; s < m1; ++s) {
+            const int m = pass == 0 ? s : jlist[s];
+            const int i = ci[m], j = cj[m];
+            double ux = pos[3 * j] - pos[3 * i], uy = pos[3 * j + 1] - pos[3 * i + 1], uz = pos[3 * j + 2] - pos[3 * i + 2];
+            min_image(ux, uy, uz, box);
+            const double L = sqrt(ux * ux + uy * uy + uz * uz);
+            ux /= L;
+            uy /= L;
+            uz /= L;
+            const double gx = gp[3 * i], gy = gp[3 * i + 1], gz = gp[3 * i + 2];
+            const double gu = gx * ux + gy * uy + gz * uz;
+            const double sgn = pass == 0 ? -1.0 : 1.0;
+            const double f = sgn * cc[m] / L;
+            v[0] += f * (gx - gu * ux);
+            v[1] += f * (gy - gu * uy);
+            v[2] += f * (gz - gu * uz);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) forces[3 * a + c] = -v[c];
+}
+
+// Energy assembly (one block): parts = {pair rows (e_el, e_lj) x npair, (sum|d|^2, u_pol) x npack,
+// u_rec x nconv}; consts = {KE * (u_self_q + u_bg), e_tail, c_self}.
+__global__ void k_energy_final(const double *__restrict__ pair_part, int npair, const double *__restrict__ pack_part, int npack,
+                               const double *__restrict__ conv_part, int nconv, double ke_const, double tail, double cself,
+                               double *__restrict__ out) {
+    __shared__ double lds[32];
+    const double e_el = block_reduce_partials(pair_part, npair, 2, 0, lds, false);
+    const double e_lj = block_reduce_partials(pair_part, npair, 2, 1, lds, false);
+    const double d2 = block_reduce_partials(pack_part, npack, 2, 0, lds, false);
+    const double upol = block_reduce_partials(pack_part, npack, 2, 1, lds, false);
+    const double urec = block_reduce_partials(conv_part, nconv, 1, 0, lds, false);
+    if (threadIdx.x == 0) {
+        const double elec = KE * 0.5 * e_el + KE * (urec - 0.5 * cself * d2 + upol) + ke_const;
+        const double vdw = 0.5 * e_lj + tail;
+        out[0] = elec;
+        out[1] = vdw;
+        out[2] = elec + vdw;
+        out[3] = urec;
+    }
+}
+
+}  // namespace pgm

@@ -1,0 +1,229 @@
+// common.h -- shared device helpers of libpgm_hip: precision traits, box geometry, wave/block
+// reductions and the Gaussian-screened radial kernels of the pGM pair terms.
+//
+// gfx950 (CDNA4): wave64.  Every kernel is templated on `real` (float: mixed precision, double:
+// the plain FP64 path); positions, minimum images and all reductions are FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PGM_DEV __device__ __forceinline__
+
+namespace pgm {
+
+constexpr int WAVE = 64;
+constexpr double KE = 138.935458;                 // Coulomb constant, kJ/mol nm / e^2 (CODATA)
+constexpr double TWO_OVER_SQRTPI = 1.1283791670955126;
+constexpr double SQRT_PI = 1.7724538509055159;
+constexpr int MAX_ORDER = 8;
+
+template <typename real> struct vec4;
+template <> struct vec4<float> { using type = float4; };
+template <> struct vec4<double> { using type = double4; };
+template <typename real> using real4 = typename vec4<real>::type;
+
+// Reduced lower-triangular box: lattice vectors as rows, h[3a+b].
+struct Box {
+    double h[9];
+    double hinv[9];   // H^-1 (lower triangular)
+    double inv_diag[3];
+};
+
+// Minimum image of a reduced box: sequential reduction along c, b, a (nearest integers).
+PGM_DEV void min_image(double &x, double &y, double &z, const Box &b) {
+    double n = rint(z * b.inv_diag[2]);
+    x -= n * b.h[6];
+    y -= n * b.h[7];
+    z -= n * b.h[8];
+    n = rint(y * b.inv_diag[1]);
+    x -= n * b.h[3];
+    y -= n * b.h[4];
+    n = rint(x * b.inv_diag[0]);
+    x -= n * b.h[0];
+}
+
+// ---------------------------------------------------------------------------- reductions
+template <typename T> PGM_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+template <typename T> PGM_DEV T wave_max(T v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+// Sum over groups of G consecutive lanes (G a power of two <= 64); every lane gets its group's sum.
+template <int G, typename T> PGM_DEV T group_sum(T v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Block sum of NV doubles; result valid in thread 0.  blockDim.x a multiple of 64, <= 1024.
+template <int NV> PGM_DEV void block_sum(double (&v)[NV], double *lds /* NV * 16 */) {
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < NV; ++k) lds[k * 16 + w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < nw; ++i) s += lds[k * 16 + i];
+            v[k] = s;
+        }
+}
+PGM_DEV double block_max(double v, double *lds) {
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    v = wave_max(v);
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < nw; ++i) m = fmax(m, lds[i]);
+        v = m;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------- radial kernels
+// B_n of phi_a(r) = erf(a r)/r: B0 = phi, B_n = -(1/r) dB_{n-1}/dr (grad_x B_n = -B_{n+1} x).
+// Series for s = (a r)^2 < 1: B_n = (2a/sqrt(pi)) (2a^2)^n sum_{m<15} (-s)^m / (m! (2m+2n+1)).
+// Closed form: B_n = [(2n-1) B_{n-1} - (2a^2)^(n-1) c] / r^2, c = (2a/sqrt(pi)) exp(-a^2 r^2).
+// For a Gaussian pair G_n = B_n[a] - B_n[b0]; when both screenings are on the closed-form branch
+// the difference is formed from erfc (G0 = (erfc(b0 r) - erfc(a r))/r), which keeps float32
+// accurate where erf(a r)/r and erf(b0 r)/r nearly cancel.
+PGM_DEV float erfc_(float x) { return erfcf(x); }
+PGM_DEV double erfc_(double x) { return erfc(x); }
+PGM_DEV float erf_(float x) { return erff(x); }
+PGM_DEV double erf_(double x) { return erf(x); }
+PGM_DEV float exp_(float x) { return expf(x); }
+PGM_DEV double exp_(double x) { return exp(x); }
+PGM_DEV float sqrt_(float x) { return sqrtf(x); }
+PGM_DEV double sqrt_(double x) { return sqrt(x); }
+PGM_DEV float rsqrt_(float x) { return rsqrtf(x); }
+PGM_DEV double rsqrt_(double x) { return 1.0 / sqrt(x); }
+
+template <typename real, int NMAX> PGM_DEV void series_B(real a, real s, real *B) {
+    real S[NMAX];
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) S[n] = real(0);
+    real t = real(1);
+#pragma unroll
+    for (int m = 0; m < 15; ++m) {
+#pragma unroll
+        for (int n = 0; n < NMAX; ++n) S[n] += t * real(1.0 / double(2 * m + 2 * n + 1));
+        t *= -s * real(1.0 / double(m + 1));
+    }
+    real c = real(TWO_OVER_SQRTPI) * a;
+    const real f = real(2) * a * a;
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) {
+        B[n] = c * S[n];
+        c *= f;
+    }
+}
+
+template <typename real, int NMAX> PGM_DEV void closed_B(real a, real r, real ir, real ir2, real *B) {
+    const real c = real(TWO_OVER_SQRTPI) * a * exp_(-(a * r) * (a * r));
+    const real f = real(2) * a * a;
+    B[0] = erf_(a * r) * ir;
+    real fc = c;
+#pragma unroll
+    for (int n = 1; n < NMAX; ++n) {
+        B[n] = (real(2 * n - 1) * B[n - 1] - fc) * ir2;
+        fc *= f;
+    }
+}
+
+template <typename real, int NMAX> PGM_DEV void one_B(real a, real r, real ir, real ir2, real *B) {
+    const real s = (a * r) * (a * r);
+    if (s < real(1))
+        series_B<real, NMAX>(a, s, B);
+    else
+        closed_B<real, NMAX>(a, r, ir, ir2, B);
+}
+
+// G_n = B_n[erf(a r)/r] - B_n[erf(b r)/r], n < NMAX.
+template <typename real, int NMAX> PGM_DEV void pair_G(real a, real b, real r2, real *G) {
+    const real ir = rsqrt_(r2);
+    const real r = r2 * ir;
+    const real ir2 = ir * ir;
+    const real sa = a * a * r2, sb = b * b * r2;
+    if (sa >= real(1) && sb >= real(1)) {
+        const real ca = real(TWO_OVER_SQRTPI) * a * exp_(-sa);
+        const real cb = real(TWO_OVER_SQRTPI) * b * exp_(-sb);
+        G[0] = (erfc_(b * r) - erfc_(a * r)) * ir;
+        const real fa = real(2) * a * a, fb = real(2) * b * b;
+        real pa = ca, pb = cb;
+#pragma unroll
+        for (int n = 1; n < NMAX; ++n) {
+            G[n] = (real(2 * n - 1) * G[n - 1] - (pa - pb)) * ir2;
+            pa *= fa;
+            pb *= fb;
+        }
+    } else {
+        real Ba[NMAX], Bb[NMAX];
+        one_B<real, NMAX>(a, r, ir, ir2, Ba);
+        one_B<real, NMAX>(b, r, ir, ir2, Bb);
+#pragma unroll
+        for (int n = 0; n < NMAX; ++n) G[n] = Ba[n] - Bb[n];
+    }
+}
+
+// ---------------------------------------------------------------------------- B-splines
+// Cardinal B-spline weights of order P at offset f in [0, 1): th[k] = M_P(f + P - 1 - k), the
+// weight of grid point floor(w) + k, and their derivatives d/df (recursion raised one order at a
+// time, derivatives from the order P-1 values).  With D2, also the second derivatives (order P-2).
+template <typename real, int P, bool D2>
+PGM_DEV void bspline(real f, real *th, real *dth, real *d2th) {
+    real d[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) d[k] = real(0);
+    d[0] = real(1) - f;
+    d[1] = f;
+    real m2[P], m1[P];
+#pragma unroll
+    for (int j = 3; j <= P; ++j) {
+        if (j == P - 1) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) m2[k] = d[k];
+        }
+        if (j == P) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) m1[k] = d[k];
+        }
+        const real div = real(1.0 / double(j - 1));
+        d[j - 1] = div * f * d[j - 2];
+#pragma unroll
+        for (int k = 1; k < j - 1; ++k) d[j - k - 1] = div * ((f + real(k)) * d[j - k - 2] + (real(j - k) - f) * d[j - k - 1]);
+        d[0] = div * (real(1) - f) * d[0];
+    }
+    if (P == 3) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) m1[k] = real(0);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        th[k] = d[k];
+        const real a = k >= 1 ? m1[k - 1] : real(0);
+        const real b = k < P - 1 ? m1[k] : real(0);
+        dth[k] = a - b;
+    }
+    if (D2) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const real a = k >= 2 ? m2[k - 2] : real(0);
+            const real b = (k >= 1 && k - 1 < P - 2) ? m2[k - 1] : real(0);
+            const real c = k < P - 2 ? m2[k] : real(0);
+            d2th[k] = a - real(2) * b + c;
+        }
+    }
+}
+
+}  // namespace pgm

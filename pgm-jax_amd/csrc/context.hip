@@ -1,0 +1,811 @@
+// context.hip -- the C ABI of libpgm_hip (include/pgm_hip.h): context, rows, PME plans, the
+// induced-dipole solve and the energy/force pass, dispatched to the float (mixed) or double
+// instantiation of the kernels.
+#include <hipcub/hipcub.hpp>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cg.h"
+#include "common.h"
+#include "pairs.h"
+#include "pme.h"
+#include "pgm_hip.h"
+
+using namespace pgm;
+
+static thread_local std::string g_err;
+
+#define HIP_TRY(x)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                        \
+            return PGM_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+#define FFT_TRY(x)                                                                         \
+    do {                                                                                   \
+        rocfft_status s_ = (x);                                                            \
+        if (s_ != rocfft_status_success) {                                                 \
+            g_err = std::string(#x) + ": rocfft status " + std::to_string((int)s_);        \
+            return PGM_ERR_FFT;                                                            \
+        }                                                                                  \
+    } while (0)
+#define PGM_TRY(x)                                                                         \
+    do {                                                                                   \
+        int r_ = (x);                                                                      \
+        if (r_ < 0) return r_;                                                             \
+    } while (0)
+#define LAUNCH_CHECK()                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess) {                                                            \
+            g_err = std::string("kernel launch: ") + hipGetErrorString(e_) + " at line " + \
+                    std::to_string(__LINE__);                                              \
+            return PGM_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b) {
+        if (b <= bytes && p) return PGM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (b == 0) return PGM_OK;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            g_err = std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e);
+            return PGM_ERR_HIP;
+        }
+        bytes = b;
+        return PGM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+struct pgm_ctx {
+    int n = 0;
+    pgm_settings s{};
+    hipStream_t stream = nullptr;
+    bool mixed = false;
+    size_t rsize = 8;  // sizeof(real)
+    // box
+    double H[9]{};
+    double volume = 0;
+    Box box{};
+    PMEGeom pg{};
+    int3 nc{1, 1, 1};
+    // topology (host copies and device arrays)
+    std::vector<double> h_q, h_R2, h_alpha, h_rh, h_se;
+    std::vector<int> h_mol;
+    double sum_q = 0, sum_q2 = 0;
+    DevBuf q, alpha64, alpha_r, rh, se, mol, R2;
+    DevBuf cstart, ci, cj, cc, jstart, jlist;
+    // geometry: the positions of the current evaluation (caller's device buffer) and xr
+    const double *pos = nullptr;
+    DevBuf xr;
+    // rows
+    int cap_full = 0, cap = 0;
+    bool rows_valid = false;
+    DevBuf rows_full, count_full, rows, count, maxcount;
+    DevBuf key, atom, skey, satom, cell_start, cell_end, sort_tmp;
+    long long entries = 0;
+    // PME
+    size_t ngrid = 0, ncplx = 0;
+    DevBuf grid, cgrid, Gc, fft_work;
+    rocfft_plan fwd = nullptr, bwd = nullptr;
+    rocfft_execution_info info = nullptr;
+    // vectors (real4 N) and FP64 (N, 3) work arrays
+    DevBuf vx, vr, vz, vp, vAp, vb, vx0, vrow, vdq, vtmp;
+    DevBuf pperm, dd, g_el, g_lj, rowf, rec_dr, gp;
+    DevBuf partA, partB, partC, pair_part, pack_part, conv_part;
+    DevBuf st, rz, err, energy;
+    int *h_pin_int = nullptr;
+    double *h_pin_dbl = nullptr;
+    int last_iters = 4;
+};
+
+extern "C" const char *pgm_last_error(void) { return g_err.c_str(); }
+extern "C" int pgm_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------- host helpers
+static void host_bspline(double f, int P, double *th) {
+    std::vector<double> d(P, 0.0);
+    d[0] = 1.0 - f;
+    d[1] = f;
+    for (int j = 3; j <= P; ++j) {
+        const double div = 1.0 / (j - 1.0);
+        d[j - 1] = div * f * d[j - 2];
+        for (int k = 1; k < j - 1; ++k) d[j - k - 1] = div * ((f + k) * d[j - k - 2] + (j - k - f) * d[j - k - 1]);
+        d[0] = div * (1.0 - f) * d[0];
+    }
+    for (int k = 0; k < P; ++k) th[k] = d[k];
+}
+
+// |b(m)|^-2 denominators (pgm_jax/md/pme.py:106-131)
+static std::vector<double> moduli(int K, int P) {
+    std::vector<double> th(P), mod(K);
+    host_bspline(0.0, P, th.data());
+    for (int m = 0; m < K; ++m) {
+        double c = 0, s = 0;
+        for (int j = 0; j < P; ++j) {
+            const double a = 2.0 * M_PI * m * j / K;
+            c += th[j] * std::cos(a);
+            s += th[j] * std::sin(a);
+        }
+        mod[m] = c * c + s * s;
+    }
+    for (int m = 0; m < K; ++m)
+        if (mod[m] < 1e-7) mod[m] = 0.5 * (mod[(m - 1 + K) % K] + mod[(m + 1) % K]);
+    return mod;
+}
+
+static void inv3(const double *H, double *Hi) {
+    const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7], i = H[8];
+    const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    Hi[0] = (e * i - f * h) / det;
+    Hi[1] = (c * h - b * i) / det;
+    Hi[2] = (b * f - c * e) / det;
+    Hi[3] = (f * g - d * i) / det;
+    Hi[4] = (a * i - c * g) / det;
+    Hi[5] = (c * d - a * f) / det;
+    Hi[6] = (d * h - e * g) / det;
+    Hi[7] = (b * g - a * h) / det;
+    Hi[8] = (a * e - b * d) / det;
+}
+
+static double lj_tail(const pgm_ctx *c) {
+    // -2 pi / (3 V rc^3) sum_{i,j} 2 eps_ij rmin_ij^6 by moments (pgm_jax/lj.py:87-114)
+    double M[7] = {0};
+    for (int i = 0; i < c->n; ++i)
+        for (int k = 0; k < 7; ++k) M[k] += c->h_se[i] * std::pow(c->h_rh[i], k);
+    static const double C6[7] = {1, 6, 15, 20, 15, 6, 1};
+    double SB = 0;
+    for (int k = 0; k < 7; ++k) SB += C6[k] * M[k] * M[6 - k];
+    SB *= 2.0;
+    return -2.0 * M_PI * SB / (3.0 * c->volume * std::pow(c->s.cutoff, 3));
+}
+
+template <typename T> static int upload(DevBuf &b, const T *h, size_t count, hipStream_t st) {
+    PGM_TRY(b.ensure(std::max<size_t>(count, 1) * sizeof(T)));
+    if (count) HIP_TRY(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- box
+static int set_box(pgm_ctx *c, const double *H) {
+    for (int k = 0; k < 9; ++k) c->H[k] = H[k];
+    if (std::fabs(H[1]) + std::fabs(H[2]) + std::fabs(H[5]) > 1e-12 * std::fabs(H[0])) {
+        g_err = "box must be lower triangular (lattice vectors as rows)";
+        return PGM_ERR_ARG;
+    }
+    double Hi[9];
+    inv3(H, Hi);
+    for (int k = 0; k < 9; ++k) c->box.h[k] = H[k];
+    for (int k = 0; k < 9; ++k) c->box.hinv[k] = Hi[k];
+    for (int a = 0; a < 3; ++a) c->box.inv_diag[a] = 1.0 / H[4 * a];
+    c->volume = std::fabs(H[0] * H[4] * H[8]);
+    for (int k = 0; k < 9; ++k) c->pg.hinv[k] = Hi[k];
+    for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) c->pg.escale[3 * b + a] = Hi[3 * b + a] * c->s.grid[a];
+    for (int a = 0; a < 3; ++a) c->pg.K[a] = c->s.grid[a];
+    // cell grid: plane heights / (cutoff + skin)
+    const double rl = c->s.cutoff + c->s.skin;
+    double ax[3] = {H[0], H[1], H[2]}, bx[3] = {H[3], H[4], H[5]}, cx[3] = {H[6], H[7], H[8]};
+    auto cross = [](const double *u, const double *v, double *w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double w[3];
+    int ncs[3];
+    const double *vec[3][2] = {{bx, cx}, {cx, ax}, {ax, bx}};
+    for (int a = 0; a < 3; ++a) {
+        cross(vec[a][0], vec[a][1], w);
+        const double h = c->volume / std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (2.0 * rl > h * (1.0 + 1e-12) && rl > 0.5 * h) {
+            g_err = "cutoff + skin exceeds half the box height";
+            return PGM_ERR_ARG;
+        }
+        ncs[a] = std::max(1, (int)std::floor(h / rl));
+    }
+    c->nc = make_int3(ncs[0], ncs[1], ncs[2]);
+    // influence function Gc = 2 G_raw B(m) on the half spectrum (pgm_jax/md/pme.py:243-267)
+    const int K1 = c->s.grid[0], K2 = c->s.grid[1], K3 = c->s.grid[2], K3h = K3 / 2 + 1;
+    std::vector<double> m1 = moduli(K1, c->s.order), m2 = moduli(K2, c->s.order), m3 = moduli(K3, c->s.order);
+    const double R[9] = {Hi[0], Hi[3], Hi[6], Hi[1], Hi[4], Hi[7], Hi[2], Hi[5], Hi[8]};  // rows: reciprocal vectors (H^-T)
+    const double beta = c->s.ewald_beta;
+    std::vector<double> G((size_t)K1 * K2 * K3h);
+    for (int a = 0; a < K1; ++a) {
+        const double fa = a <= (K1 - 1) / 2 ? a : a - K1;  // fftfreq
+        for (int b = 0; b < K2; ++b) {
+            const double fb = b <= (K2 - 1) / 2 ? b : b - K2;
+            for (int k = 0; k < K3h; ++k) {
+                const double mx = fa * R[0] + fb * R[3] + k * R[6];
+                const double my = fa * R[1] + fb * R[4] + k * R[7];
+                const double mz = fa * R[2] + fb * R[5] + k * R[8];
+                const double msq = mx * mx + my * my + mz * mz;
+                double g = 0.0;
+                if (msq > 0) g = std::exp(-M_PI * M_PI * msq / (beta * beta)) / (2.0 * M_PI * c->volume * msq);
+                G[((size_t)a * K2 + b) * K3h + k] = 2.0 * g / (m1[a] * m2[b] * m3[k]);
+            }
+        }
+    }
+    if (c->mixed) {
+        std::vector<float> Gf(G.begin(), G.end());
+        PGM_TRY(upload(c->Gc, Gf.data(), Gf.size(), c->stream));
+    } else {
+        PGM_TRY(upload(c->Gc, G.data(), G.size(), c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->rows_valid = false;
+    return PGM_OK;
+}
+
+static int make_plans(pgm_ctx *c) {
+    static bool setup_done = false;
+    if (!setup_done) {
+        FFT_TRY(rocfft_setup());
+        setup_done = true;
+    }
+    const size_t len[3] = {(size_t)c->s.grid[2], (size_t)c->s.grid[1], (size_t)c->s.grid[0]};
+    const rocfft_precision prec = c->mixed ? rocfft_precision_single : rocfft_precision_double;
+    FFT_TRY(rocfft_plan_create(&c->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3, len, 1, nullptr));
+    FFT_TRY(rocfft_plan_create(&c->bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3, len, 1, nullptr));
+    size_t w1 = 0, w2 = 0;
+    FFT_TRY(rocfft_plan_get_work_buffer_size(c->fwd, &w1));
+    FFT_TRY(rocfft_plan_get_work_buffer_size(c->bwd, &w2));
+    PGM_TRY(c->fft_work.ensure(std::max(w1, w2)));
+    FFT_TRY(rocfft_execution_info_create(&c->info));
+    if (c->fft_work.bytes) FFT_TRY(rocfft_execution_info_set_work_buffer(c->info, c->fft_work.p, c->fft_work.bytes));
+    FFT_TRY(rocfft_execution_info_set_stream(c->info, c->stream));
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- rows
+__global__ void k_pack_xr(int n, const double *__restrict__ pos, const double *__restrict__ R2, double4 *__restrict__ xr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    xr[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], R2[i]);
+}
+
+static int alloc_rows(pgm_ctx *c, int cap) {
+    c->cap_full = cap;
+    c->cap = cap;
+    PGM_TRY(c->rows_full.ensure((size_t)c->n * cap * sizeof(int)));
+    PGM_TRY(c->rows.ensure((size_t)c->n * cap * sizeof(int)));
+    return PGM_OK;
+}
+
+static int build_rows(pgm_ctx *c, const double *pos) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const int ncell = c->nc.x * c->nc.y * c->nc.z;
+    PGM_TRY(c->cell_start.ensure(sizeof(int) * ncell));
+    PGM_TRY(c->cell_end.ensure(sizeof(int) * ncell));
+    k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->box, c->nc, c->key.as<int>(), c->atom.as<int>());
+    LAUNCH_CHECK();
+    int bits = 1;
+    while ((1 << bits) < ncell) ++bits;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->key.as<int>(), c->skey.as<int>(), c->atom.as<int>(),
+                                               c->satom.as<int>(), n, 0, bits, st));
+    PGM_TRY(c->sort_tmp.ensure(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp.p, tmp, c->key.as<int>(), c->skey.as<int>(), c->atom.as<int>(),
+                                               c->satom.as<int>(), n, 0, bits, st));
+    k_fill_int<<<nblk(ncell, 256), 256, 0, st>>>(ncell, c->cell_start.as<int>(), 0);
+    k_fill_int<<<nblk(ncell, 256), 256, 0, st>>>(ncell, c->cell_end.as<int>(), 0);
+    k_cell_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->skey.as<int>(), c->cell_start.as<int>(), c->cell_end.as<int>());
+    LAUNCH_CHECK();
+    const double rl = c->s.cutoff + c->s.skin;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        k_fill_int<<<1, 64, 0, st>>>(1, c->maxcount.as<int>(), 0);
+        k_build_rows<<<nblk(n, 128), 128, 0, st>>>(n, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
+                                                   c->cell_end.as<int>(), rl * rl, c->cap_full, c->rows_full.as<int>(),
+                                                   c->count_full.as<int>(), c->maxcount.as<int>());
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_int, c->maxcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const int mc = c->h_pin_int[0];
+        if (mc <= c->cap_full) {
+            c->rows_valid = true;
+            return PGM_OK;
+        }
+        PGM_TRY(alloc_rows(c, (int)(mc * 1.2) + 16));
+    }
+    g_err = "row capacity could not be sized";
+    return PGM_ERR_STATE;
+}
+
+// Pack positions and compact the rows to the cutoff (every force evaluation).
+static int prepare(pgm_ctx *c, const double *pos) {
+    if (!c->rows_valid) PGM_TRY(build_rows(c, pos));
+    c->pos = pos;
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    k_pack_xr<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->R2.as<double>(), c->xr.as<double4>());
+    k_compact_rows<<<nblk(n, 4), 256, 0, st>>>(n, c->xr.as<double4>(), c->box, c->rows_full.as<int>(), c->count_full.as<int>(),
+                                               c->cap_full, c->s.cutoff * c->s.cutoff, c->cap, c->rows.as<int>(),
+                                               c->count.as<int>());
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- PME dispatch
+#define ORDER_SWITCH(P, ...)                                          \
+    switch (P) {                                                      \
+        case 4: { constexpr int PO = 4; __VA_ARGS__; } break;         \
+        case 5: { constexpr int PO = 5; __VA_ARGS__; } break;         \
+        case 6: { constexpr int PO = 6; __VA_ARGS__; } break;         \
+        case 7: { constexpr int PO = 7; __VA_ARGS__; } break;         \
+        case 8: { constexpr int PO = 8; __VA_ARGS__; } break;         \
+        default: g_err = "PME order must be 4..8"; return PGM_ERR_ARG; \
+    }
+
+template <typename real>
+static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
+    const int P = c->s.order;
+    if (withq) {
+        ORDER_SWITCH(P, (k_spread<real, PO, true><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
+                            n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+    } else {
+        ORDER_SWITCH(P, (k_spread<real, PO, false><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
+                            n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+    }
+    LAUNCH_CHECK();
+    void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
+    FFT_TRY(rocfft_execute(c->fwd, in, out, c->info));
+    const int K3h = c->s.grid[2] / 2 + 1;
+    if (energy)
+        k_convolve<real, true><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
+            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), c->conv_part.as<double>(), conv, iter);
+    else
+        k_convolve<real, false><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
+            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, conv, iter);
+    LAUNCH_CHECK();
+    void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
+    FFT_TRY(rocfft_execute(c->bwd, in2, out2, c->info));
+    return PGM_OK;
+}
+
+// One field evaluation of the dipoles (and charges) dq: spread -> FFT -> convolve -> FFT, the row
+// field, then the gather combined with the row field according to `mode` (pme.h).
+template <typename real>
+static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, const real4<real> *extra, real4<real> *out,
+                      double *part, int iter) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const int *conv = iter >= 0 ? &c->st.as<CGState>()->conv_it : nullptr;
+    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
+    const real beta = (real)c->s.ewald_beta;
+    if (withq)
+        k_row_field<real, true><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<real4<real>>(),
+            conv, iter);
+    else
+        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<real4<real>>(),
+            conv, iter);
+    LAUNCH_CHECK();
+    const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
+    ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
+                                 n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                                 extra, cself, mode, out, part, conv, iter)));
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+static double c_self(const pgm_ctx *c) { return 4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI); }
+
+// ---------------------------------------------------------------------------- induced dipoles
+// Predictor, initial residual (fused or plain) and CG (pgm_jax/md/forcefield.py:1917-2008, :1714-1821).
+template <typename real>
+static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    using R4 = real4<real>;
+    const int nb = nblk(n, 256);
+    R4 *x = c->vx.as<R4>(), *r = c->vr.as<R4>(), *z = c->vz.as<R4>(), *p = c->vp.as<R4>(), *Ap = c->vAp.as<R4>();
+    R4 *b = c->vb.as<R4>(), *x0 = c->vx0.as<R4>(), *dq = c->vdq.as<R4>();
+    const double *q = c->q.as<double>();
+    CGState *S = c->st.as<CGState>();
+    double *rz = c->rz.as<double>(), *err = c->err.as<double>();
+    // predictor coefficients (newest first)
+    double cf[4] = {0, 0, 0, 0};
+    int K = 0;
+    switch (c->s.predictor) {
+        case 1: cf[0] = 3; cf[1] = -3; cf[2] = 1; K = 3; break;
+        case 2: cf[0] = 4; cf[1] = -6; cf[2] = 4; cf[3] = -1; K = 4; break;
+        case 3: cf[0] = 1; K = 1; break;
+        default: K = 0;
+    }
+    const bool have = K > 0 && ind->count >= K;
+    if (have) {
+        k_predict<real><<<nb, 256, 0, st>>>(n, ind->hist, cf[0], cf[1], cf[2], cf[3], x0);
+        LAUNCH_CHECK();
+    }
+    const bool fused = have && c->s.fused && (c->s.norm_refresh <= 0 || ind->count % c->s.norm_refresh != 0);
+    if (fused) {
+        k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), x0, q, dq);
+        LAUNCH_CHECK();
+        PGM_TRY(field_eval<real>(c, dq, true, MODE_FUSED, x0, r, nullptr, -1));
+        k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
+    } else {
+        k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, q, dq);
+        LAUNCH_CHECK();
+        PGM_TRY(field_eval<real>(c, dq, true, MODE_FIELD, nullptr, b, c->partA.as<double>(), -1));
+        k_set_norm<<<1, 256, 0, st>>>(c->partA.as<double>(), nb, 1.0 / (3.0 * n), S);
+        if (!have) k_scale_alpha<real><<<nb, 256, 0, st>>>(n, b, c->alpha_r.as<real>(), x0);
+        LAUNCH_CHECK();
+        PGM_TRY(field_eval<real>(c, x0, false, MODE_PLAIN, b, r, nullptr, -1));
+    }
+    HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
+    k_cg_start<real><<<nb, 256, 0, st>>>(n, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
+    k_cg_start_finish<<<1, 256, 0, st>>>(c->partB.as<double>(), nb, S, rz, err, c->s.dipole_tol, c->s.max_iter);
+    LAUNCH_CHECK();
+    const int max_iter = c->s.max_iter;
+    int launched = 0;
+    int conv = 0;
+    HIP_TRY(hipMemcpyAsync(c->h_pin_int, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    conv = c->h_pin_int[0];
+    while (conv > launched && launched < max_iter) {
+        int batch = launched == 0 ? std::max(1, std::min(c->last_iters, max_iter)) : 1;
+        batch = std::min(batch, max_iter - launched);
+        for (int t = 0; t < batch; ++t) {
+            const int k = launched + t;
+            PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
+            k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), nb, rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
+                                                   c->partB.as<double>(), S);
+            k_cg_update2<real><<<nb, 256, 0, st>>>(n, c->partB.as<double>(), nb, rz, err, k, z, p, S, c->s.dipole_tol,
+                                                   max_iter);
+            LAUNCH_CHECK();
+        }
+        launched += batch;
+        HIP_TRY(hipMemcpyAsync(c->h_pin_int, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        conv = c->h_pin_int[0];
+    }
+    const int it = std::min(conv, launched);
+    c->last_iters = std::max(1, it);
+    k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
+    k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, err + it, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &S->norm, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (iterations) *iterations = it;
+    if (residual) *residual = c->h_pin_dbl[0];
+    ind->norm = c->h_pin_dbl[1];
+    ind->count += 1;
+    return (it >= max_iter && c->h_pin_dbl[0] > c->s.dipole_tol) ? PGM_NOT_CONVERGED : PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- energy and forces
+template <typename real>
+static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    using R4 = real4<real>;
+    const int nb = nblk(n, 256);
+    R4 *dq = c->vdq.as<R4>();
+    k_energy_pack<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), mu, c->q.as<double>(), c->alpha64.as<double>(),
+                                            c->dd.as<double>(), dq, c->pack_part.as<double>());
+    LAUNCH_CHECK();
+    const int npair = nblk((long long)n * ROWL, 256);
+    k_pair_forces<real><<<npair, 256, 0, st>>>(n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+                                               (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(),
+                                               c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(),
+                                               c->pair_part.as<double>());
+    LAUNCH_CHECK();
+    PGM_TRY(spread_fft<real>(c, dq, true, true, nullptr, -1));
+    ORDER_SWITCH(c->s.order, (k_gather_forces<real, PO><<<nb, 256, 0, st>>>(n, c->pos, c->pg, c->grid.as<real>(), dq,
+                                                                           c->dd.as<double>(), c->rowf.as<double>(), c_self(c),
+                                                                           c->rec_dr.as<double>(), c->gp.as<double>())));
+    LAUNCH_CHECK();
+    if (forces) {
+        k_forces_final<<<nb, 256, 0, st>>>(n, c->pos, c->box, c->g_el.as<double>(), c->g_lj.as<double>(), c->rec_dr.as<double>(),
+                                           c->gp.as<double>(), c->cstart.as<int>(), c->ci.as<int>(), c->cj.as<int>(),
+                                           c->cc.as<double>(), c->jstart.as<int>(), c->jlist.as<int>(), forces);
+        LAUNCH_CHECK();
+    }
+    const double b0 = c->s.ewald_beta;
+    const double ke_const = KE * (-(b0 / SQRT_PI) * c->sum_q2 - M_PI * c->sum_q * c->sum_q / (2.0 * c->volume * b0 * b0));
+    const double tail = c->s.lj_lrc ? lj_tail(c) : 0.0;
+    k_energy_final<<<1, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), nb, c->conv_part.as<double>(),
+                                      nblk((long long)c->ncplx, 256), ke_const, tail, c_self(c), c->energy.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (energy) {
+        energy->elec = c->h_pin_dbl[0];
+        energy->vdw = c->h_pin_dbl[1];
+        energy->total = c->h_pin_dbl[2];
+    }
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- C ABI
+extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_settings *s, const double *H, void *stream) {
+    if (!out || !top || !s || !H || top->n_atoms <= 0) {
+        g_err = "pgm_ctx_create: null argument or no atoms";
+        return PGM_ERR_ARG;
+    }
+    if (s->order < 4 || s->order > 8 || s->grid[0] < s->order || s->grid[1] < s->order || s->grid[2] < s->order) {
+        g_err = "pgm_ctx_create: PME order must be 4..8 and each grid size at least the order";
+        return PGM_ERR_ARG;
+    }
+    if (!(s->cutoff > 0) || s->skin < 0 || !(s->ewald_beta > 0) || s->max_iter < 0 || s->max_iter > 100000) {
+        g_err = "pgm_ctx_create: invalid cutoff, skin, ewald_beta or max_iter";
+        return PGM_ERR_ARG;
+    }
+    pgm_ctx *c = new pgm_ctx();
+    c->n = top->n_atoms;
+    c->s = *s;
+    c->stream = (hipStream_t)stream;
+    c->mixed = s->precision == 1;
+    c->rsize = c->mixed ? sizeof(float) : sizeof(double);
+    const int n = c->n;
+    c->h_q.assign(top->q, top->q + n);
+    c->h_alpha.assign(top->alpha, top->alpha + n);
+    c->h_rh.assign(top->lj_rmin_half, top->lj_rmin_half + n);
+    c->h_se.assign(top->lj_sqrt_eps, top->lj_sqrt_eps + n);
+    c->h_mol.assign(top->mol, top->mol + n);
+    c->h_R2.resize(n);
+    for (int i = 0; i < n; ++i) {
+        c->h_R2[i] = top->radius[i] * top->radius[i];
+        c->sum_q += c->h_q[i];
+        c->sum_q2 += c->h_q[i] * c->h_q[i];
+        if (!(c->h_alpha[i] > 0)) {
+            g_err = "pgm_ctx_create: every polarizability must be positive";
+            delete c;
+            return PGM_ERR_ARG;
+        }
+    }
+    auto fail = [&](int r) {
+        pgm_ctx_destroy(c);
+        return r;
+    };
+    int r;
+    hipStream_t st = c->stream;
+    if ((r = upload(c->q, c->h_q.data(), n, st)) < 0) return fail(r);
+    if ((r = upload(c->alpha64, c->h_alpha.data(), n, st)) < 0) return fail(r);
+    if ((r = upload(c->rh, c->h_rh.data(), n, st)) < 0) return fail(r);
+    if ((r = upload(c->se, c->h_se.data(), n, st)) < 0) return fail(r);
+    if ((r = upload(c->mol, c->h_mol.data(), n, st)) < 0) return fail(r);
+    if ((r = upload(c->R2, c->h_R2.data(), n, st)) < 0) return fail(r);
+    if (c->mixed) {
+        std::vector<float> af(c->h_alpha.begin(), c->h_alpha.end());
+        if ((r = upload(c->alpha_r, af.data(), n, st)) < 0) return fail(r);
+    } else {
+        if ((r = upload(c->alpha_r, c->h_alpha.data(), n, st)) < 0) return fail(r);
+    }
+    // covalent dipoles: CSR by owner atom (cstart over m sorted by cov_i) and by target (jstart/jlist)
+    const int nc = top->n_cov;
+    std::vector<int> order(nc), ci(nc), cj(nc), cs(n + 1, 0), js(n + 1, 0), jl(nc);
+    std::vector<double> cc(nc);
+    for (int m = 0; m < nc; ++m) {
+        if (top->cov_i[m] < 0 || top->cov_i[m] >= n || top->cov_j[m] < 0 || top->cov_j[m] >= n) {
+            g_err = "pgm_ctx_create: covalent dipole atom out of range";
+            return fail(PGM_ERR_ARG);
+        }
+        order[m] = m;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return top->cov_i[a] < top->cov_i[b]; });
+    for (int k = 0; k < nc; ++k) {
+        const int m = order[k];
+        ci[k] = top->cov_i[m];
+        cj[k] = top->cov_j[m];
+        cc[k] = top->cov_c[m];
+        cs[ci[k] + 1]++;
+        js[cj[k] + 1]++;
+    }
+    for (int i = 0; i < n; ++i) {
+        cs[i + 1] += cs[i];
+        js[i + 1] += js[i];
+    }
+    std::vector<int> fillj(js.begin(), js.end() - 1);
+    for (int k = 0; k < nc; ++k) jl[fillj[cj[k]]++] = k;
+    if ((r = upload(c->cstart, cs.data(), n + 1, st)) < 0) return fail(r);
+    if ((r = upload(c->ci, ci.data(), nc, st)) < 0) return fail(r);
+    if ((r = upload(c->cj, cj.data(), nc, st)) < 0) return fail(r);
+    if ((r = upload(c->cc, cc.data(), nc, st)) < 0) return fail(r);
+    if ((r = upload(c->jstart, js.data(), n + 1, st)) < 0) return fail(r);
+    if ((r = upload(c->jlist, jl.data(), nc, st)) < 0) return fail(r);
+    // buffers
+    const size_t r4 = 4 * c->rsize * (size_t)n, d3 = sizeof(double) * 3 * (size_t)n;
+    DevBuf *v4[] = {&c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp};
+    for (DevBuf *b : v4)
+        if ((r = b->ensure(r4)) < 0) return fail(r);
+    DevBuf *v3[] = {&c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp};
+    for (DevBuf *b : v3)
+        if ((r = b->ensure(d3)) < 0) return fail(r);
+    if ((r = c->xr.ensure(sizeof(double4) * n)) < 0) return fail(r);
+    const int nbmax = nblk((long long)n * ROWL, 64) + 16;
+    DevBuf *parts[] = {&c->partA, &c->partB, &c->partC, &c->pair_part, &c->pack_part};
+    for (DevBuf *b : parts)
+        if ((r = b->ensure(sizeof(double) * 4 * nbmax)) < 0) return fail(r);
+    if ((r = c->st.ensure(sizeof(CGState))) < 0) return fail(r);
+    if ((r = c->rz.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
+    if ((r = c->err.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
+    if ((r = c->energy.ensure(sizeof(double) * 4)) < 0) return fail(r);
+    if ((r = c->maxcount.ensure(sizeof(int))) < 0) return fail(r);
+    DevBuf *ib[] = {&c->key, &c->atom, &c->skey, &c->satom, &c->count_full, &c->count};
+    for (DevBuf *b : ib)
+        if ((r = b->ensure(sizeof(int) * n)) < 0) return fail(r);
+    if (hipHostMalloc((void **)&c->h_pin_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pin_dbl, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        return fail(PGM_ERR_HIP);
+    }
+    c->ngrid = (size_t)s->grid[0] * s->grid[1] * s->grid[2];
+    c->ncplx = (size_t)s->grid[0] * s->grid[1] * (s->grid[2] / 2 + 1);
+    if ((r = c->grid.ensure(c->rsize * c->ngrid)) < 0) return fail(r);
+    if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx)) < 0) return fail(r);
+    if ((r = c->conv_part.ensure(sizeof(double) * (nblk((long long)c->ncplx, 256) + 16))) < 0) return fail(r);
+    if ((r = set_box(c, H)) < 0) return fail(r);
+    // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
+    const double rl = s->cutoff + s->skin;
+    const int cap0 = (int)(1.25 * 4.0 / 3.0 * M_PI * rl * rl * rl * n / c->volume) + 32;
+    if ((r = alloc_rows(c, std::min(cap0, n))) < 0) return fail(r);
+    if ((r = make_plans(c)) < 0) return fail(r);
+    *out = c;
+    return PGM_OK;
+}
+
+extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
+    if (!c) return PGM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    if (c->fwd) rocfft_plan_destroy(c->fwd);
+    if (c->bwd) rocfft_plan_destroy(c->bwd);
+    if (c->info) rocfft_execution_info_destroy(c->info);
+    DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
+                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->maxcount, &c->key,
+                     &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
+                     &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
+                     &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
+                     &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy};
+    for (DevBuf *b : all) b->release();
+    if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
+    if (c->h_pin_dbl) (void)hipHostFree(c->h_pin_dbl);
+    delete c;
+    return PGM_OK;
+}
+
+extern "C" int pgm_set_stream(pgm_ctx *c, void *stream) {
+    if (!c) return PGM_ERR_ARG;
+    c->stream = (hipStream_t)stream;
+    if (c->info) FFT_TRY(rocfft_execution_info_set_stream(c->info, c->stream));
+    return PGM_OK;
+}
+
+extern "C" int pgm_set_box(pgm_ctx *c, const double *H) {
+    if (!c || !H) return PGM_ERR_ARG;
+    return set_box(c, H);
+}
+
+extern "C" int pgm_build_rows(pgm_ctx *c, const double *pos) {
+    if (!c || !pos) return PGM_ERR_ARG;
+    return build_rows(c, pos);
+}
+
+extern "C" long long pgm_row_entries(pgm_ctx *c) {
+    if (!c || !c->rows_valid) return -1;
+    std::vector<int> cnt(c->n);
+    if (hipMemcpyAsync(cnt.data(), c->count.p, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    long long s = 0;
+    for (int v : cnt) s += v;
+    return s;
+}
+
+extern "C" int pgm_perm_dipoles(pgm_ctx *c, const double *pos, double *p) {
+    if (!c || !pos || !p) return PGM_ERR_ARG;
+    k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
+                                                            c->cc.as<double>(), p);
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+template <typename real> static int field_api(pgm_ctx *c, const double *pos, int withq, const double *d, double *field) {
+    PGM_TRY(prepare(c, pos));
+    using R4 = real4<real>;
+    const int nb = nblk(c->n, 256);
+    k_pack_dq<real><<<nb, 256, 0, c->stream>>>(c->n, d, nullptr, withq ? c->q.as<double>() : nullptr, c->vdq.as<R4>());
+    LAUNCH_CHECK();
+    PGM_TRY(field_eval<real>(c, c->vdq.as<R4>(), withq != 0, MODE_FIELD, nullptr, c->vtmp.as<R4>(), nullptr, -1));
+    k_to_double3<real><<<nb, 256, 0, c->stream>>>(c->n, c->vtmp.as<R4>(), field);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PGM_OK;
+}
+
+extern "C" int pgm_field(pgm_ctx *c, const double *pos, int with_charges, const double *d, double *field) {
+    if (!c || !pos || !d || !field) return PGM_ERR_ARG;
+    return c->mixed ? field_api<float>(c, pos, with_charges, d, field) : field_api<double>(c, pos, with_charges, d, field);
+}
+
+template <typename real>
+static int pme_api(pgm_ctx *c, const double *pos, int withq, const double *d, double *e_rec, double *grad_d) {
+    using R4 = real4<real>;
+    c->pos = pos;
+    const int nb = nblk(c->n, 256);
+    k_pack_dq<real><<<nb, 256, 0, c->stream>>>(c->n, d, nullptr, withq ? c->q.as<double>() : nullptr, c->vdq.as<R4>());
+    LAUNCH_CHECK();
+    PGM_TRY(spread_fft<real>(c, c->vdq.as<R4>(), withq != 0, true, nullptr, -1));
+    if (grad_d) {
+        ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nb, 256, 0, c->stream>>>(
+                                     c->n, pos, c->pg, c->grid.as<real>(), c->vdq.as<R4>(), c->vrow.as<R4>(),
+                                     c->alpha_r.as<real>(), nullptr, real(0), MODE_REC, c->vtmp.as<R4>(), nullptr, nullptr, -1)));
+        k_to_double3<real><<<nb, 256, 0, c->stream>>>(c->n, c->vtmp.as<R4>(), grad_d);
+        LAUNCH_CHECK();
+    }
+    // energy: sum of the convolution partials
+    k_energy_final<<<1, 256, 0, c->stream>>>(c->pair_part.as<double>(), 0, c->pack_part.as<double>(), 0, c->conv_part.as<double>(),
+                                             nblk((long long)c->ncplx, 256), 0.0, 0.0, 0.0, c->energy.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e_rec) *e_rec = c->h_pin_dbl[3];
+    return PGM_OK;
+}
+
+extern "C" int pgm_pme_energy(pgm_ctx *c, const double *pos, int with_charges, const double *d, double *e_rec, double *grad_d) {
+    if (!c || !pos || !d) return PGM_ERR_ARG;
+    return c->mixed ? pme_api<float>(c, pos, with_charges, d, e_rec, grad_d) : pme_api<double>(c, pos, with_charges, d, e_rec, grad_d);
+}
+
+template <typename real>
+static int compute_api(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces, int *iterations,
+                       double *residual) {
+    PGM_TRY(prepare(c, pos));
+    k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
+                                                            c->cc.as<double>(), c->pperm.as<double>());
+    LAUNCH_CHECK();
+    const int rs = solve_impl<real>(c, ind, iterations, residual);
+    if (rs < 0) return rs;
+    PGM_TRY(energy_forces_impl<real>(c, ind->mu, energy, forces));
+    return rs;
+}
+
+extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces, int *iterations,
+                           double *residual) {
+    if (!c || !pos || !ind || !ind->mu || !ind->hist) return PGM_ERR_ARG;
+    return c->mixed ? compute_api<float>(c, pos, ind, energy, forces, iterations, residual)
+                    : compute_api<double>(c, pos, ind, energy, forces, iterations, residual);
+}
+
+template <typename real>
+static int ef_api(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces) {
+    PGM_TRY(prepare(c, pos));
+    k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
+                                                            c->cc.as<double>(), c->pperm.as<double>());
+    LAUNCH_CHECK();
+    return energy_forces_impl<real>(c, mu, energy, forces);
+}
+
+extern "C" int pgm_energy_forces(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces) {
+    if (!c || !pos || !mu) return PGM_ERR_ARG;
+    return c->mixed ? ef_api<float>(c, pos, mu, energy, forces) : ef_api<double>(c, pos, mu, energy, forces);
+}

@@ -1,0 +1,283 @@
+// pairs.h -- neighbour rows and the real-space pair kernels of the pGM force field.
+//
+// Rows: for every atom i, the partners j with |min_image(r_i - r_j)| below a cutoff, stored as
+// int32 in row-major (N, cap) order; every pair appears in both rows (no scatter, no atomics: the
+// per-atom sums are deterministic).  Rows are built from a cell list every few steps with the
+// cutoff + skin, and compacted to the cutoff at every force evaluation.
+//
+// Pair kernels use ROWL lanes per atom (lane l takes entries l, l + ROWL, ...), so one wave64
+// covers 64/ROWL atoms; the partial sums are reduced across the ROWL lanes with shuffles.  The
+// displacement is formed in FP64 from the FP64 positions (exact for intramolecular pairs whatever
+// the absolute coordinates) and then rounded to `real`.
+#pragma once
+
+#include "common.h"
+
+namespace pgm {
+
+constexpr int ROWL = 8;  // lanes per atom in the row kernels
+
+// Packed per-atom geometry xr: x, y, z [nm] and R^2 (Gaussian radius squared) [nm^2].
+// ---------------------------------------------------------------------------- cells
+// Cell of every atom in fractional coordinates of the box; key = (c0 * n1 + c1) * n2 + c2.
+__global__ void k_cell_keys(int n, const double *__restrict__ pos, Box box, int3 nc, int *__restrict__ key,
+                            int *__restrict__ atom) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double *hi = box.hinv;  // u_b = sum_a r_a hinv[a][b]
+    double u[3] = {x * hi[0] + y * hi[3] + z * hi[6], x * hi[1] + y * hi[4] + z * hi[7], x * hi[2] + y * hi[5] + z * hi[8]};
+    int c[3];
+    const int ncs[3] = {nc.x, nc.y, nc.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double f = u[a] - floor(u[a]);
+        int k = (int)(f * ncs[a]);
+        c[a] = k < 0 ? 0 : (k >= ncs[a] ? ncs[a] - 1 : k);
+    }
+    key[i] = (c[0] * nc.y + c[1]) * nc.z + c[2];
+    atom[i] = i;
+}
+
+__global__ void k_fill_int(int n, int *__restrict__ p, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// cell_start[c] / cell_end[c] from the sorted keys.
+__global__ void k_cell_bounds(int n, const int *__restrict__ skey, int *__restrict__ start, int *__restrict__ end) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = skey[s];
+    if (s == 0 || skey[s - 1] != k) start[k] = s;
+    if (s == n - 1 || skey[s + 1] != k) end[k] = s + 1;
+}
+
+// Full rows within rc_list (cutoff + skin) by a search of the neighbouring cells (27, or every
+// distinct cell along an axis with fewer than 3 cells).  One thread per atom; the partners come in
+// cell order and, within a cell, in atom order (deterministic).
+__global__ void k_build_rows(int n, const double *__restrict__ pos, Box box, int3 nc, const int *__restrict__ satom,
+                             const int *__restrict__ cstart, const int *__restrict__ cend, double rc2, int cap,
+                             int *__restrict__ rows, int *__restrict__ count, int *__restrict__ maxcount) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const double *hi = box.hinv;
+    const double u[3] = {xi * hi[0] + yi * hi[3] + zi * hi[6], xi * hi[1] + yi * hi[4] + zi * hi[7],
+                         xi * hi[2] + yi * hi[5] + zi * hi[8]};
+    const int ncs[3] = {nc.x, nc.y, nc.z};
+    int c[3], lo[3], hiO[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double f = u[a] - floor(u[a]);
+        int k = (int)(f * ncs[a]);
+        c[a] = k < 0 ? 0 : (k >= ncs[a] ? ncs[a] - 1 : k);
+        lo[a] = ncs[a] >= 3 ? -1 : 0;
+        hiO[a] = ncs[a] >= 3 ? 1 : ncs[a] - 1;
+    }
+    int cnt = 0;
+    int *row = rows + (size_t)i * cap;
+    for (int o0 = lo[0]; o0 <= hiO[0]; ++o0) {
+        const int c0 = (c[0] + o0 + ncs[0]) % ncs[0];
+        for (int o1 = lo[1]; o1 <= hiO[1]; ++o1) {
+            const int c1 = (c[1] + o1 + ncs[1]) % ncs[1];
+            for (int o2 = lo[2]; o2 <= hiO[2]; ++o2) {
+                const int c2 = (c[2] + o2 + ncs[2]) % ncs[2];
+                const int cell = (c0 * nc.y + c1) * nc.z + c2;
+                const int s0 = cstart[cell], s1 = cend[cell];
+                for (int s = s0; s < s1; ++s) {
+                    const int j = satom[s];
+                    if (j == i) continue;
+                    double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+                    min_image(dx, dy, dz, box);
+                    if (dx * dx + dy * dy + dz * dz < rc2) {
+                        if (cnt < cap) row[cnt] = j;
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    count[i] = cnt;
+    atomicMax(maxcount, cnt);
+}
+
+// Compacted rows within rc (order kept): one wave per atom, ballot + prefix count.
+__global__ __launch_bounds__(256) void k_compact_rows(int n, const double4 *__restrict__ xr, Box box,
+                                                      const int *__restrict__ rows_full, const int *__restrict__ count_full,
+                                                      int cap_full, double rc2, int cap, int *__restrict__ rows,
+                                                      int *__restrict__ count) {
+    const int lane = threadIdx.x % WAVE;
+    const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (i >= n) return;
+    const double4 a = xr[i];
+    const int cf = count_full[i] < cap_full ? count_full[i] : cap_full;
+    const int *rf = rows_full + (size_t)i * cap_full;
+    int *rc = rows + (size_t)i * cap;
+    int out = 0;
+    for (int e0 = 0; e0 < cf; e0 += WAVE) {
+        const int e = e0 + lane;
+        bool keep = false;
+        int j = 0;
+        if (e < cf) {
+            j = rf[e];
+            const double4 b = xr[j];
+            double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+            min_image(dx, dy, dz, box);
+            keep = dx * dx + dy * dy + dz * dz < rc2;
+        }
+        const unsigned long long m = __ballot(keep);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (keep && out + before < cap) rc[out + before] = j;
+        out += __popcll(m);
+    }
+    if (lane == 0) count[i] = out < cap ? out : cap;
+}
+
+// ---------------------------------------------------------------------------- field rows
+// out_i = sum_k de_ik/dd_i = sum_k (-q_k G1 - G2 (d_k . x)) x + G1 d_k, x = r_i - r_k
+// (minus the direct-space field at atom i; pgm_jax/md/forcefield.py:1395-1402).
+// dq holds (d_x, d_y, d_z, q) per atom; WITHQ adds the charge term.
+template <typename real, bool WITHQ>
+__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ rows, const int *__restrict__ count, int cap,
+                                                   const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
+                                                   real beta, real4<real> *__restrict__ out, const int *__restrict__ conv_it,
+                                                   int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    const int lane = threadIdx.x % ROWL;
+    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    if (i >= n) return;
+    const double4 a = xr[i];
+    const int c = count[i];
+    const int *row = rows + (size_t)i * cap;
+    real ex = 0, ey = 0, ez = 0;
+    for (int e = lane; e < c; e += ROWL) {
+        const int j = row[e];
+        const double4 b = xr[j];
+        double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        min_image(dx, dy, dz, box);
+        const real x = (real)dx, y = (real)dy, z = (real)dz;
+        const real r2 = x * x + y * y + z * z;
+        const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
+        real G[3];
+        pair_G<real, 3>(aij, beta, r2, G);
+        const real4<real> dk = dq[j];
+        const real dkx = dk.x * x + dk.y * y + dk.z * z;
+        real cf = -G[2] * dkx;
+        if (WITHQ) cf -= dk.w * G[1];
+        ex += cf * x + G[1] * dk.x;
+        ey += cf * y + G[1] * dk.y;
+        ez += cf * z + G[1] * dk.z;
+    }
+    ex = group_sum<ROWL>(ex);
+    ey = group_sum<ROWL>(ey);
+    ez = group_sum<ROWL>(ez);
+    if (lane == 0) {
+        real4<real> o;
+        o.x = ex;
+        o.y = ey;
+        o.z = ez;
+        o.w = 0;
+        out[i] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------- pair forces
+// Per atom i, over its row (forcefield.py:2294-2360, :2116-2146):
+//   e_ik  = q_i q_k G0 + (q_i d_k.x - q_k d_i.x) G1 - G2 (d_i.x)(d_k.x) + G1 d_i.d_k
+//   de/dx = x [-q_i q_k G1 - t G2 + G3 (d_i.x)(d_k.x) - G2 d_i.d_k] + q_i G1 d_k - q_k G1 d_i
+//           - G2 (d_i (d_k.x) + d_k (d_i.x)),   t = q_i d_k.x - q_k d_i.x
+//   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2.
+// Outputs (FP64): g_el = sum de/dx, g_lj, the row field sum de/dd_i, and per-block energy sums
+// (each pair counted in both rows: the caller halves them).
+template <typename real>
+__global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restrict__ rows, const int *__restrict__ count, int cap,
+                                                     const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
+                                                     real beta, const double *__restrict__ rh, const double *__restrict__ se,
+                                                     const int *__restrict__ mol, double *__restrict__ g_el,
+                                                     double *__restrict__ g_lj, double *__restrict__ rowf,
+                                                     double *__restrict__ epart) {
+    __shared__ double lds[2 * 16];
+    const int lane = threadIdx.x % ROWL;
+    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    double esum[2] = {0.0, 0.0};
+    if (i < n) {
+        const double4 a = xr[i];
+        const real4<real> di = dq[i];
+        const real qi = di.w;
+        const double rhi = rh[i], sei = se[i];
+        const int mi = mol[i];
+        const int c = count[i];
+        const int *row = rows + (size_t)i * cap;
+        double gx = 0, gy = 0, gz = 0, lx = 0, ly = 0, lz = 0, fx = 0, fy = 0, fz = 0, ee = 0, el = 0;
+        for (int e = lane; e < c; e += ROWL) {
+            const int j = row[e];
+            const double4 b = xr[j];
+            double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+            min_image(dx, dy, dz, box);
+            const real x = (real)dx, y = (real)dy, z = (real)dz;
+            const real r2 = x * x + y * y + z * z;
+            const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
+            real G[4];
+            pair_G<real, 4>(aij, beta, r2, G);
+            const real4<real> dk = dq[j];
+            const real qk = dk.w;
+            const real dix = di.x * x + di.y * y + di.z * z;
+            const real dkx = dk.x * x + dk.y * y + dk.z * z;
+            const real didk = di.x * dk.x + di.y * dk.y + di.z * dk.z;
+            const real t = qi * dkx - qk * dix;
+            const real ep = qi * qk * G[0] + t * G[1] - G[2] * dix * dkx + G[1] * didk;
+            const real radial = -qi * qk * G[1] - t * G[2] + G[3] * dix * dkx - G[2] * didk;
+            const real qiG1 = qi * G[1], qkG1 = qk * G[1];
+            gx += (double)(radial * x + qiG1 * dk.x - qkG1 * di.x - G[2] * (di.x * dkx + dk.x * dix));
+            gy += (double)(radial * y + qiG1 * dk.y - qkG1 * di.y - G[2] * (di.y * dkx + dk.y * dix));
+            gz += (double)(radial * z + qiG1 * dk.z - qkG1 * di.z - G[2] * (di.z * dkx + dk.z * dix));
+            ee += (double)ep;
+            const real cfield = -qk * G[1] - G[2] * dkx;
+            fx += (double)(cfield * x + G[1] * dk.x);
+            fy += (double)(cfield * y + G[1] * dk.y);
+            fz += (double)(cfield * z + G[1] * dk.z);
+            const double eps = sei * se[j];
+            if (mol[j] != mi && eps != 0.0) {
+                const double rmin = rhi + rh[j];
+                const double rr2 = dx * dx + dy * dy + dz * dz;
+                const double s2 = rmin * rmin / rr2;
+                const double s6 = s2 * s2 * s2;
+                el += eps * (s6 * s6 - 2.0 * s6);
+                const double dl = eps * 12.0 * (s6 - s6 * s6) / rr2;
+                lx += dl * dx;
+                ly += dl * dy;
+                lz += dl * dz;
+            }
+        }
+        gx = group_sum<ROWL>(gx);
+        gy = group_sum<ROWL>(gy);
+        gz = group_sum<ROWL>(gz);
+        lx = group_sum<ROWL>(lx);
+        ly = group_sum<ROWL>(ly);
+        lz = group_sum<ROWL>(lz);
+        fx = group_sum<ROWL>(fx);
+        fy = group_sum<ROWL>(fy);
+        fz = group_sum<ROWL>(fz);
+        if (lane == 0) {
+            g_el[3 * i] = gx;
+            g_el[3 * i + 1] = gy;
+            g_el[3 * i + 2] = gz;
+            g_lj[3 * i] = lx;
+            g_lj[3 * i + 1] = ly;
+            g_lj[3 * i + 2] = lz;
+            rowf[3 * i] = fx;
+            rowf[3 * i + 1] = fy;
+            rowf[3 * i + 2] = fz;
+        }
+        esum[0] = ee;
+        esum[1] = el;
+    }
+    block_sum<2>(esum, lds);
+    if (threadIdx.x == 0) {
+        epart[2 * blockIdx.x] = esum[0];
+        epart[2 * blockIdx.x + 1] = esum[1];
+    }
+}
+
+}  // namespace pgm

@@ -1,0 +1,292 @@
+// pme.h -- smooth PME of Gaussian charges and point dipoles: spread, convolution, gathers.
+//
+// Per atom: u = r H^-1 (FP64), w = K frac(u), base = floor(w), f = w - base (rounded to real);
+// theta / dtheta (and for forces d2theta) of order P are recomputed from f in registers by every
+// kernel (no per-atom spline storage).  Grid point (base + k) mod K gets weight theta[k].
+//   Q(g) = sum_i q_i t1 t2 t3 + e_i1 d1 t2 t3 + e_i2 t1 d2 t3 + e_i3 t1 t2 d3,  e_i = d_i H^-1 K
+// (pgm_jax/md/pme.py:306-337).  phi = dE_rec/dQ comes from r2c FFT -> multiply by
+// Gc = 2 G / w3 -> c2r FFT (unnormalised), pme.py:373-376.
+#pragma once
+
+#include "common.h"
+
+namespace pgm {
+
+struct PMEGeom {
+    double hinv[9];      // H^-1
+    double escale[9];    // H^-1 with column b scaled by K_b: e = d . escale
+    int K[3];
+};
+
+template <typename real, int P>
+PGM_DEV void stencil(const double *__restrict__ pos, int i, const PMEGeom &g, int *base, real *f) {
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double *hi = g.hinv;
+    const double u[3] = {x * hi[0] + y * hi[3] + z * hi[6], x * hi[1] + y * hi[4] + z * hi[7], x * hi[2] + y * hi[5] + z * hi[8]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double w = (u[a] - floor(u[a])) * (double)g.K[a];
+        const double b = floor(w);
+        base[a] = (int)b;
+        f[a] = (real)(w - b);
+    }
+}
+
+template <typename real>
+PGM_DEV void scaled_dipole(const real4<real> &d, const PMEGeom &g, real *e) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) e[b] = d.x * (real)g.escale[b] + d.y * (real)g.escale[3 + b] + d.z * (real)g.escale[6 + b];
+}
+
+template <typename T> __global__ void k_fill(size_t n, T *__restrict__ p, T v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Spread: one thread per (atom, x offset); 36 atomic adds each (v1 of the spread; deterministic
+// order is not guaranteed).  dq = (d, q); WITHQ includes the charges.
+template <typename real, int P, bool WITHQ>
+__global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict__ pos, const real4<real> *__restrict__ dq,
+                                                PMEGeom g, real *__restrict__ grid, const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / P, ix = t - i * P;
+    if (i >= n) return;
+    int base[3];
+    real f[3];
+    stencil<real, P>(pos, i, g, base, f);
+    real t1[P], d1[P], t2[P], d2[P], t3[P], d3[P];
+    bspline<real, P, false>(f[0], t1, d1, nullptr);
+    bspline<real, P, false>(f[1], t2, d2, nullptr);
+    bspline<real, P, false>(f[2], t3, d3, nullptr);
+    const real4<real> d = dq[i];
+    real e[3];
+    scaled_dipole<real>(d, g, e);
+    const real q = WITHQ ? d.w : real(0);
+    const real a1 = q * t1[ix] + e[0] * d1[ix];
+    const real b1 = e[1] * t1[ix];
+    const real c1 = e[2] * t1[ix];
+    const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    int gx = base[0] + ix;
+    gx -= gx >= K1 ? K1 : 0;
+#pragma unroll
+    for (int iy = 0; iy < P; ++iy) {
+        const real A = a1 * t2[iy] + b1 * d2[iy];
+        const real C = c1 * t2[iy];
+        int gy = base[1] + iy;
+        gy -= gy >= K2 ? K2 : 0;
+        real *line = grid + ((size_t)gx * K2 + gy) * K3;
+#pragma unroll
+        for (int iz = 0; iz < P; ++iz) {
+            int gz = base[2] + iz;
+            gz -= gz >= K3 ? K3 : 0;
+            unsafeAtomicAdd(line + gz, A * t3[iz] + C * d3[iz]);
+        }
+    }
+}
+
+// Convolution on the half spectrum: F[Q] *= Gc; with ENERGY also sum (w3/2) Gc |F[Q]|^2 per block
+// (= sum G |F[Q]|^2 with G of pme.py:243-267), FP64.
+template <typename real, bool ENERGY>
+__global__ __launch_bounds__(256) void k_convolve(size_t nc, int K3h, int K3even, real *__restrict__ cgrid,
+                                                  const real *__restrict__ Gc, double *__restrict__ epart,
+                                                  const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    __shared__ double lds[16];
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double e[1] = {0.0};
+    if (s < nc) {
+        const real gc = Gc[s];
+        const real re = cgrid[2 * s], im = cgrid[2 * s + 1];
+        if (ENERGY) {
+            const int k3 = (int)(s % (size_t)K3h);
+            const double w = (k3 == 0 || (K3even && k3 == K3h - 1)) ? 0.5 : 1.0;
+            e[0] = w * (double)gc * ((double)re * re + (double)im * im);
+        }
+        cgrid[2 * s] = re * gc;
+        cgrid[2 * s + 1] = im * gc;
+    }
+    if (ENERGY) {
+        block_sum<1>(e, lds);
+        if (threadIdx.x == 0) epart[blockIdx.x] = e[0];
+    }
+}
+
+// Gather of the dipole gradient (pme.py:378-388): g_a = sum phi * (derivative spline along a),
+// dU/dd = escale . g; then combined with the row field according to `mode`:
+//   0 FIELD   out = -(row + rec - c_self d)                          [+ sum |alpha out| per block]
+//   1 AV      out = d / alpha + row + rec - c_self d                 [+ sum d . out per block]
+//   2 FUSED   out = -(row + rec - c_self d) - x0 / alpha             (d = p + x0, extra = x0)
+//   3 PLAIN   out = b - (d / alpha + row + rec - c_self d)           (d = x0, extra = b)
+//   4 REC     out = dU_rec/dd alone
+enum { MODE_FIELD = 0, MODE_AV = 1, MODE_FUSED = 2, MODE_PLAIN = 3, MODE_REC = 4 };
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_combine(int n, const double *__restrict__ pos, PMEGeom g,
+                                                        const real *__restrict__ phi, const real4<real> *__restrict__ dq,
+                                                        const real4<real> *__restrict__ row, const real *__restrict__ alpha,
+                                                        const real4<real> *__restrict__ extra, real cself, int mode,
+                                                        real4<real> *__restrict__ out, double *__restrict__ part,
+                                                        const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    __shared__ double lds[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < n) {
+        int base[3];
+        real f[3];
+        stencil<real, P>(pos, i, g, base, f);
+        real t1[P], d1[P], t2[P], d2[P], t3[P], d3[P];
+        bspline<real, P, false>(f[0], t1, d1, nullptr);
+        bspline<real, P, false>(f[1], t2, d2, nullptr);
+        bspline<real, P, false>(f[2], t3, d3, nullptr);
+        const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+        real g1 = 0, g2 = 0, g3 = 0;
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) {
+            int gx = base[0] + ix;
+            gx -= gx >= K1 ? K1 : 0;
+            real s_tt = 0, s_dt = 0, s_td = 0;  // sums over (iy, iz) of phi t2 t3, phi d2 t3, phi t2 d3
+#pragma unroll
+            for (int iy = 0; iy < P; ++iy) {
+                int gy = base[1] + iy;
+                gy -= gy >= K2 ? K2 : 0;
+                const real *line = phi + ((size_t)gx * K2 + gy) * K3;
+                real lt = 0, ld = 0;
+#pragma unroll
+                for (int iz = 0; iz < P; ++iz) {
+                    int gz = base[2] + iz;
+                    gz -= gz >= K3 ? K3 : 0;
+                    const real v = line[gz];
+                    lt += v * t3[iz];
+                    ld += v * d3[iz];
+                }
+                s_tt += lt * t2[iy];
+                s_dt += lt * d2[iy];
+                s_td += ld * t2[iy];
+            }
+            g1 += s_tt * d1[ix];
+            g2 += s_dt * t1[ix];
+            g3 += s_td * t1[ix];
+        }
+        const double *es = g.escale;
+        const real rx = (real)es[0] * g1 + (real)es[1] * g2 + (real)es[2] * g3;
+        const real ry = (real)es[3] * g1 + (real)es[4] * g2 + (real)es[5] * g3;
+        const real rz = (real)es[6] * g1 + (real)es[7] * g2 + (real)es[8] * g3;
+        const real4<real> d = dq[i];
+        const real4<real> rw = row[i];
+        const real tx = rw.x + rx - cself * d.x;  // dU/dd (direct + reciprocal + self)
+        const real ty = rw.y + ry - cself * d.y;
+        const real tz = rw.z + rz - cself * d.z;
+        real4<real> o;
+        o.w = 0;
+        const real ia = real(1) / alpha[i];
+        if (mode == MODE_FIELD) {
+            o.x = -tx;
+            o.y = -ty;
+            o.z = -tz;
+            const real a = alpha[i];
+            acc[0] = fabs((double)(a * o.x)) + fabs((double)(a * o.y)) + fabs((double)(a * o.z));
+        } else if (mode == MODE_AV) {
+            o.x = d.x * ia + tx;
+            o.y = d.y * ia + ty;
+            o.z = d.z * ia + tz;
+            acc[0] = (double)(d.x * o.x) + (double)(d.y * o.y) + (double)(d.z * o.z);
+        } else if (mode == MODE_FUSED) {
+            const real4<real> x0 = extra[i];
+            o.x = -tx - x0.x * ia;
+            o.y = -ty - x0.y * ia;
+            o.z = -tz - x0.z * ia;
+        } else if (mode == MODE_REC) {
+            o.x = rx;
+            o.y = ry;
+            o.z = rz;
+        } else {
+            const real4<real> b = extra[i];
+            o.x = b.x - (d.x * ia + tx);
+            o.y = b.y - (d.y * ia + ty);
+            o.z = b.z - (d.z * ia + tz);
+        }
+        out[i] = o;
+    }
+    if (part) {
+        block_sum<1>(acc, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+    }
+}
+
+// Force gather: dU_rec/dd and dU_rec/dr at fixed q, d (the autodiff of pme.py:390-411 written
+// out, with the second spline derivative), FP64 outputs; also gp = KE (rowf + dU_rec/dd - c_self d),
+// the total dE/dd of the atom for the covalent-frame pull-back.
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_forces(int n, const double *__restrict__ pos, PMEGeom g,
+                                                       const real *__restrict__ phi, const real4<real> *__restrict__ dq,
+                                                       const double *__restrict__ dd, const double *__restrict__ rowf,
+                                                       double cself, double *__restrict__ rec_dr, double *__restrict__ gp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int base[3];
+    real f[3];
+    stencil<real, P>(pos, i, g, base, f);
+    real t[3][P], dt[3][P], d2t[3][P];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bspline<real, P, true>(f[a], t[a], dt[a], d2t[a]);
+    const real4<real> d = dq[i];
+    real e[3];
+    scaled_dipole<real>(d, g, e);
+    const real q = d.w;
+    const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    // phi-weighted sums S[wx][wy][wz] with w = 0 (theta), 1 (dtheta), 2 (d2theta), needed:
+    // gd_a = S(d along a), gw_a = q S(d along a) + sum_b e_b S(d along a, d along b / d2 if a == b)
+    double s100 = 0, s010 = 0, s001 = 0, s200 = 0, s020 = 0, s002 = 0, s110 = 0, s101 = 0, s011 = 0;
+#pragma unroll
+    for (int ix = 0; ix < P; ++ix) {
+        int gx = base[0] + ix;
+        gx -= gx >= K1 ? K1 : 0;
+        real a_0 = 0, a_y = 0, a_z = 0, a_yy = 0, a_zz = 0, a_yz = 0;
+#pragma unroll
+        for (int iy = 0; iy < P; ++iy) {
+            int gy = base[1] + iy;
+            gy -= gy >= K2 ? K2 : 0;
+            const real *line = phi + ((size_t)gx * K2 + gy) * K3;
+            real l0 = 0, l1 = 0, l2 = 0;
+#pragma unroll
+            for (int iz = 0; iz < P; ++iz) {
+                int gz = base[2] + iz;
+                gz -= gz >= K3 ? K3 : 0;
+                const real v = line[gz];
+                l0 += v * t[2][iz];
+                l1 += v * dt[2][iz];
+                l2 += v * d2t[2][iz];
+            }
+            a_0 += l0 * t[1][iy];
+            a_y += l0 * dt[1][iy];
+            a_z += l1 * t[1][iy];
+            a_yy += l0 * d2t[1][iy];
+            a_zz += l2 * t[1][iy];
+            a_yz += l1 * dt[1][iy];
+        }
+        s100 += (double)(a_0 * dt[0][ix]);
+        s010 += (double)(a_y * t[0][ix]);
+        s001 += (double)(a_z * t[0][ix]);
+        s200 += (double)(a_0 * d2t[0][ix]);
+        s020 += (double)(a_yy * t[0][ix]);
+        s002 += (double)(a_zz * t[0][ix]);
+        s110 += (double)(a_y * dt[0][ix]);
+        s101 += (double)(a_z * dt[0][ix]);
+        s011 += (double)(a_yz * t[0][ix]);
+    }
+    const double gd[3] = {s100, s010, s001};
+    const double qq = (double)q, e0 = (double)e[0], e1 = (double)e[1], e2 = (double)e[2];
+    const double gw[3] = {qq * s100 + e0 * s200 + e1 * s110 + e2 * s101, qq * s010 + e0 * s110 + e1 * s020 + e2 * s011,
+                          qq * s001 + e0 * s101 + e1 * s011 + e2 * s002};
+    const double *es = g.escale;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const double rdd = es[3 * b] * gd[0] + es[3 * b + 1] * gd[1] + es[3 * b + 2] * gd[2];
+        rec_dr[3 * i + b] = es[3 * b] * gw[0] + es[3 * b + 1] * gw[1] + es[3 * b + 2] * gw[2];
+        gp[3 * i + b] = KE * (rowf[3 * i + b] + rdd - cself * dd[3 * i + b]);
+    }
+}
+
+}  // namespace pgm

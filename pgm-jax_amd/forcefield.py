@@ -1,0 +1,218 @@
+"""PGMForceField on the MI355X: the reference's MD force-field entry points over the HIP C ABI.
+
+Mirrors pgm_jax/md/forcefield.py `PGMForceField` (:810-2695) for the qpi + LJ + PME hot path:
+
+    ff = PGMForceField(system, H, MDSettings(precision="mixed"))
+    res = ff.compute(pos, H, None, ff.init_induction())
+    res.energy["total"], res.forces, res.induction.mu, res.iterations, res.residual
+
+Arrays are torch float64 tensors on the GPU (positions, forces, dipoles); numpy inputs are copied to
+the device.  Rows come from the library's own cell list (`idx` of the reference is accepted and
+ignored); they are rebuilt when an atom has moved by more than half the skin since the last build.
+Errors raise PGMError; there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from .settings import PREDICTORS, MDSettings, grid_size
+from .system import System
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise _lib.PGMError("no GPU visible: the HIP path needs an AMD GPU")
+    return torch
+
+
+def _dev(x, shape=None):
+    """Return x as a contiguous float64 tensor on the GPU."""
+    torch = _torch()
+    t = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, float))
+    t = t.to(device="cuda", dtype=torch.float64).contiguous()
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
+    return t
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclasses.dataclass
+class InductionState:
+    """Induced-dipole solver state (forcefield.py:679-713): mu (N,3), hist (4,N,3) newest first, count, norm."""
+
+    mu: object
+    hist: object
+    count: int
+    norm: float
+
+
+@dataclasses.dataclass
+class Result:
+    """Result of compute (forcefield.py:716-746): energy {"elec","vdw","total"} [kJ/mol], forces (N,3) [kJ/mol/nm]."""
+
+    energy: dict
+    forces: object
+    induction: InductionState
+    iterations: int
+    residual: float
+    overflow: bool = False
+
+
+class PGMForceField:
+    """The pGM + LJ force field of one system on the GPU (forcefield.py:810-1000)."""
+
+    def __init__(self, sys: System, H, settings: MDSettings = MDSettings()) -> None:
+        """Build the device context: topology, PME grid (settings or grid_size(H, spacing)), plans."""
+        torch = _torch()
+        self.sys, self.s = sys, settings
+        self.n = sys.n
+        H = np.ascontiguousarray(H, float)
+        self.K = tuple(int(k) for k in (settings.pme_grid or grid_size(H, settings.pme_spacing)))
+        L = _lib.lib()
+        top = _lib.Topology(
+            sys.n, _ptr(sys.q, C.c_double), _ptr(sys.radius, C.c_double), _ptr(sys.alpha, C.c_double),
+            _ptr(sys.lj_rmin_half, C.c_double), _ptr(sys.lj_sqrt_eps, C.c_double), _ptr(sys.mol, C.c_int),
+            len(sys.cov_i), _ptr(sys.cov_i, C.c_int), _ptr(sys.cov_j, C.c_int), _ptr(sys.cov_c, C.c_double),
+        )
+        cs = _lib.Settings()
+        cs.cutoff, cs.skin, cs.ewald_beta = settings.cutoff, settings.skin, settings.ewald_beta
+        cs.grid[:] = list(self.K)
+        cs.order = settings.pme_order
+        cs.lj_lrc = int(settings.lj_lrc)
+        cs.dipole_tol, cs.max_iter = settings.dipole_tol, settings.max_iter
+        cs.predictor = PREDICTORS[settings.predictor]
+        cs.fused, cs.norm_refresh, cs.peek = int(settings.fused), settings.norm_refresh, settings.peek
+        cs.precision = 1 if settings.precision == "mixed" else 0
+        self._stream = torch.cuda.current_stream()
+        ctx = C.c_void_p()
+        _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),
+                                    C.c_void_p(self._stream.cuda_stream)), "pgm_ctx_create")
+        self._ctx = ctx
+        self._H = H.copy()
+        self._built_at = None
+
+    def __del__(self) -> None:
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            try:
+                _lib.lib().pgm_ctx_destroy(ctx)
+            except Exception:  # noqa: BLE001 (interpreter shutdown)
+                pass
+            self._ctx = None
+
+    # ------------------------------------------------------------------ state
+    def init_induction(self) -> InductionState:
+        """Return a fresh solver state: zero dipoles and history, normaliser 1 (forcefield.py:1436-1450)."""
+        torch = _torch()
+        z = torch.zeros((self.n, 3), dtype=torch.float64, device="cuda")
+        return InductionState(z, torch.zeros((4, self.n, 3), dtype=torch.float64, device="cuda"), 0, 1.0)
+
+    def _box(self, H) -> None:
+        if H is None:
+            return
+        H = np.ascontiguousarray(H, float)
+        if not np.array_equal(H, self._H):
+            _lib.check(_lib.lib().pgm_set_box(self._ctx, _ptr(H, C.c_double)), "pgm_set_box")
+            self._H = H.copy()
+            self._built_at = None
+
+    def build_rows(self, pos) -> None:
+        """Rebuild the neighbour rows (cell list, cutoff + skin) at pos."""
+        _lib.check(_lib.lib().pgm_build_rows(self._ctx, C.c_void_p(pos.data_ptr())), "pgm_build_rows")
+        self._built_at = pos.clone()
+
+    def _ensure_rows(self, pos) -> None:
+        torch = _torch()
+        if self._built_at is None:
+            self.build_rows(pos)
+            return
+        disp = torch.max(torch.linalg.norm(pos - self._built_at, dim=1)).item()
+        if disp > 0.5 * self.s.skin:
+            self.build_rows(pos)
+
+    # ------------------------------------------------------------------ entry points
+    def compute(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None) -> Result:
+        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560)."""
+        if params is not None:
+            raise NotImplementedError("parameter pytrees are not part of the HIP path")
+        pos = _dev(pos, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        ind = self.init_induction() if ind is None else ind
+        torch = _torch()
+        forces = torch.empty((self.n, 3), dtype=torch.float64, device="cuda")
+        cind = _lib.Induction(ind.mu.data_ptr(), ind.hist.data_ptr(), ind.count, ind.norm)
+        e = _lib.Energy()
+        it = C.c_int()
+        res = C.c_double()
+        st = _lib.check(_lib.lib().pgm_compute(self._ctx, C.c_void_p(pos.data_ptr()), C.byref(cind), C.byref(e),
+                                               C.c_void_p(forces.data_ptr()), C.byref(it), C.byref(res)), "pgm_compute")
+        new = InductionState(ind.mu, ind.hist, cind.count, cind.norm)
+        energy = {"elec": e.elec, "vdw": e.vdw, "total": e.total}
+        return Result(energy, forces, new, it.value, res.value, st == _lib.PGM_ROW_OVERFLOW)
+
+    def energy_forces(self, pos, mu, H=None) -> tuple[dict, object]:
+        """Return (energy parts, forces) at fixed induced dipoles mu (forcefield.py:2249-2292, :2362-2430)."""
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        mu = _dev(mu, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        forces = torch.empty((self.n, 3), dtype=torch.float64, device="cuda")
+        e = _lib.Energy()
+        _lib.check(_lib.lib().pgm_energy_forces(self._ctx, C.c_void_p(pos.data_ptr()), C.c_void_p(mu.data_ptr()), C.byref(e),
+                                                C.c_void_p(forces.data_ptr())), "pgm_energy_forces")
+        return {"elec": e.elec, "vdw": e.vdw, "total": e.total}, forces
+
+    def energy_fixed_mu(self, pos, H, mu) -> float:
+        """Return the total energy [kJ/mol] with the induced dipoles held at mu (forcefield.py:2249-2292)."""
+        return self.energy_forces(pos, mu, H)[0]["total"]
+
+    def perm_dipoles(self, pos, H=None):
+        """Return the permanent dipoles p (N,3) [e nm] (forcefield.py:1032-1042)."""
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        self._box(H)
+        p = torch.empty((self.n, 3), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().pgm_perm_dipoles(self._ctx, C.c_void_p(pos.data_ptr()), C.c_void_p(p.data_ptr())),
+                   "pgm_perm_dipoles")
+        torch.cuda.current_stream().synchronize()
+        return p
+
+    def field(self, pos, d, with_charges: bool = True, H=None):
+        """Return the total field -dU/dd (N,3) [e/nm^2] of the charges and dipoles d (forcefield.py:1428-1433)."""
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        d = _dev(d, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        out = torch.empty((self.n, 3), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().pgm_field(self._ctx, C.c_void_p(pos.data_ptr()), int(with_charges), C.c_void_p(d.data_ptr()),
+                                        C.c_void_p(out.data_ptr())), "pgm_field")
+        return out
+
+    def pme_energy(self, pos, d, with_charges: bool = True, H=None) -> tuple[float, object]:
+        """Return (E_rec [e^2/nm], dE_rec/dd (N,3)) of PME (pgm_jax/md/pme.py:339-411)."""
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        d = _dev(d, (self.n, 3))
+        self._box(H)
+        g = torch.empty((self.n, 3), dtype=torch.float64, device="cuda")
+        e = C.c_double()
+        _lib.check(_lib.lib().pgm_pme_energy(self._ctx, C.c_void_p(pos.data_ptr()), int(with_charges), C.c_void_p(d.data_ptr()),
+                                             C.byref(e), C.c_void_p(g.data_ptr())), "pgm_pme_energy")
+        return e.value, g
+
+    def row_entries(self) -> int:
+        """Number of stored row entries within the cutoff at the last evaluation (pairs counted twice)."""
+        return int(_lib.lib().pgm_row_entries(self._ctx))

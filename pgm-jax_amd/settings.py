@@ -1,0 +1,83 @@
+"""MDSettings of the hot path: the reference's settings under their flat names.
+
+Mirrors pgm_jax/md/forcefield.py:142-454 (MDSettings with its groups Terms, Cutoffs, NeighborList,
+PMESettings, Induction) restricted to what the electrostatics hot path reads, with the same
+defaults and the same flat names that MDSettings.replace accepts (FLAT_SETTINGS, forcefield.py:331-367).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+PREDICTORS = {"none": 0, "mu3": 1, "mu4": 2, "previous": 3}
+
+
+@dataclasses.dataclass(frozen=True)
+class MDSettings:
+    """Frozen settings; change them with `replace` (flat names).
+
+    cutoff [nm], skin [nm], ewald_beta [1/nm], pme_grid (K1, K2, K3) or None (from pme_spacing [nm]),
+    pme_order, lj_lrc, dipole_tol (max|alpha r| / mean|alpha b|), max_iter, predictor
+    ("mu4", "mu3", "previous", "none"), fused, norm_refresh, peek, precision ("mixed" | "double").
+    """
+
+    elec: str = "qpi"
+    vdw: str = "lj"
+    cutoff: float = 0.9
+    skin: float = 0.1
+    ewald_beta: float = 4.0
+    pme_grid: tuple | None = None
+    pme_spacing: float = 0.08
+    pme_order: int = 6
+    lj_lrc: bool = True
+    dipole_tol: float = 1e-5
+    max_iter: int = 50
+    predictor: str = "mu4"
+    fused: bool = True
+    norm_refresh: int = 1000
+    peek: float = 0.65
+    precision: str = "mixed"
+
+    def __post_init__(self) -> None:
+        """Validate like PGMForceField.__init__ (forcefield.py:917-1000)."""
+        if self.elec != "qpi" or self.vdw != "lj":
+            raise ValueError("the HIP path implements elec='qpi' with vdw='lj'")
+        if self.predictor not in PREDICTORS:
+            raise ValueError(f"unknown predictor {self.predictor!r}")
+        if self.precision not in ("mixed", "double"):
+            raise ValueError(f"precision must be 'mixed' or 'double', got {self.precision!r}")
+        if not (self.cutoff > 0 and self.ewald_beta > 0):
+            raise ValueError("cutoff and ewald_beta must be positive")
+        if not 4 <= int(self.pme_order) <= 8:
+            raise ValueError("pme_order must be in 4..8")
+
+    def replace(self, **changes) -> MDSettings:
+        """Return a copy with some settings changed (TypeError for an unknown name, as the reference)."""
+        names = {f.name for f in dataclasses.fields(self)}
+        for k in changes:
+            if k not in names:
+                raise TypeError(f"MDSettings.replace: unknown setting {k!r}")
+        return dataclasses.replace(self, **changes)
+
+
+def grid_size(H, spacing: float = 0.05, factors=(2, 3, 5)) -> tuple[int, int, int]:
+    """Return the smallest even (2,3,5)-smooth grid with plane spacing <= spacing (pgm_jax/md/pme.py:134-172)."""
+    import numpy as np
+
+    H = np.asarray(H, float)
+    V = abs(np.linalg.det(H))
+    heights = [V / np.linalg.norm(np.cross(H[(a + 1) % 3], H[(a + 2) % 3])) for a in range(3)]
+
+    def smooth(n: int) -> bool:
+        for f in factors:
+            while n % f == 0:
+                n //= f
+        return n == 1
+
+    out = []
+    for h in heights:
+        n = max(8, int(np.ceil(h / spacing)))
+        while not smooth(n) or n % 2:
+            n += 1
+        out.append(n)
+    return tuple(out)

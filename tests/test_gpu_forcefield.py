@@ -1,0 +1,119 @@
+"""HIP force field against the oracle and the reference's golden single points (GPU).
+
+Double precision must reproduce the reference's float64 goldens to rounding (energies 1e-9
+relative, forces 1e-7 of the largest, dipoles 1e-8, the same CG iteration count); mixed precision
+(float32 kernels, FP64 accumulation) must stay within 1e-6 relative in energy and 1e-5 relative
+RMS in forces of the float64 result (north_star asks |dE|/E <= 1e-7 for the mixed path in MD; the
+single-point bound here is the reference's own mixed/double agreement criterion,
+tests/test_water512.py:10-12).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import systems
+from oracle.forcefield import ForceField, Settings
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+SMALL = dict(cutoff=0.6, ewald_beta=6.0, pme_grid=(32, 32, 32), pme_order=6, lj_lrc=True, dipole_tol=1e-10, max_iter=500)
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(top):
+    import pgm_jax_amd as P
+
+    return P.System(q=top.q, radius=top.radius, alpha=top.alpha, lj_rmin_half=top.lj_rmin_half,
+                    lj_sqrt_eps=top.lj_sqrt_eps, masses=top.masses, mol=top.mol, cov_i=top.cov_i,
+                    cov_j=top.cov_j, cov_c=top.cov_c)
+
+
+def _ff(top, H, **kw):
+    import pgm_jax_amd as P
+
+    return P.PGMForceField(_system(top), H, P.MDSettings(**kw))
+
+
+def _rel_rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b**2)))
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_small_box_against_golden(precision):
+    top, pos, H = systems.small_box(0)
+    print(f"small_box precision={precision}", flush=True)
+    res = _ff(top, H, skin=0.05, precision=precision, **SMALL).compute(pos, H)
+    with np.load(os.path.join(GOLD, "ff_small_box.npz")) as g:
+        F = res.forces.cpu().numpy()
+        mu = res.induction.mu.cpu().numpy()
+        if precision == "double":
+            for k in ("elec", "vdw", "total"):
+                assert abs(res.energy[k] - float(g[f"double.energy.{k}"])) < 1e-9 * abs(float(g[f"double.energy.{k}"])), k
+            assert np.abs(F - g["double.forces"]).max() < 1e-7 * np.abs(g["double.forces"]).max()
+            assert np.abs(mu - g["double.mu"]).max() < 1e-8 * np.abs(g["double.mu"]).max()
+            assert res.iterations == int(g["double.iterations"])
+            assert abs(res.residual - float(g["double.residual"])) < 1e-3 * float(g["double.residual"])
+        else:
+            for k in ("elec", "vdw", "total"):
+                ref = float(g[f"double.energy.{k}"])
+                assert abs(res.energy[k] - ref) < 1e-6 * abs(ref), (k, res.energy[k], ref)
+            assert _rel_rms(F, g["double.forces"]) < 1e-5
+            assert _rel_rms(mu, g["double.mu"]) < 1e-5
+            assert abs(res.iterations - int(g["double.iterations"])) <= 1
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_water512_against_golden(precision):
+    top, pos, H, _ = systems.water512()
+    print(f"water512 precision={precision}", flush=True)
+    res = _ff(top, H, precision=precision).compute(pos, H)
+    with np.load(os.path.join(GOLD, "ff_pgm3p25_512.npz")) as g:
+        F = res.forces.cpu().numpy()
+        rt = 1e-9 if precision == "double" else 1e-6
+        for k in ("elec", "vdw", "total"):
+            ref = float(g[f"double.energy.{k}"])
+            assert abs(res.energy[k] - ref) < rt * abs(ref), (k, res.energy[k], ref)
+        assert _rel_rms(F, g["double.forces"]) < (1e-9 if precision == "double" else 1e-5)
+        assert abs(res.iterations - int(g["double.iterations"])) <= (0 if precision == "double" else 1)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_field_and_pme_against_oracle(precision):
+    """The total field of random dipoles (with charges) and the PME energy/gradient vs the oracle."""
+    top, pos, H = systems.small_box(3)
+    rng = np.random.default_rng(5)
+    d = rng.normal(scale=0.01, size=pos.shape)
+    print(f"field precision={precision}", flush=True)
+    o = ForceField(top, H, Settings(**SMALL))
+    g = o.geometry(pos, H)
+    S = o.pme.setup(pos, H)
+    Gk = o.pme.influence(H)
+    ref = o.field(g, S, Gk, top.q, d)
+    ff = _ff(top, H, skin=0.05, precision=precision, **SMALL)
+    got = ff.field(pos, d, True, H).cpu().numpy()
+    tol = 1e-11 if precision == "double" else 2e-6
+    assert _rel_rms(got, ref) < tol
+    e_ref = o.pme.energy(S, Gk, top.q, d)
+    gd_ref, _ = o.pme.gradients(S, Gk, top.q, d)
+    e, gd = ff.pme_energy(pos, d, True, H)
+    assert abs(e - e_ref) < (1e-11 if precision == "double" else 1e-6) * abs(e_ref)
+    assert _rel_rms(gd.cpu().numpy(), gd_ref) < tol
+    p = ff.perm_dipoles(pos, H).cpu().numpy()
+    assert np.abs(p - o.perm_dipoles(pos, H)).max() < 1e-14
+
+
+def test_mixed_vs_double_on_a_water_box():
+    """4,096-water box (config-2 size, 12,288 atoms): mixed vs double energy, forces, dipoles, CG."""
+    top, pos, H = systems.water_box(4096, 4.966, seed=2)
+    kw = dict(pme_grid=(64, 64, 64), dipole_tol=1e-7)
+    print("water_box 4096", flush=True)
+    rd = _ff(top, H, precision="double", **kw).compute(pos, H)
+    rm = _ff(top, H, precision="mixed", **kw).compute(pos, H)
+    Fd, Fm = rd.forces.cpu().numpy(), rm.forces.cpu().numpy()
+    print("E double", rd.energy, "mixed", rm.energy, "it", rd.iterations, rm.iterations, flush=True)
+    assert abs(rm.energy["total"] - rd.energy["total"]) < 1e-6 * abs(rd.energy["total"])
+    assert _rel_rms(Fm, Fd) < 1e-5
+    assert _rel_rms(rm.induction.mu.cpu().numpy(), rd.induction.mu.cpu().numpy()) < 1e-5
+    assert rd.residual <= 1e-7 and rm.residual <= 1e-7
