@@ -119,6 +119,38 @@ int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy 
  * and forces at fixed induced dipoles mu (N,3) f64 device. */
 int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_energy *energy, double *forces);
 
+/* MD state (caller-owned device buffers, float64 (N,3)) and the report of pgm_md_run. */
+typedef struct {
+    double *pos, *vel, *forces;
+    pgm_induction ind;
+    int have_forces; /* forces hold the forces at pos (set by pgm_md_run after the first evaluation) */
+    long long step;
+} pgm_md_state;
+
+typedef struct {
+    double epot, elec, vdw, ekin, etot, temperature;
+    long long cg_iterations_total;
+    int cg_iterations_max, cg_iterations_last;
+    double cg_residual_max;
+    int row_rebuilds;
+} pgm_md_report;
+
+/* Rigid-molecule MD set-up (replaces the rigid engine's constraints, pgm_jax/md/constraints.py:254-299
+ * with MDTopology.rigid): consecutive 3-site molecules, masses (N) host [amu], constraint lengths
+ * taken from the device positions pos (N,3). */
+int pgm_md_setup(pgm_ctx *ctx, const double *masses, const double *pos);
+
+/* nsteps of NVE velocity Verlet with SHAKE/RATTLE in FP64 (Integrator._step / FlexibleIntegrator._step,
+ * pgm_jax/md/integrate.py:743-761, md/flexible.py:776-799), one force evaluation (pgm_compute) per
+ * step, rows rebuilt when an atom moved by more than skin/2 (MDEngine's list update). */
+int pgm_md_run(pgm_ctx *ctx, pgm_md_state *state, int nsteps, double dt, pgm_md_report *report);
+
+/* Measurement (no reference counterpart): average duration [ms] of the hot kernels over reps
+ * launches, timed with HIP events on the context stream, for the dipoles p(pos):
+ * ms[0] block merge of the tiled spread (grid clear without tiles), [1] spread, [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field,
+ * [6] gather + combine, [7] one whole dipole-field matvec.  ms must hold 8 doubles. */
+int pgm_time_kernels(pgm_ctx *ctx, const double *pos, int reps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

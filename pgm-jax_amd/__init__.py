@@ -14,3 +14,7 @@ from .forcefield import InductionState, PGMForceField, Result
 from ._lib import PGMError
 
 __all__ = ["MDSettings", "grid_size", "Molecule", "System", "PGMForceField", "InductionState", "Result", "PGMError"]
+from .md import RigidWaterMD  # noqa: E402
+from . import models  # noqa: E402
+
+__all__ += ["RigidWaterMD", "models"]

@@ -31,6 +31,9 @@ EXPORTS = (
     "pgm_pme_energy",
     "pgm_compute",
     "pgm_energy_forces",
+    "pgm_md_setup",
+    "pgm_md_run",
+    "pgm_time_kernels",
 )
 
 
@@ -84,6 +87,37 @@ class Energy(C.Structure):
     _fields_ = [("elec", C.c_double), ("vdw", C.c_double), ("total", C.c_double)]
 
 
+class MDState(C.Structure):
+    """pgm_md_state."""
+
+    _fields_ = [
+        ("pos", C.c_void_p),
+        ("vel", C.c_void_p),
+        ("forces", C.c_void_p),
+        ("ind", Induction),
+        ("have_forces", C.c_int),
+        ("step", C.c_longlong),
+    ]
+
+
+class MDReport(C.Structure):
+    """pgm_md_report."""
+
+    _fields_ = [
+        ("epot", C.c_double),
+        ("elec", C.c_double),
+        ("vdw", C.c_double),
+        ("ekin", C.c_double),
+        ("etot", C.c_double),
+        ("temperature", C.c_double),
+        ("cg_iterations_total", C.c_longlong),
+        ("cg_iterations_max", C.c_int),
+        ("cg_iterations_last", C.c_int),
+        ("cg_residual_max", C.c_double),
+        ("row_rebuilds", C.c_int),
+    ]
+
+
 class PGMError(RuntimeError):
     """An error status of the HIP library."""
 
@@ -114,6 +148,9 @@ def lib() -> C.CDLL:
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
     L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
+    L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]
+    L.pgm_md_run.argtypes = [P, C.POINTER(MDState), I, D, C.POINTER(MDReport)]
+    L.pgm_time_kernels.argtypes = [P, P, I, C.POINTER(D)]
     for name in EXPORTS:
         if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries"):
             getattr(L, name).restype = I

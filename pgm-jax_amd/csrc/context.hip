@@ -15,6 +15,7 @@
 #include "cg.h"
 #include "common.h"
 #include "pairs.h"
+#include "md.h"
 #include "pme.h"
 #include "pgm_hip.h"
 
@@ -54,6 +55,16 @@ static thread_local std::string g_err;
     } while (0)
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+#define ORDER_SWITCH(P, ...)                                          \
+    switch (P) {                                                      \
+        case 4: { constexpr int PO = 4; __VA_ARGS__; } break;         \
+        case 5: { constexpr int PO = 5; __VA_ARGS__; } break;         \
+        case 6: { constexpr int PO = 6; __VA_ARGS__; } break;         \
+        case 7: { constexpr int PO = 7; __VA_ARGS__; } break;         \
+        case 8: { constexpr int PO = 8; __VA_ARGS__; } break;         \
+        default: g_err = "PME order must be 4..8"; return PGM_ERR_ARG; \
+    }
 
 struct DevBuf {
     void *p = nullptr;
@@ -112,6 +123,10 @@ struct pgm_ctx {
     DevBuf grid, cgrid, Gc, fft_work;
     rocfft_plan fwd = nullptr, bwd = nullptr;
     rocfft_execution_info info = nullptr;
+    // tiled spread (atoms sorted by grid tile)
+    bool tiled = false;
+    PMETiles tiles{};
+    DevBuf pkey, pidx, pskey, psidx, tstart, tend, spl, loc, blocks, pme_sort_tmp;
     // vectors (real4 N) and FP64 (N, 3) work arrays
     DevBuf vx, vr, vz, vp, vAp, vb, vx0, vrow, vdq, vtmp;
     DevBuf pperm, dd, g_el, g_lj, rowf, rec_dr, gp;
@@ -120,6 +135,11 @@ struct pgm_ctx {
     int *h_pin_int = nullptr;
     double *h_pin_dbl = nullptr;
     int last_iters = 4;
+    // MD (rigid 3-site molecules)
+    DevBuf pos_build, minv, mass, d2, md_part, md_fail;
+    int nmol = 0;
+    bool md_ready = false;
+    int rebuilds = 0;
 };
 
 extern "C" const char *pgm_last_error(void) { return g_err.c_str(); }
@@ -259,6 +279,78 @@ static int set_box(pgm_ctx *c, const double *H) {
     return PGM_OK;
 }
 
+// Tile widths of the spread: per axis the smallest T in 8..16 (else 7..P-1) with at least two tiles
+// and a last tile at least P-1 wide; no such T: the atomic spread is used.
+static int setup_tiles(pgm_ctx *c) {
+    const int P = c->s.order;
+    PMETiles t{};
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+        const int K = c->s.grid[a];
+        int best = 0;
+        std::vector<int> cand;
+        for (int T = 8; T <= 16; ++T) cand.push_back(T);
+        for (int T = 7; T >= P - 1; --T) cand.push_back(T);
+        for (int T : cand) {
+            const int nt = (K + T - 1) / T;
+            const int last = K - (nt - 1) * T;
+            if (nt >= 2 && last >= P - 1) {
+                best = T;
+                break;
+            }
+        }
+        if (!best) {
+            ok = false;
+            break;
+        }
+        t.T[a] = best;
+        t.nt[a] = (K + best - 1) / best;
+        t.B[a] = best + P - 1;
+    }
+    c->tiled = ok;
+    if (!ok) return PGM_OK;
+    t.ntiles = t.nt[0] * t.nt[1] * t.nt[2];
+    t.bsize = t.B[0] * t.B[1] * t.B[2];
+    c->tiles = t;
+    const int n = c->n;
+    PGM_TRY(c->pkey.ensure(sizeof(int) * n));
+    PGM_TRY(c->pidx.ensure(sizeof(int) * n));
+    PGM_TRY(c->pskey.ensure(sizeof(int) * n));
+    PGM_TRY(c->psidx.ensure(sizeof(int) * n));
+    PGM_TRY(c->loc.ensure(sizeof(int) * n));
+    PGM_TRY(c->tstart.ensure(sizeof(int) * t.ntiles));
+    PGM_TRY(c->tend.ensure(sizeof(int) * t.ntiles));
+    PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
+    PGM_TRY(c->blocks.ensure(c->rsize * (size_t)t.ntiles * t.bsize));
+    return PGM_OK;
+}
+
+// Sort the atoms by grid tile and store their spline weights (once per geometry).
+template <typename real> static int pme_bin(pgm_ctx *c, const double *pos) {
+    if (!c->tiled) return PGM_OK;
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const PMETiles &t = c->tiles;
+    k_pme_bin<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t, c->pkey.as<int>(), c->pidx.as<int>());
+    LAUNCH_CHECK();
+    int bits = 1;
+    while ((1 << bits) < t.ntiles) ++bits;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
+                                               c->psidx.as<int>(), n, 0, bits, st));
+    PGM_TRY(c->pme_sort_tmp.ensure(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->pme_sort_tmp.p, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
+                                               c->psidx.as<int>(), n, 0, bits, st));
+    k_fill_int<<<nblk(t.ntiles, 256), 256, 0, st>>>(t.ntiles, c->tstart.as<int>(), 0);
+    k_fill_int<<<nblk(t.ntiles, 256), 256, 0, st>>>(t.ntiles, c->tend.as<int>(), 0);
+    k_cell_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->pskey.as<int>(), c->tstart.as<int>(), c->tend.as<int>());
+    LAUNCH_CHECK();
+    ORDER_SWITCH(c->s.order, (k_pme_splines<real, PO><<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t, c->psidx.as<int>(),
+                                                                                    c->spl.as<real>(), c->loc.as<int>())));
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
 static int make_plans(pgm_ctx *c) {
     static bool setup_done = false;
     if (!setup_done) {
@@ -326,6 +418,9 @@ static int build_rows(pgm_ctx *c, const double *pos) {
         const int mc = c->h_pin_int[0];
         if (mc <= c->cap_full) {
             c->rows_valid = true;
+            PGM_TRY(c->pos_build.ensure(sizeof(double) * 3 * (size_t)n));
+            HIP_TRY(hipMemcpyAsync(c->pos_build.p, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
+            c->rebuilds += 1;
             return PGM_OK;
         }
         PGM_TRY(alloc_rows(c, (int)(mc * 1.2) + 16));
@@ -345,32 +440,39 @@ static int prepare(pgm_ctx *c, const double *pos) {
                                                c->cap_full, c->s.cutoff * c->s.cutoff, c->cap, c->rows.as<int>(),
                                                c->count.as<int>());
     LAUNCH_CHECK();
-    return PGM_OK;
+    return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos);
 }
 
 // ---------------------------------------------------------------------------- PME dispatch
-#define ORDER_SWITCH(P, ...)                                          \
-    switch (P) {                                                      \
-        case 4: { constexpr int PO = 4; __VA_ARGS__; } break;         \
-        case 5: { constexpr int PO = 5; __VA_ARGS__; } break;         \
-        case 6: { constexpr int PO = 6; __VA_ARGS__; } break;         \
-        case 7: { constexpr int PO = 7; __VA_ARGS__; } break;         \
-        case 8: { constexpr int PO = 8; __VA_ARGS__; } break;         \
-        default: g_err = "PME order must be 4..8"; return PGM_ERR_ARG; \
-    }
-
 template <typename real>
 static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter) {
     const int n = c->n;
     hipStream_t st = c->stream;
-    k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
     const int P = c->s.order;
+    if (c->tiled) {
+        const PMETiles &t = c->tiles;
+        const size_t lds = c->rsize * t.bsize;
+        if (withq) {
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, WAVE, lds, st>>>(
+                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+        } else {
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+        }
+        LAUNCH_CHECK();
+        ORDER_SWITCH(P, (k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
+                            t, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
+    } else {
+    k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
     if (withq) {
         ORDER_SWITCH(P, (k_spread<real, PO, true><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
                             n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
     } else {
         ORDER_SWITCH(P, (k_spread<real, PO, false><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
                             n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+    }
     }
     LAUNCH_CHECK();
     void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
@@ -663,6 +765,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = c->grid.ensure(c->rsize * c->ngrid)) < 0) return fail(r);
     if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx)) < 0) return fail(r);
     if ((r = c->conv_part.ensure(sizeof(double) * (nblk((long long)c->ncplx, 256) + 16))) < 0) return fail(r);
+    if ((r = setup_tiles(c)) < 0) return fail(r);
     if ((r = set_box(c, H)) < 0) return fail(r);
     // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
     const double rl = s->cutoff + s->skin;
@@ -685,7 +788,10 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
-                     &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy};
+                     &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy,
+                     &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail,
+                     &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->spl, &c->loc, &c->blocks,
+                     &c->pme_sort_tmp};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
     if (c->h_pin_dbl) (void)hipHostFree(c->h_pin_dbl);
@@ -750,6 +856,7 @@ template <typename real>
 static int pme_api(pgm_ctx *c, const double *pos, int withq, const double *d, double *e_rec, double *grad_d) {
     using R4 = real4<real>;
     c->pos = pos;
+    PGM_TRY(pme_bin<real>(c, pos));
     const int nb = nblk(c->n, 256);
     k_pack_dq<real><<<nb, 256, 0, c->stream>>>(c->n, d, nullptr, withq ? c->q.as<double>() : nullptr, c->vdq.as<R4>());
     LAUNCH_CHECK();
@@ -808,4 +915,216 @@ static int ef_api(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *e
 extern "C" int pgm_energy_forces(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces) {
     if (!c || !pos || !mu) return PGM_ERR_ARG;
     return c->mixed ? ef_api<float>(c, pos, mu, energy, forces) : ef_api<double>(c, pos, mu, energy, forces);
+}
+
+// ---------------------------------------------------------------------------- MD
+extern "C" int pgm_md_setup(pgm_ctx *c, const double *masses, const double *pos) {
+    if (!c || !masses || !pos) return PGM_ERR_ARG;
+    const int n = c->n;
+    if (n % 3 != 0) {
+        g_err = "pgm_md_setup: rigid MD needs 3-site molecules";
+        return PGM_ERR_ARG;
+    }
+    for (int i = 0; i < n; ++i)
+        if (c->h_mol[i] != i / 3 || !(masses[i] > 0)) {
+            g_err = "pgm_md_setup: molecules must be consecutive 3-atom groups with positive masses";
+            return PGM_ERR_ARG;
+        }
+    c->nmol = n / 3;
+    std::vector<double> minv(n);
+    for (int i = 0; i < n; ++i) minv[i] = 1.0 / masses[i];
+    PGM_TRY(upload(c->mass, masses, n, c->stream));
+    PGM_TRY(upload(c->minv, minv.data(), n, c->stream));
+    // constraint lengths from the given geometry (squared, per molecule)
+    std::vector<double> hp(3 * (size_t)n), d2(3 * (size_t)c->nmol);
+    HIP_TRY(hipMemcpyAsync(hp.data(), pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int A[3] = {0, 0, 1}, B[3] = {1, 2, 2};
+    for (int m = 0; m < c->nmol; ++m)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (int x = 0; x < 3; ++x) {
+                const double d = hp[3 * (3 * m + A[k]) + x] - hp[3 * (3 * m + B[k]) + x];
+                s += d * d;
+            }
+            d2[3 * m + k] = s;
+        }
+    PGM_TRY(upload(c->d2, d2.data(), d2.size(), c->stream));
+    PGM_TRY(c->md_part.ensure(sizeof(double) * (nblk(n, 256) + 16)));
+    PGM_TRY(c->md_fail.ensure(sizeof(int) * 4));
+    k_fill_int<<<1, 64, 0, c->stream>>>(1, c->md_fail.as<int>(), 0);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->md_ready = true;
+    return PGM_OK;
+}
+
+static int md_kinetic(pgm_ctx *c, double *ke) {
+    const int nb = nblk(c->nmol, 256);
+    k_reduce_partials<<<1, 256, 0, c->stream>>>(c->md_part.as<double>(), nb, 0, c->energy.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 8, c->energy.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *ke = 0.5 * c->h_pin_dbl[8];
+    return PGM_OK;
+}
+
+extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pgm_md_report *rep) {
+    if (!c || !s || !s->pos || !s->vel || !s->forces || nsteps < 0 || !(dt > 0)) return PGM_ERR_ARG;
+    if (!c->md_ready) {
+        g_err = "pgm_md_run: call pgm_md_setup first";
+        return PGM_ERR_STATE;
+    }
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    pgm_energy e{};
+    int it = 0;
+    double res = 0.0;
+    long long it_total = 0;
+    int it_max = 0;
+    double res_max = 0.0;
+    const int rebuilds0 = c->rebuilds;
+    if (!s->have_forces) {
+        const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res)
+                               : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res);
+        if (r < 0) return r;
+        s->have_forces = 1;
+    }
+    int status = PGM_OK;
+    for (int k = 0; k < nsteps; ++k) {
+        k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
+                                                       c->d2.as<double>(), dt, c->md_fail.as<int>());
+        LAUNCH_CHECK();
+        // neighbour rows: rebuild when an atom moved by more than half the skin
+        k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
+        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, c->energy.as<double>() + 3);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, c->energy.as<double>() + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (!c->rows_valid || c->h_pin_dbl[9] > 0.5 * c->s.skin) PGM_TRY(build_rows(c, s->pos));
+        const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res)
+                               : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res);
+        if (r < 0) return r;
+        if (r == PGM_NOT_CONVERGED) status = r;
+        it_total += it;
+        it_max = std::max(it_max, it);
+        res_max = std::max(res_max, res);
+        k_md_second<<<nblk(c->nmol, 256), 256, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
+                                                        c->mass.as<double>(), dt, c->md_part.as<double>());
+        LAUNCH_CHECK();
+        s->step += 1;
+    }
+    double ke = 0.0;
+    if (nsteps > 0) PGM_TRY(md_kinetic(c, &ke));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_int + 4, c->md_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (c->h_pin_int[4] != 0) {
+        g_err = "SHAKE did not converge for some molecule";
+        return PGM_ERR_STATE;
+    }
+    if (rep) {
+        rep->epot = e.total;
+        rep->elec = e.elec;
+        rep->vdw = e.vdw;
+        rep->ekin = ke;
+        rep->etot = e.total + ke;
+        const double dof = 3.0 * n - 3.0 * c->nmol - 3.0;
+        rep->temperature = 2.0 * ke / (dof * 0.0083144626181532);
+        rep->cg_iterations_total = it_total;
+        rep->cg_iterations_max = it_max;
+        rep->cg_iterations_last = it;
+        rep->cg_residual_max = res_max;
+        rep->row_rebuilds = c->rebuilds - rebuilds0;
+    }
+    return status;
+}
+
+// ---------------------------------------------------------------------------- kernel timing
+// Average duration [ms] of each hot kernel over `reps` launches, HIP events on the context stream:
+// out[0] block merge (tiled spread; grid clear otherwise), [1] spread (dipoles), [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field
+// (the CG matvec's pair part), [6] gather + combine, [7] one whole CG matvec (all of the above).
+template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int reps, double *ms) {
+    using R4 = real4<real>;
+    PGM_TRY(prepare(c, pos));
+    hipStream_t st = c->stream;
+    const int n = c->n;
+    k_perm_dipoles<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(), c->cc.as<double>(),
+                                                 c->pperm.as<double>());
+    k_pack_dq<real><<<nblk(n, 256), 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, nullptr, c->vp.as<R4>());
+    LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const R4 *dq = c->vp.as<R4>();
+    const real beta = (real)c->s.ewald_beta;
+    const real cself = (real)c_self(c);
+    const int K3h = c->s.grid[2] / 2 + 1;
+    auto timed = [&](int slot, auto &&launch) -> int {
+        launch();  // warm-up
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) launch();
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        ms[slot] = (double)t / reps;
+        return PGM_OK;
+    };
+    int rc = PGM_OK;
+    if (c->tiled) {
+        const PMETiles &t = c->tiles;
+        const size_t lds = c->rsize * t.bsize;
+        ORDER_SWITCH(c->s.order, rc = timed(1, [&] {
+                                     k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+                                         t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                         c->loc.as<int>(), dq, c->blocks.as<real>(), nullptr, -1);
+                                 }));
+        if (rc < 0) return rc;
+        ORDER_SWITCH(c->s.order, rc = timed(0, [&] {
+                                     k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
+                                         t, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
+                                 }));
+        if (rc < 0) return rc;
+    } else {
+        rc = timed(0, [&] { k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0)); });
+        if (rc < 0) return rc;
+        ORDER_SWITCH(c->s.order, rc = timed(1, [&] {
+                                     k_spread<real, PO, false><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
+                                         n, pos, dq, c->pg, c->grid.as<real>(), nullptr, -1);
+                                 }));
+        if (rc < 0) return rc;
+    }
+    void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
+    rc = timed(2, [&] { rocfft_execute(c->fwd, in, out, c->info); });
+    if (rc < 0) return rc;
+    rc = timed(3, [&] {
+        k_convolve<real, false><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
+            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, nullptr, -1);
+    });
+    if (rc < 0) return rc;
+    void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
+    rc = timed(4, [&] { rocfft_execute(c->bwd, in2, out2, c->info); });
+    if (rc < 0) return rc;
+    rc = timed(5, [&] {
+        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<R4>(), nullptr, -1);
+    });
+    if (rc < 0) return rc;
+    ORDER_SWITCH(c->s.order, rc = timed(6, [&] {
+                                 k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
+                                     n, pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<R4>(), c->alpha_r.as<real>(), nullptr, cself,
+                                     MODE_AV, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1);
+                             }));
+    if (rc < 0) return rc;
+    rc = timed(7, [&] { field_eval<real>(c, dq, false, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), -1); });
+    if (rc < 0) return rc;
+    LAUNCH_CHECK();
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    return PGM_OK;
+}
+
+extern "C" int pgm_time_kernels(pgm_ctx *c, const double *pos, int reps, double *ms) {
+    if (!c || !pos || !ms || reps <= 0) return PGM_ERR_ARG;
+    return c->mixed ? time_impl<float>(c, pos, reps, ms) : time_impl<double>(c, pos, reps, ms);
 }

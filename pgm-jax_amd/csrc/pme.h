@@ -85,6 +85,157 @@ __global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------- tiled spread
+// Atomic-free, deterministic spread.  The grid is cut into tiles (T_a points per axis, every tile at
+// least P-1 wide, at least two tiles per axis); every atom belongs to the tile holding its stencil
+// base.  Atoms are sorted by tile once per geometry (k_pme_bin + radix sort + k_pme_splines, which
+// also stores the spline weights in sorted order).  One wave per tile accumulates its atoms' full
+// P^3 stencils into an LDS block of (T+P-1)^3 points: one atom at a time, the 64 lanes take 64
+// distinct stencil points, so plain LDS read-add-write is race free and the order is fixed.  The
+// blocks are written out and k_merge_blocks sums, for every grid point, the (up to 8) blocks that
+// cover it in a fixed order.
+struct PMETiles {
+    int T[3];    // tile width per axis (the last tile may be narrower, >= P-1)
+    int nt[3];   // tiles per axis
+    int B[3];    // LDS block extent per axis: T + P - 1
+    int ntiles;  // nt0 nt1 nt2
+    int bsize;   // B0 B1 B2
+};
+
+PGM_DEV int tile_size(const PMETiles &t, int a, int u, const PMEGeom &g) {
+    return u < t.nt[a] - 1 ? t.T[a] : g.K[a] - (t.nt[a] - 1) * t.T[a];
+}
+
+// key = linear tile index of each atom's stencil base; idx = atom index.
+__global__ void k_pme_bin(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t, int *__restrict__ key,
+                          int *__restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int base[3];
+    double f[3];
+    stencil<double, 4>(pos, i, g, base, f);
+    int tt[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int b = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
+        int u = b / t.T[a];
+        tt[a] = u < t.nt[a] ? u : t.nt[a] - 1;
+    }
+    key[i] = (tt[0] * t.nt[1] + tt[1]) * t.nt[2] + tt[2];
+    idx[i] = i;
+}
+
+// Spline weights in tile-sorted order: spl[s] = (t0[P], d0[P], t1[P], d1[P], t2[P], d2[P]) of atom
+// sidx[s]; loc[s] = the stencil base relative to its tile start, packed (lx | ly << 10 | lz << 20).
+template <typename real, int P>
+__global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t, const int *__restrict__ sidx,
+                              real *__restrict__ spl, int *__restrict__ loc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = sidx[s];
+    int base[3];
+    real f[3];
+    stencil<real, P>(pos, i, g, base, f);
+    int l[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int b = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
+        int u = b / t.T[a];
+        u = u < t.nt[a] ? u : t.nt[a] - 1;
+        l[a] = b - u * t.T[a];
+        real th[P], dth[P];
+        bspline<real, P, false>(f[a], th, dth, nullptr);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            spl[(size_t)s * 6 * P + (2 * a) * P + k] = th[k];
+            spl[(size_t)s * 6 * P + (2 * a + 1) * P + k] = dth[k];
+        }
+    }
+    loc[s] = l[0] | (l[1] << 10) | (l[2] << 20);
+}
+
+template <typename real, int P, bool WITHQ>
+__global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
+                                                     const int *__restrict__ tend, const int *__restrict__ sidx,
+                                                     const real *__restrict__ spl, const int *__restrict__ loc,
+                                                     const real4<real> *__restrict__ dq, real *__restrict__ blocks,
+                                                     const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    real *blk = reinterpret_cast<real *>(lds_raw);
+    const int lane = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int BY = t.B[1], BZ = t.B[2];
+    for (int k = lane; k < t.bsize; k += WAVE) blk[k] = real(0);
+    __syncthreads();
+    const int s0 = tstart[tile], s1 = tend[tile];
+    constexpr int NP = P * P * P;
+    constexpr int NCH = (NP + WAVE - 1) / WAVE;
+    for (int s = s0; s < s1; ++s) {
+        const real v = lane < 6 * P ? spl[(size_t)s * 6 * P + lane] : real(0);
+        const real4<real> d = dq[sidx[s]];
+        real e[3];
+        scaled_dipole<real>(d, g, e);
+        const real q = WITHQ ? d.w : real(0);
+        const int lc = loc[s];
+        const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int p = c * WAVE + lane;
+            const int pp = p < NP ? p : NP - 1;
+            const int ix = pp / (P * P), iy = (pp / P) % P, iz = pp % P;
+            const real tx = __shfl(v, ix, WAVE), dx = __shfl(v, P + ix, WAVE);
+            const real ty = __shfl(v, 2 * P + iy, WAVE), dy = __shfl(v, 3 * P + iy, WAVE);
+            const real tz = __shfl(v, 4 * P + iz, WAVE), dz = __shfl(v, 5 * P + iz, WAVE);
+            const real w = (q * tx + e[0] * dx) * ty * tz + tx * (e[1] * dy * tz + e[2] * ty * dz);
+            if (p < NP) {
+                const int a = ((lx + ix) * BY + (ly + iy)) * BZ + (lz + iz);
+                blk[a] += w;
+            }
+        }
+    }
+    __syncthreads();
+    real *out = blocks + (size_t)tile * t.bsize;
+    for (int k = lane; k < t.bsize; k += WAVE) out[k] = blk[k];
+}
+
+// Grid point value = sum of the blocks covering it: along each axis its own tile (local l) and,
+// when l <= P-2, the previous tile (local l + width of that tile); fixed order (deterministic).
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMEGeom g, const real *__restrict__ blocks,
+                                                      real *__restrict__ grid, const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    if (idx >= (size_t)K1 * K2 * K3) return;
+    const int gz = (int)(idx % K3), gy = (int)((idx / K3) % K2), gx = (int)(idx / ((size_t)K3 * K2));
+    const int gg[3] = {gx, gy, gz};
+    int u[3][2], l[3][2], nu[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int tt = gg[a] / t.T[a];
+        tt = tt < t.nt[a] ? tt : t.nt[a] - 1;
+        const int la = gg[a] - tt * t.T[a];
+        u[a][0] = tt;
+        l[a][0] = la;
+        nu[a] = 1;
+        if (la <= P - 2) {
+            const int tp = tt > 0 ? tt - 1 : t.nt[a] - 1;
+            u[a][1] = tp;
+            l[a][1] = la + tile_size(t, a, tp, g);
+            nu[a] = 2;
+        }
+    }
+    real acc = real(0);
+    for (int i0 = 0; i0 < nu[0]; ++i0)
+        for (int i1 = 0; i1 < nu[1]; ++i1)
+            for (int i2 = 0; i2 < nu[2]; ++i2) {
+                const int tile = (u[0][i0] * t.nt[1] + u[1][i1]) * t.nt[2] + u[2][i2];
+                acc += blocks[(size_t)tile * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
+            }
+    grid[idx] = acc;
+}
+
 // Convolution on the half spectrum: F[Q] *= Gc; with ENERGY also sum (w3/2) Gc |F[Q]|^2 per block
 // (= sum G |F[Q]|^2 with G of pme.py:243-267), FP64.
 template <typename real, bool ENERGY>

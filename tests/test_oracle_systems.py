@@ -28,3 +28,15 @@ def test_water_box_geometry():
     c = np.einsum("na,na->n", x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]) / oh[:, 0] / oh[:, 1]
     assert np.allclose(np.degrees(np.arccos(c)), 104.52, atol=1e-9)
     assert top.n == 192 and np.allclose(H, np.eye(3) * 1.241)
+
+
+def test_product_builder_matches_oracle_builder():
+    """The product's synthetic water box equals the oracle's (same seed use, same parameters)."""
+    import pgm_jax_amd as P
+
+    s1, p1, H1 = P.models.water_box(64, 1.241, seed=1)
+    t2, p2, H2 = systems.water_box(64, 1.241, seed=1)
+    assert np.allclose(p1, p2, rtol=0, atol=1e-14) and np.array_equal(H1, H2)
+    for f in ("q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps", "masses"):
+        assert np.allclose(getattr(s1, f), getattr(t2, f), rtol=1e-15)
+    assert np.array_equal(s1.cov_i, t2.cov_i) and np.array_equal(s1.cov_j, t2.cov_j) and np.allclose(s1.cov_c, t2.cov_c)
