@@ -145,6 +145,10 @@ int pgm_md_setup(pgm_ctx *ctx, const double *masses, const double *pos);
  * step, rows rebuilt when an atom moved by more than skin/2 (MDEngine's list update). */
 int pgm_md_run(pgm_ctx *ctx, pgm_md_state *state, int nsteps, double dt, pgm_md_report *report);
 
+/* Velocity rescaling v *= factor (constraints are preserved): the velocity-rescaling thermostat
+ * used to equilibrate (pgm_jax/md/thermostats.py's role for this engine). */
+int pgm_md_rescale(pgm_ctx *ctx, pgm_md_state *state, double factor);
+
 /* Measurement (no reference counterpart): average duration [ms] of the hot kernels over reps
  * launches, timed with HIP events on the context stream, for the dipoles p(pos):
  * ms[0] block merge of the tiled spread (grid clear without tiles), [1] spread, [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field,

@@ -33,6 +33,7 @@ EXPORTS = (
     "pgm_energy_forces",
     "pgm_md_setup",
     "pgm_md_run",
+    "pgm_md_rescale",
     "pgm_time_kernels",
 )
 
@@ -151,6 +152,7 @@ def lib() -> C.CDLL:
     L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]
     L.pgm_md_run.argtypes = [P, C.POINTER(MDState), I, D, C.POINTER(MDReport)]
     L.pgm_time_kernels.argtypes = [P, P, I, C.POINTER(D)]
+    L.pgm_md_rescale.argtypes = [P, C.POINTER(MDState), D]
     for name in EXPORTS:
         if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries"):
             getattr(L, name).restype = I

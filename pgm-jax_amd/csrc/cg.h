@@ -245,6 +245,11 @@ __global__ __launch_bounds__(256) void k_cg_update2(int n, const double *__restr
     }
 }
 
+// stats slot = err[it] (the final residual of a solve, kept on the device).
+__global__ void k_record_residual(const double *__restrict__ err, int it, double *__restrict__ slot) {
+    if (threadIdx.x == 0) slot[0] = err[it];
+}
+
 // mu = x + peek alpha r (FP64 out).
 template <typename real>
 __global__ void k_cg_finish(int n, const real4<real> *__restrict__ x, const real4<real> *__restrict__ r,

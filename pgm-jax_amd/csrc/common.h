@@ -149,6 +149,25 @@ template <typename real, int NMAX> PGM_DEV void one_B(real a, real r, real ir, r
         closed_B<real, NMAX>(a, r, ir, ir2, B);
 }
 
+// G_n when both (a r)^2 and (b r)^2 are >= 1 (the closed-form difference only).
+template <typename real, int NMAX> PGM_DEV void pair_G_far(real a, real b, real r2, real *G) {
+    const real ir = rsqrt_(r2);
+    const real r = r2 * ir;
+    const real ir2 = ir * ir;
+    const real sa = a * a * r2, sb = b * b * r2;
+    const real ca = real(TWO_OVER_SQRTPI) * a * exp_(-sa);
+    const real cb = real(TWO_OVER_SQRTPI) * b * exp_(-sb);
+    G[0] = (erfc_(b * r) - erfc_(a * r)) * ir;
+    const real fa = real(2) * a * a, fb = real(2) * b * b;
+    real pa = ca, pb = cb;
+#pragma unroll
+    for (int n = 1; n < NMAX; ++n) {
+        G[n] = (real(2 * n - 1) * G[n - 1] - (pa - pb)) * ir2;
+        pa *= fa;
+        pb *= fb;
+    }
+}
+
 // G_n = B_n[erf(a r)/r] - B_n[erf(b r)/r], n < NMAX.
 template <typename real, int NMAX> PGM_DEV void pair_G(real a, real b, real r2, real *G) {
     const real ir = rsqrt_(r2);

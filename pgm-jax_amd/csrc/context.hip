@@ -95,6 +95,9 @@ struct pgm_ctx {
     int n = 0;
     pgm_settings s{};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;           // second stream: the pair rows run beside the PME chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_read[4] = {nullptr, nullptr, nullptr, nullptr};
     bool mixed = false;
     size_t rsize = 8;  // sizeof(real)
     // box
@@ -115,7 +118,7 @@ struct pgm_ctx {
     // rows
     int cap_full = 0, cap = 0;
     bool rows_valid = false;
-    DevBuf rows_full, count_full, rows, count, maxcount;
+    DevBuf rows_full, count_full, rows, count, count_near, maxcount, G1, G2;
     DevBuf key, atom, skey, satom, cell_start, cell_end, sort_tmp;
     long long entries = 0;
     // PME
@@ -133,8 +136,13 @@ struct pgm_ctx {
     DevBuf partA, partB, partC, pair_part, pack_part, conv_part;
     DevBuf st, rz, err, energy;
     int *h_pin_int = nullptr;
+    int *h_pin_ring = nullptr;   // conv_it read-backs of the pipelined CG (4 slots)
     double *h_pin_dbl = nullptr;
     int last_iters = 4;
+    bool norm_on_device = false;  // CGState.norm holds the normaliser of the last unfused solve
+    DevBuf stats;                 // per-solve final residuals (ring of stats_cap)
+    int stats_cap = 4096;
+    long long stats_slot = 0;
     // MD (rigid 3-site molecules)
     DevBuf pos_build, minv, mass, d2, md_part, md_fail;
     int nmol = 0;
@@ -383,6 +391,8 @@ static int alloc_rows(pgm_ctx *c, int cap) {
     c->cap = cap;
     PGM_TRY(c->rows_full.ensure((size_t)c->n * cap * sizeof(int)));
     PGM_TRY(c->rows.ensure((size_t)c->n * cap * sizeof(int)));
+    PGM_TRY(c->G1.ensure((size_t)c->n * cap * c->rsize));
+    PGM_TRY(c->G2.ensure((size_t)c->n * cap * c->rsize));
     return PGM_OK;
 }
 
@@ -437,8 +447,17 @@ static int prepare(pgm_ctx *c, const double *pos) {
     hipStream_t st = c->stream;
     k_pack_xr<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->R2.as<double>(), c->xr.as<double4>());
     k_compact_rows<<<nblk(n, 4), 256, 0, st>>>(n, c->xr.as<double4>(), c->box, c->rows_full.as<int>(), c->count_full.as<int>(),
-                                               c->cap_full, c->s.cutoff * c->s.cutoff, c->cap, c->rows.as<int>(),
-                                               c->count.as<int>());
+                                               c->cap_full, c->s.cutoff * c->s.cutoff, c->s.ewald_beta * c->s.ewald_beta, c->cap,
+                                               c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>());
+    LAUNCH_CHECK();
+    if (c->mixed)
+        k_pair_tensors<float><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
+            (float)c->s.ewald_beta, c->G1.as<float>(), c->G2.as<float>());
+    else
+        k_pair_tensors<double><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
+            c->s.ewald_beta, c->G1.as<double>(), c->G2.as<double>());
     LAUNCH_CHECK();
     return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos);
 }
@@ -498,17 +517,21 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     const int n = c->n;
     hipStream_t st = c->stream;
     const int *conv = iter >= 0 ? &c->st.as<CGState>()->conv_it : nullptr;
-    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
-    const real beta = (real)c->s.ewald_beta;
+    // fork: the row field on the side stream, concurrently with spread -> FFT -> convolve -> FFT
+    HIP_TRY(hipEventRecord(c->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     if (withq)
-        k_row_field<real, true><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<real4<real>>(),
-            conv, iter);
+        k_row_field<real, true><<<nblk((long long)n * ROWL, 256), 256, 0, c->side>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
+            c->xr.as<double4>(), dq, c->box, c->vrow.as<real4<real>>(), conv, iter);
     else
-        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<real4<real>>(),
-            conv, iter);
+        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, c->side>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
+            c->xr.as<double4>(), dq, c->box, c->vrow.as<real4<real>>(), conv, iter);
     LAUNCH_CHECK();
+    HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
+    HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
     ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
                                  n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
@@ -521,8 +544,13 @@ static double c_self(const pgm_ctx *c) { return 4.0 * std::pow(c->s.ewald_beta, 
 
 // ---------------------------------------------------------------------------- induced dipoles
 // Predictor, initial residual (fused or plain) and CG (pgm_jax/md/forcefield.py:1917-2008, :1714-1821).
+// The host queues iterations without waiting: up to the iteration count of the last solve it does
+// not look at the device at all; from there on, after queueing iteration k it waits for the
+// convergence flag written by iteration k-1 (iteration k keeps the GPU busy meanwhile, and turns
+// into no-ops if k-1 converged).  With `quiet` (the MD loop) nothing else is read back: the
+// residual and the normaliser stay on the device (stats slot `slot`).
 template <typename real>
-static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual) {
+static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual, bool quiet = false) {
     const int n = c->n;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
@@ -551,7 +579,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), x0, q, dq);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, dq, true, MODE_FUSED, x0, r, nullptr, -1));
-        k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
+        if (!quiet || !c->norm_on_device) k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
     } else {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, q, dq);
         LAUNCH_CHECK();
@@ -561,51 +589,76 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, x0, false, MODE_PLAIN, b, r, nullptr, -1));
     }
+    c->norm_on_device = true;
     HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
     k_cg_start<real><<<nb, 256, 0, st>>>(n, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
     k_cg_start_finish<<<1, 256, 0, st>>>(c->partB.as<double>(), nb, S, rz, err, c->s.dipole_tol, c->s.max_iter);
     LAUNCH_CHECK();
     const int max_iter = c->s.max_iter;
+    const int expect = std::max(1, std::min(c->last_iters, max_iter));
     int launched = 0;
-    int conv = 0;
-    HIP_TRY(hipMemcpyAsync(c->h_pin_int, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    conv = c->h_pin_int[0];
-    while (conv > launched && launched < max_iter) {
-        int batch = launched == 0 ? std::max(1, std::min(c->last_iters, max_iter)) : 1;
-        batch = std::min(batch, max_iter - launched);
-        for (int t = 0; t < batch; ++t) {
-            const int k = launched + t;
-            PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
-            k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), nb, rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
-                                                   c->partB.as<double>(), S);
-            k_cg_update2<real><<<nb, 256, 0, st>>>(n, c->partB.as<double>(), nb, rz, err, k, z, p, S, c->s.dipole_tol,
-                                                   max_iter);
-            LAUNCH_CHECK();
+    int conv = 0x7fffffff;  // last conv_it seen on the host
+    auto read_conv = [&](int k) -> int {  // queue the read-back of conv_it after iteration k-1 (slot k)
+        int *slot = c->h_pin_ring + (k & 3);
+        HIP_TRY(hipMemcpyAsync(slot, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(c->ev_read[k & 3], st));
+        return PGM_OK;
+    };
+    PGM_TRY(read_conv(0));  // converged before any iteration?
+    while (launched < max_iter) {
+        const int k = launched;
+        PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
+        k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), nb, rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
+                                               c->partB.as<double>(), S);
+        k_cg_update2<real><<<nb, 256, 0, st>>>(n, c->partB.as<double>(), nb, rz, err, k, z, p, S, c->s.dipole_tol, max_iter);
+        LAUNCH_CHECK();
+        launched = k + 1;
+        PGM_TRY(read_conv(launched));
+        if (launched >= expect - 1 || launched == max_iter) {
+            // wait for the state after iteration k-1 (read slot k) while iteration k runs
+            HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
+            conv = c->h_pin_ring[k & 3];
+            if (conv <= k) break;  // converged before iteration k: k was a no-op
         }
-        launched += batch;
-        HIP_TRY(hipMemcpyAsync(c->h_pin_int, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        conv = c->h_pin_int[0];
+    }
+    if (conv > launched) {  // not yet seen converged: the state after the last launched iteration decides
+        HIP_TRY(hipEventSynchronize(c->ev_read[launched & 3]));
+        conv = c->h_pin_ring[launched & 3];
     }
     const int it = std::min(conv, launched);
     c->last_iters = std::max(1, it);
     k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
     k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
+    k_record_residual<<<1, 64, 0, st>>>(err, it, c->stats.as<double>() + (c->stats_slot % c->stats_cap));
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, err + it, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &S->norm, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (iterations) *iterations = it;
-    if (residual) *residual = c->h_pin_dbl[0];
-    ind->norm = c->h_pin_dbl[1];
+    c->stats_slot += 1;
     ind->count += 1;
-    return (it >= max_iter && c->h_pin_dbl[0] > c->s.dipole_tol) ? PGM_NOT_CONVERGED : PGM_OK;
+    if (iterations) *iterations = it;
+    if (!quiet) {
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, err + it, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &S->norm, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (residual) *residual = c->h_pin_dbl[0];
+        ind->norm = c->h_pin_dbl[1];
+        return (it >= max_iter && c->h_pin_dbl[0] > c->s.dipole_tol) ? PGM_NOT_CONVERGED : PGM_OK;
+    }
+    return (it >= max_iter) ? PGM_NOT_CONVERGED : PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- energy and forces
+static int read_energy(pgm_ctx *c, pgm_energy *energy) {
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (energy) {
+        energy->elec = c->h_pin_dbl[0];
+        energy->vdw = c->h_pin_dbl[1];
+        energy->total = c->h_pin_dbl[2];
+    }
+    return PGM_OK;
+}
+
 template <typename real>
-static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces) {
+static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces, bool quiet = false) {
     const int n = c->n;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
@@ -615,7 +668,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                             c->dd.as<double>(), dq, c->pack_part.as<double>());
     LAUNCH_CHECK();
     const int npair = nblk((long long)n * ROWL, 256);
-    k_pair_forces<real><<<npair, 256, 0, st>>>(n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+    k_pair_forces<real><<<npair, 256, 0, st>>>(n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
                                                (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(),
                                                c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(),
                                                c->pair_part.as<double>());
@@ -637,14 +690,8 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     k_energy_final<<<1, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), nb, c->conv_part.as<double>(),
                                       nblk((long long)c->ncplx, 256), ke_const, tail, c_self(c), c->energy.as<double>());
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (energy) {
-        energy->elec = c->h_pin_dbl[0];
-        energy->vdw = c->h_pin_dbl[1];
-        energy->total = c->h_pin_dbl[2];
-    }
-    return PGM_OK;
+    if (quiet) return PGM_OK;  // energies stay in c->energy
+    return read_energy(c, energy);
 }
 
 // ---------------------------------------------------------------------------- C ABI
@@ -751,12 +798,25 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = c->rz.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
     if ((r = c->err.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
     if ((r = c->energy.ensure(sizeof(double) * 4)) < 0) return fail(r);
+    if ((r = c->stats.ensure(sizeof(double) * c->stats_cap)) < 0) return fail(r);
     if ((r = c->maxcount.ensure(sizeof(int))) < 0) return fail(r);
-    DevBuf *ib[] = {&c->key, &c->atom, &c->skey, &c->satom, &c->count_full, &c->count};
+    DevBuf *ib[] = {&c->key, &c->atom, &c->skey, &c->satom, &c->count_full, &c->count, &c->count_near};
     for (DevBuf *b : ib)
         if ((r = b->ensure(sizeof(int) * n)) < 0) return fail(r);
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        g_err = "stream/event creation failed";
+        return fail(PGM_ERR_HIP);
+    }
+    for (int k = 0; k < 4; ++k)
+        if (hipEventCreateWithFlags(&c->ev_read[k], hipEventDisableTiming) != hipSuccess) {
+            g_err = "event creation failed";
+            return fail(PGM_ERR_HIP);
+        }
     if (hipHostMalloc((void **)&c->h_pin_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_pin_dbl, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_pin_dbl, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pin_ring, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         return fail(PGM_ERR_HIP);
     }
@@ -780,20 +840,27 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (!c) return PGM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     else (void)hipDeviceSynchronize();
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    for (int k = 0; k < 4; ++k)
+        if (c->ev_read[k]) (void)hipEventDestroy(c->ev_read[k]);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fwd) rocfft_plan_destroy(c->fwd);
     if (c->bwd) rocfft_plan_destroy(c->bwd);
     if (c->info) rocfft_execution_info_destroy(c->info);
     DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
-                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->maxcount, &c->key,
+                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->G1, &c->G2, &c->key,
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
                      &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->spl, &c->loc, &c->blocks,
-                     &c->pme_sort_tmp};
+                     &c->pme_sort_tmp, &c->stats};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
+    if (c->h_pin_ring) (void)hipHostFree(c->h_pin_ring);
     if (c->h_pin_dbl) (void)hipHostFree(c->h_pin_dbl);
     delete c;
     return PGM_OK;
@@ -885,14 +952,14 @@ extern "C" int pgm_pme_energy(pgm_ctx *c, const double *pos, int with_charges, c
 
 template <typename real>
 static int compute_api(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces, int *iterations,
-                       double *residual) {
+                       double *residual, bool quiet = false) {
     PGM_TRY(prepare(c, pos));
     k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
                                                             c->cc.as<double>(), c->pperm.as<double>());
     LAUNCH_CHECK();
-    const int rs = solve_impl<real>(c, ind, iterations, residual);
+    const int rs = solve_impl<real>(c, ind, iterations, residual, quiet);
     if (rs < 0) return rs;
-    PGM_TRY(energy_forces_impl<real>(c, ind->mu, energy, forces));
+    PGM_TRY(energy_forces_impl<real>(c, ind->mu, energy, forces, quiet));
     return rs;
 }
 
@@ -982,33 +1049,46 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     double res = 0.0;
     long long it_total = 0;
     int it_max = 0;
-    double res_max = 0.0;
     const int rebuilds0 = c->rebuilds;
+    const long long slot0 = c->stats_slot;
     if (!s->have_forces) {
-        const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res)
-                               : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res);
+        const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true)
+                               : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true);
         if (r < 0) return r;
         s->have_forces = 1;
     }
     int status = PGM_OK;
+    // The displacement test of the rows is read one step late (asynchronously), so it fires at
+    // half the skin minus a margin that covers one step's motion.
+    const double trigger = 0.5 * c->s.skin - 0.01;
+    bool disp_pending = false;
     for (int k = 0; k < nsteps; ++k) {
+        if (disp_pending) {  // the read-back queued last step is complete: the CG waited on later work
+            if (!c->rows_valid || c->h_pin_dbl[9] > trigger) {
+                k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
+                                                               c->d2.as<double>(), dt, c->md_fail.as<int>());
+                LAUNCH_CHECK();
+                PGM_TRY(build_rows(c, s->pos));
+                goto forces;
+            }
+        }
         k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
                                                        c->d2.as<double>(), dt, c->md_fail.as<int>());
         LAUNCH_CHECK();
-        // neighbour rows: rebuild when an atom moved by more than half the skin
+    forces:
         k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
         k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, c->energy.as<double>() + 3);
         LAUNCH_CHECK();
         HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, c->energy.as<double>() + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        if (!c->rows_valid || c->h_pin_dbl[9] > 0.5 * c->s.skin) PGM_TRY(build_rows(c, s->pos));
-        const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res)
-                               : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res);
-        if (r < 0) return r;
-        if (r == PGM_NOT_CONVERGED) status = r;
+        disp_pending = true;
+        {
+            const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true)
+                                   : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true);
+            if (r < 0) return r;
+            if (r == PGM_NOT_CONVERGED) status = r;
+        }
         it_total += it;
         it_max = std::max(it_max, it);
-        res_max = std::max(res_max, res);
         k_md_second<<<nblk(c->nmol, 256), 256, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
                                                         c->mass.as<double>(), dt, c->md_part.as<double>());
         LAUNCH_CHECK();
@@ -1016,8 +1096,19 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     }
     double ke = 0.0;
     if (nsteps > 0) PGM_TRY(md_kinetic(c, &ke));
+    PGM_TRY(read_energy(c, &e));
+    // residuals of the solves of this call (device ring), the normaliser, SHAKE failures
+    const long long nsolve = std::min<long long>(c->stats_slot - slot0, c->stats_cap);
+    double res_max = 0.0;
+    if (nsolve > 0) {
+        std::vector<double> ring((size_t)c->stats_cap);
+        HIP_TRY(hipMemcpy(ring.data(), c->stats.p, sizeof(double) * ring.size(), hipMemcpyDeviceToHost));
+        for (long long j = c->stats_slot - nsolve; j < c->stats_slot; ++j) res_max = std::max(res_max, ring[(size_t)(j % c->stats_cap)]);
+    }
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &c->st.as<CGState>()->norm, sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(c->h_pin_int + 4, c->md_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    s->ind.norm = c->h_pin_dbl[1];
     if (c->h_pin_int[4] != 0) {
         g_err = "SHAKE did not converge for some molecule";
         return PGM_ERR_STATE;
@@ -1037,6 +1128,13 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         rep->row_rebuilds = c->rebuilds - rebuilds0;
     }
     return status;
+}
+
+extern "C" int pgm_md_rescale(pgm_ctx *c, pgm_md_state *s, double factor) {
+    if (!c || !s || !s->vel || !(factor > 0)) return PGM_ERR_ARG;
+    k_scale3<<<nblk(3LL * c->n, 256), 256, 0, c->stream>>>(3 * c->n, s->vel, factor);
+    LAUNCH_CHECK();
+    return PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- kernel timing
@@ -1107,7 +1205,8 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     if (rc < 0) return rc;
     rc = timed(5, [&] {
         k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box, beta, c->vrow.as<R4>(), nullptr, -1);
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
+            c->xr.as<double4>(), dq, c->box, c->vrow.as<R4>(), nullptr, -1);
     });
     if (rc < 0) return rc;
     ORDER_SWITCH(c->s.order, rc = timed(6, [&] {

@@ -185,6 +185,11 @@ __global__ __launch_bounds__(256) void k_max_disp(int n, const double *__restric
     if (threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
+__global__ void k_scale3(int n3, double *__restrict__ v, double f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) v[i] *= f;
+}
+
 // One block: out = reduction of np partials (sum or max).
 __global__ void k_reduce_partials(const double *__restrict__ part, int np, int is_max, double *__restrict__ out) {
     __shared__ double lds[32];

@@ -102,11 +102,15 @@ __global__ void k_build_rows(int n, const double *__restrict__ pos, Box box, int
     atomicMax(maxcount, cnt);
 }
 
-// Compacted rows within rc (order kept): one wave per atom, ballot + prefix count.
+// Compacted rows within rc: one wave per atom, ballot + prefix count, one pass.  "Near" partners
+// (those for which a series branch of the radial kernels may be taken: (a r)^2 or (beta r)^2
+// below 1.05, FP64) fill the row from the front, the others from the back (row[cap - 1 - k]);
+// count_near[i] and count[i] - count_near[i] give the two parts, so the kernels run the far part on
+// the closed form only (no divergent series).  Entry order within each part is the build order.
 __global__ __launch_bounds__(256) void k_compact_rows(int n, const double4 *__restrict__ xr, Box box,
                                                       const int *__restrict__ rows_full, const int *__restrict__ count_full,
-                                                      int cap_full, double rc2, int cap, int *__restrict__ rows,
-                                                      int *__restrict__ count) {
+                                                      int cap_full, double rc2, double beta2, int cap, int *__restrict__ rows,
+                                                      int *__restrict__ count, int *__restrict__ count_near) {
     const int lane = threadIdx.x % WAVE;
     const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (i >= n) return;
@@ -114,24 +118,70 @@ __global__ __launch_bounds__(256) void k_compact_rows(int n, const double4 *__re
     const int cf = count_full[i] < cap_full ? count_full[i] : cap_full;
     const int *rf = rows_full + (size_t)i * cap_full;
     int *rc = rows + (size_t)i * cap;
-    int out = 0;
+    int on = 0, of = 0;
     for (int e0 = 0; e0 < cf; e0 += WAVE) {
         const int e = e0 + lane;
-        bool keep = false;
+        bool kn = false, kf = false;
         int j = 0;
         if (e < cf) {
             j = rf[e];
             const double4 b = xr[j];
             double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
             min_image(dx, dy, dz, box);
-            keep = dx * dx + dy * dy + dz * dz < rc2;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const bool near = r2 < 1.05 * 2.0 * (a.w + b.w) || beta2 * r2 < 1.05;
+            kn = r2 < rc2 && near;
+            kf = r2 < rc2 && !near;
         }
-        const unsigned long long m = __ballot(keep);
-        const int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (keep && out + before < cap) rc[out + before] = j;
-        out += __popcll(m);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const unsigned long long mn = __ballot(kn), mf = __ballot(kf);
+        const int pn = on + __popcll(mn & below), pf = of + __popcll(mf & below);
+        if (kn && pn + of < cap) rc[pn] = j;                 // the far part starts at cap - of
+        if (kf && on + pf < cap) rc[cap - 1 - pf] = j;
+        on += __popcll(mn);
+        of += __popcll(mf);
     }
-    if (lane == 0) count[i] = out < cap ? out : cap;
+    if (lane == 0) {
+        const int tot = on + of < cap ? on + of : cap;
+        count_near[i] = on < tot ? on : tot;
+        count[i] = tot;
+    }
+}
+
+// Pair tensors of the compacted rows, once per geometry: G1 and G2 of every entry (the dipole
+// field needs only these), in the layout of the rows.  ROWL lanes per atom, near part with the
+// series branches, far part with the closed form.
+template <typename real>
+__global__ __launch_bounds__(256) void k_pair_tensors(int n, const int *__restrict__ rows, const int *__restrict__ count,
+                                                      const int *__restrict__ count_near, int cap,
+                                                      const double4 *__restrict__ xr, Box box, real beta,
+                                                      real *__restrict__ G1, real *__restrict__ G2) {
+    const int lane = threadIdx.x % ROWL;
+    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    if (i >= n) return;
+    const double4 a = xr[i];
+    const int c = count[i], cn = count_near[i];
+    const size_t o = (size_t)i * cap;
+    for (int e = lane; e < cn; e += ROWL) {
+        const double4 b = xr[rows[o + e]];
+        double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        min_image(dx, dy, dz, box);
+        const real x = (real)dx, y = (real)dy, z = (real)dz;
+        real G[3];
+        pair_G<real, 3>(rsqrt_(real(2) * (real)(a.w + b.w)), beta, x * x + y * y + z * z, G);
+        G1[o + e] = G[1];
+        G2[o + e] = G[2];
+    }
+    for (int e = cap - (c - cn) + lane; e < cap; e += ROWL) {
+        const double4 b = xr[rows[o + e]];
+        double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        min_image(dx, dy, dz, box);
+        const real x = (real)dx, y = (real)dy, z = (real)dz;
+        real G[3];
+        pair_G_far<real, 3>(rsqrt_(real(2) * (real)(a.w + b.w)), beta, x * x + y * y + z * z, G);
+        G1[o + e] = G[1];
+        G2[o + e] = G[2];
+    }
 }
 
 // ---------------------------------------------------------------------------- field rows
@@ -139,46 +189,55 @@ __global__ __launch_bounds__(256) void k_compact_rows(int n, const double4 *__re
 // (minus the direct-space field at atom i; pgm_jax/md/forcefield.py:1395-1402).
 // dq holds (d_x, d_y, d_z, q) per atom; WITHQ adds the charge term.
 template <typename real, bool WITHQ>
-__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ rows, const int *__restrict__ count, int cap,
+PGM_DEV void row_field_entry(const double4 &a, const double4 &b, const real4<real> &dk, const Box &box, real g1, real g2,
+                             real &ex, real &ey, real &ez) {
+    double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    min_image(dx, dy, dz, box);
+    const real x = (real)dx, y = (real)dy, z = (real)dz;
+    const real dkx = dk.x * x + dk.y * y + dk.z * z;
+    real cf = -g2 * dkx;
+    if (WITHQ) cf -= dk.w * g1;
+    ex += cf * x + g1 * dk.x;
+    ey += cf * y + g1 * dk.y;
+    ez += cf * z + g1 * dk.z;
+}
+
+// out_i = sum_k de_ik/dd_i = sum_k (-q_k G1 - G2 (d_k . x)) x + G1 d_k, x = r_i - r_k
+// (minus the direct-space field at atom i; pgm_jax/md/forcefield.py:1395-1402), with G1, G2 of the
+// rows precomputed by k_pair_tensors.  dq holds (d_x, d_y, d_z, q) per atom; WITHQ adds the
+// charge term.
+template <typename real, bool WITHQ>
+__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ rows, const int *__restrict__ count,
+                                                   const int *__restrict__ count_near, int cap,
+                                                   const real *__restrict__ G1, const real *__restrict__ G2,
                                                    const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
-                                                   real beta, real4<real> *__restrict__ out, const int *__restrict__ conv_it,
-                                                   int iter) {
+                                                   real4<real> *__restrict__ out, const int *__restrict__ conv_it, int iter) {
     if (conv_it && iter >= *conv_it) return;
     const int lane = threadIdx.x % ROWL;
     const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
     if (i >= n) return;
     const double4 a = xr[i];
-    const int c = count[i];
-    const int *row = rows + (size_t)i * cap;
+    const int c = count[i], cn = count_near[i];
+    const size_t o = (size_t)i * cap;
     real ex = 0, ey = 0, ez = 0;
-    for (int e = lane; e < c; e += ROWL) {
-        const int j = row[e];
-        const double4 b = xr[j];
-        double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-        min_image(dx, dy, dz, box);
-        const real x = (real)dx, y = (real)dy, z = (real)dz;
-        const real r2 = x * x + y * y + z * z;
-        const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
-        real G[3];
-        pair_G<real, 3>(aij, beta, r2, G);
-        const real4<real> dk = dq[j];
-        const real dkx = dk.x * x + dk.y * y + dk.z * z;
-        real cf = -G[2] * dkx;
-        if (WITHQ) cf -= dk.w * G[1];
-        ex += cf * x + G[1] * dk.x;
-        ey += cf * y + G[1] * dk.y;
-        ez += cf * z + G[1] * dk.z;
+    for (int e = lane; e < cn; e += ROWL) {
+        const int j = rows[o + e];
+        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[o + e], G2[o + e], ex, ey, ez);
+    }
+    for (int e = cap - (c - cn) + lane; e < cap; e += ROWL) {
+        const int j = rows[o + e];
+        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[o + e], G2[o + e], ex, ey, ez);
     }
     ex = group_sum<ROWL>(ex);
     ey = group_sum<ROWL>(ey);
     ez = group_sum<ROWL>(ez);
     if (lane == 0) {
-        real4<real> o;
-        o.x = ex;
-        o.y = ey;
-        o.z = ez;
-        o.w = 0;
-        out[i] = o;
+        real4<real> v;
+        v.x = ex;
+        v.y = ey;
+        v.z = ez;
+        v.w = 0;
+        out[i] = v;
     }
 }
 
@@ -190,8 +249,49 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
 //   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2.
 // Outputs (FP64): g_el = sum de/dx, g_lj, the row field sum de/dd_i, and per-block energy sums
 // (each pair counted in both rows: the caller halves them).
+#define PAIR_FORCE_ENTRY(GFUN) \
+            const double4 b = xr[j]; \
+            double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z; \
+            min_image(dx, dy, dz, box); \
+            const real x = (real)dx, y = (real)dy, z = (real)dz; \
+            const real r2 = x * x + y * y + z * z; \
+            const real aij = rsqrt_(real(2) * (real)(a.w + b.w)); \
+            real G[4]; \
+            GFUN<real, 4>(aij, beta, r2, G); \
+            const real4<real> dk = dq[j]; \
+            const real qk = dk.w; \
+            const real dix = di.x * x + di.y * y + di.z * z; \
+            const real dkx = dk.x * x + dk.y * y + dk.z * z; \
+            const real didk = di.x * dk.x + di.y * dk.y + di.z * dk.z; \
+            const real t = qi * dkx - qk * dix; \
+            const real ep = qi * qk * G[0] + t * G[1] - G[2] * dix * dkx + G[1] * didk; \
+            const real radial = -qi * qk * G[1] - t * G[2] + G[3] * dix * dkx - G[2] * didk; \
+            const real qiG1 = qi * G[1], qkG1 = qk * G[1]; \
+            gx += (double)(radial * x + qiG1 * dk.x - qkG1 * di.x - G[2] * (di.x * dkx + dk.x * dix)); \
+            gy += (double)(radial * y + qiG1 * dk.y - qkG1 * di.y - G[2] * (di.y * dkx + dk.y * dix)); \
+            gz += (double)(radial * z + qiG1 * dk.z - qkG1 * di.z - G[2] * (di.z * dkx + dk.z * dix)); \
+            ee += (double)ep; \
+            const real cfield = -qk * G[1] - G[2] * dkx; \
+            fx += (double)(cfield * x + G[1] * dk.x); \
+            fy += (double)(cfield * y + G[1] * dk.y); \
+            fz += (double)(cfield * z + G[1] * dk.z); \
+            const double eps = sei * se[j]; \
+            if (mol[j] != mi && eps != 0.0) { \
+                const double rmin = rhi + rh[j]; \
+                const double rr2 = dx * dx + dy * dy + dz * dz; \
+                const double s2 = rmin * rmin / rr2; \
+                const double s6 = s2 * s2 * s2; \
+                el += eps * (s6 * s6 - 2.0 * s6); \
+                const double dl = eps * 12.0 * (s6 - s6 * s6) / rr2; \
+                lx += dl * dx; \
+                ly += dl * dy; \
+                lz += dl * dz; \
+            } \
+        
+
 template <typename real>
-__global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restrict__ rows, const int *__restrict__ count, int cap,
+__global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restrict__ rows, const int *__restrict__ count,
+                                                     const int *__restrict__ count_near, int cap,
                                                      const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
                                                      real beta, const double *__restrict__ rh, const double *__restrict__ se,
                                                      const int *__restrict__ mol, double *__restrict__ g_el,
@@ -210,45 +310,14 @@ __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restric
         const int c = count[i];
         const int *row = rows + (size_t)i * cap;
         double gx = 0, gy = 0, gz = 0, lx = 0, ly = 0, lz = 0, fx = 0, fy = 0, fz = 0, ee = 0, el = 0;
-        for (int e = lane; e < c; e += ROWL) {
+        const int cn = count_near[i];
+        for (int e = lane; e < cn; e += ROWL) {
             const int j = row[e];
-            const double4 b = xr[j];
-            double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-            min_image(dx, dy, dz, box);
-            const real x = (real)dx, y = (real)dy, z = (real)dz;
-            const real r2 = x * x + y * y + z * z;
-            const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
-            real G[4];
-            pair_G<real, 4>(aij, beta, r2, G);
-            const real4<real> dk = dq[j];
-            const real qk = dk.w;
-            const real dix = di.x * x + di.y * y + di.z * z;
-            const real dkx = dk.x * x + dk.y * y + dk.z * z;
-            const real didk = di.x * dk.x + di.y * dk.y + di.z * dk.z;
-            const real t = qi * dkx - qk * dix;
-            const real ep = qi * qk * G[0] + t * G[1] - G[2] * dix * dkx + G[1] * didk;
-            const real radial = -qi * qk * G[1] - t * G[2] + G[3] * dix * dkx - G[2] * didk;
-            const real qiG1 = qi * G[1], qkG1 = qk * G[1];
-            gx += (double)(radial * x + qiG1 * dk.x - qkG1 * di.x - G[2] * (di.x * dkx + dk.x * dix));
-            gy += (double)(radial * y + qiG1 * dk.y - qkG1 * di.y - G[2] * (di.y * dkx + dk.y * dix));
-            gz += (double)(radial * z + qiG1 * dk.z - qkG1 * di.z - G[2] * (di.z * dkx + dk.z * dix));
-            ee += (double)ep;
-            const real cfield = -qk * G[1] - G[2] * dkx;
-            fx += (double)(cfield * x + G[1] * dk.x);
-            fy += (double)(cfield * y + G[1] * dk.y);
-            fz += (double)(cfield * z + G[1] * dk.z);
-            const double eps = sei * se[j];
-            if (mol[j] != mi && eps != 0.0) {
-                const double rmin = rhi + rh[j];
-                const double rr2 = dx * dx + dy * dy + dz * dz;
-                const double s2 = rmin * rmin / rr2;
-                const double s6 = s2 * s2 * s2;
-                el += eps * (s6 * s6 - 2.0 * s6);
-                const double dl = eps * 12.0 * (s6 - s6 * s6) / rr2;
-                lx += dl * dx;
-                ly += dl * dy;
-                lz += dl * dz;
-            }
+            PAIR_FORCE_ENTRY(pair_G)
+        }
+        for (int e = cap - (c - cn) + lane; e < cap; e += ROWL) {
+            const int j = row[e];
+            PAIR_FORCE_ENTRY(pair_G_far)
         }
         gx = group_sum<ROWL>(gx);
         gy = group_sum<ROWL>(gy);

@@ -52,6 +52,19 @@ class RigidWaterMD:
             "row_rebuilds": rep.row_rebuilds, "not_converged": st == _lib.PGM_NOT_CONVERGED,
         }
 
+    def equilibrate(self, nsteps: int, temperature: float, dt_fs: float = 1.0, every: int = 10) -> dict:
+        """Run nsteps with the velocities rescaled to `temperature` [K] every `every` steps; return the last report."""
+        rep = None
+        done = 0
+        while done < nsteps:
+            k = min(every, nsteps - done)
+            rep = self.run(k, dt_fs)
+            done += k
+            if rep["temperature"] > 0:
+                _lib.check(_lib.lib().pgm_md_rescale(self.ff._ctx, C.byref(self._state),
+                                                     (temperature / rep["temperature"]) ** 0.5), "pgm_md_rescale")
+        return rep
+
     def time_kernels(self, reps: int = 20) -> dict:
         """Average duration [ms] of the hot kernels (pgm_time_kernels) at the current positions."""
         ms = (C.c_double * 8)()
