@@ -304,6 +304,64 @@ class ForceField:
         energy, forces = self.energy_forces(pos, H, mu, g, S, Gk)
         return Result(energy, forces, mu, it, err, ind)
 
+    def strain_derivative(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> np.ndarray:
+        """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, molecular scaling, minus the tail impulse E_lrc I.
+
+        forcefield.py:2628-2695 takes it by autodiff of the whole energy (full_strain_derivative,
+        :608-662); here it is written out.  Under x -> x + (com eps^T)[mol], H -> H (1 + eps)^T:
+          pairs: the displacement x_ij moves by eps X_ij, X_ij = x_ij - o_i + o_j (o = r - com):
+                 W += sum de/dx (x) X;
+          PME:   fractional coordinates move by -o eps^T H^-1 and e = d H^-1 K by -d eps^T H^-1 K:
+                 W -= sum dE/dr (x) o + sum dE/dd_rec (x) d; the influence function G(m*, V) gives
+                 W_ab += sum |F Q|^2 G [2 (pi^2/beta^2 + 1/|m*|^2) m*_a m*_b - delta_ab];
+          background and LJ tail scale as 1/V: -E delta_ab each, and the tail impulse -E_lrc I.
+        """
+        t = self.top
+        pos, H = np.asarray(pos, float), np.asarray(H, float)
+        g = self.geometry(pos, H)
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        p = self.perm_dipoles(pos, H)
+        d = p + mu
+        m = t.masses
+        com = np.zeros((t.nmol, 3))
+        np.add.at(com, t.mol, m[:, None] * pos)
+        msum = np.bincount(t.mol, weights=m, minlength=t.nmol)
+        com /= msum[:, None]
+        o = pos - com[t.mol]
+        i, j, x, r = g["i"], g["j"], g["x"], g["r"]
+        X = x - o[i] + o[j]
+        # pair gradients de/dx (electrostatic in KE units, LJ)
+        G0, G1, G2, G3 = g["G"]
+        qi, qk, di, dk = t.q[i], t.q[j], d[i], d[j]
+        dix, dkx, didk = np.sum(di * x, -1), np.sum(dk * x, -1), np.sum(di * dk, -1)
+        tt = qi * dkx - qk * dix
+        radial = -qi * qk * G1 - tt * G2 + G3 * dix * dkx - G2 * didk
+        gx = radial[:, None] * x + (qi * G1)[:, None] * dk - (qk * G1)[:, None] * di - G2[:, None] * (
+            di * dkx[:, None] + dk * dix[:, None])
+        s6 = (g["rmin"] / r) ** 6
+        on = (g["wv"] != 0) & (g["eps"] != 0)
+        dlj = np.where(on, g["eps"] * 12.0 * (s6 - s6 * s6) / (r * r), 0.0)
+        W = KE * gx.T @ X + (dlj[:, None] * x).T @ X
+        # reciprocal space
+        gd, gr = self.pme.gradients(S, Gk, t.q, d)
+        W -= KE * (gr.T @ o + gd.T @ d)
+        FQ = np.fft.rfftn(self.pme.spread(S, t.q, d))
+        F2 = FQ.real**2 + FQ.imag**2
+        R = np.linalg.inv(H).T
+        m1, m2, m3 = self.pme.m
+        mv = m1[:, None, None, None] * R[0] + m2[None, :, None, None] * R[1] + m3[None, None, :, None] * R[2]
+        msq = np.sum(mv * mv, -1)
+        safe = np.where(msq > 0, msq, 1.0)
+        fac = np.where(msq > 0, 2.0 * (np.pi**2 / self.b0**2 + 1.0 / safe), 0.0) * Gk * F2
+        W += KE * (np.einsum("abc,abci,abcj->ij", fac, mv, mv) - np.sum(Gk * F2) * np.eye(3))
+        V = abs(np.linalg.det(H))
+        e_bg = -math.pi * np.sum(t.q) ** 2 / (2.0 * V * self.b0**2) * KE
+        W -= e_bg * np.eye(3)
+        tail = self.vdw_tail(H)
+        W -= 2.0 * tail * np.eye(3)
+        return W
+
     def energy_fixed_mu(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> float:
         """Return the total energy [kJ/mol] at fixed induced dipoles (forcefield.py:2249-2292)."""
         g = self.geometry(pos, H)

@@ -77,3 +77,33 @@ def test_forces_are_minus_the_energy_gradient():
         d[a, c] = h
         fd = -(ff.energy_fixed_mu(pos + d, H, res.mu) - ff.energy_fixed_mu(pos - d, H, res.mu)) / (2 * h)
         assert abs(fd - res.forces[a, c]) < 1e-6 * max(1.0, abs(fd)), (a, c, fd, res.forces[a, c])
+
+
+@pytest.mark.parametrize("case", ["small_box", "water512"])
+def test_strain_derivative_matches_golden(case):
+    """The analytic molecular strain derivative (full 3x3, tail impulse included) vs the golden `strain`."""
+    if case == "small_box":
+        top, pos, H = systems.small_box(0)
+        ff, gold = ForceField(top, H, SMALL), "ff_small_box.npz"
+    else:
+        top, pos, H, _ = systems.water512()
+        ff, gold = ForceField(top, H, Settings()), "ff_pgm3p25_512.npz"
+    res = ff.compute(pos, H)
+    W = ff.strain_derivative(pos, H, res.mu)
+    with np.load(os.path.join(GOLD, gold)) as g:
+        ref = g["double.strain"]
+    assert np.abs(W - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_strain_derivative_trace_by_isotropic_scaling():
+    """tr(W) + 3 E_lrc equals the finite difference of the fixed-mu energy under isotropic molecular scaling."""
+    top, pos, H = systems.small_box(2)
+    ff = ForceField(top, H, SMALL)
+    res = ff.compute(pos, H)
+    W = ff.strain_derivative(pos, H, res.mu)
+    m = top.masses
+    com = np.array([np.average(pos[top.mol == k], 0, weights=m[top.mol == k]) for k in range(top.nmol)])
+    e = lambda s: ff.energy_fixed_mu(pos + (s * com)[top.mol], H * (1 + s), res.mu)  # noqa: E731
+    h = 1e-6
+    fd = (e(h) - e(-h)) / (2 * h)
+    assert abs(fd - (np.trace(W) + 3 * ff.vdw_tail(H))) < 1e-6 * max(1.0, abs(fd))
