@@ -75,6 +75,65 @@ def cpu_baseline(waters: int) -> dict:
     }
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (profiles/), or None.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE per dispatch [KiB], each already divided by the
+    ratio the same counter reads for a copy kernel of known size.
+    """
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        k = d["kernels"][kernel]
+        return round((k["fetch_kib"] + k["write_kib"]) * 1024.0)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline_of(live: dict, kern: dict, entries: int, n: int, K3: int) -> dict:
+    """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
+
+    Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
+    timed steps (row field on its side stream; spread + merge and gather on the main stream).
+
+    Algorithmic bytes per launch (each logical read or write once):
+      row field: per row entry j (4 B) + G1, G2 (FP32, 8 B); per atom the dipole read (16 B) and the
+                 field written (16 B);
+      spread:    per atom positions 12 + q 4 + d 12 B, the FP32 grid written once (K^3 x 4 B) -- the
+                 tiled spread plus its block merge;
+      gather:    the FP32 grid read once (K^3 x 4 B), per atom d in 12 B and the field out 12 B.
+    """
+    row_bytes = entries * 12 + n * 32
+    spread_bytes = n * 28 + K3 * 4
+    gather_bytes = K3 * 4 + n * 24
+    row_ms, spread_ms, gather_ms = live["row_field"]["ms"], live["spread"]["ms"], live["gather"]["ms"]
+
+    def gbs(b, ms):
+        return b / (ms * 1e-3) / 1e9
+
+    return {
+        "kernel": "k_row_field (CG matvec real-space rows, FP32, sliced-ELL j/G1/G2 stream)",
+        "bound": "hbm",
+        "achieved": round(gbs(row_bytes, row_ms), 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs(row_bytes, row_ms) / HBM_PEAK_GBS, 5),
+        "traffic": pmc_traffic("k_row_field"),
+        "algorithmic_bytes": row_bytes,
+        "ms_per_launch": round(row_ms, 5),
+        "launches": live["row_field"]["launches"],
+        "row_entries": entries,
+        "spread": {"kernels": "k_spread_tiles + k_merge_blocks", "achieved": round(gbs(spread_bytes, spread_ms), 2),
+                   "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
+                   "ms_per_launch": round(spread_ms, 5)},
+        "gather": {"kernels": "k_gather_combine", "achieved": round(gbs(gather_bytes, gather_ms), 2),
+                   "frac": round(gbs(gather_bytes, gather_ms) / HBM_PEAK_GBS, 5),
+                   "algorithmic_bytes": gather_bytes, "ms_per_launch": round(gather_ms, 5)},
+        "isolated_ms": {k: round(v, 5) for k, v in kern.items()},
+    }
+
+
 def main() -> int:  # noqa: C901
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,60 +174,56 @@ def main() -> int:  # noqa: C901
     vel = mde.vel.cpu().numpy()
     del mde, ffe
 
-    def run(precision: str, steps: int, warmup: int):
+    def run(precision: str, steps: int, warmup: int, live_timing: bool = False):
         ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, **base))
         md = P.RigidWaterMD(ff, pos, vel)
         log(precision, "warmup", warmup)
         md.run(warmup, a.dt_fs)
         torch.cuda.synchronize()
         log(precision, "timed", steps)
+        if live_timing:
+            ff.set_timing(True)
         t0 = time.perf_counter()
         rep = md.run(steps, a.dt_fs)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        if live_timing:
+            rep["kernel_timing"] = ff.kernel_timing()
+            ff.set_timing(False)
         log(precision, f"{ms:.3f} ms/step, {rep['cg_iterations'] / steps:.2f} CG it/step, T {rep['temperature']:.1f} K, "
             f"Etot {rep['etot']:.3f}, rebuilds {rep['row_rebuilds']}")
         return ff, md, rep, ms
 
-    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup)
+    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, live_timing=True)
+    live = repm["kernel_timing"]
+    log("live kernel timing over the timed steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
-    log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}))
-    ffd, mdd, repd, ms_d = run("double", a.fp64_steps, max(2, a.warmup // 4))
-
-    # mixed vs FP64 at the final mixed configuration (same positions, fresh solves)
-    x = mdm.pos
-    rm = ffm.compute(x)
-    rd = ffd.compute(x)
-    Fm, Fd = rm.forces.cpu().numpy(), rd.forces.cpu().numpy()
-    mum, mud = rm.induction.mu.cpu().numpy(), rd.induction.mu.cpu().numpy()
-    errors = {
-        "energy_rel": abs(rm.energy["total"] - rd.energy["total"]) / abs(rd.energy["total"]),
-        "force_rms_rel": float(np.sqrt(np.mean((Fm - Fd) ** 2)) / np.sqrt(np.mean(Fd**2))),
-        "dipole_rms_rel": float(np.sqrt(np.mean((mum - mud) ** 2)) / np.sqrt(np.mean(mud**2))),
-        "cg_residual_mixed": rm.residual,
-        "cg_residual_fp64": rd.residual,
-    }
-    log("errors vs fp64:", json.dumps(errors))
+    entries = ffm.row_entries()
+    log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}), "row entries", entries)
+    errors, fp64 = None, None
+    if a.fp64_steps > 0:
+        ffd, mdd, repd, ms_d = run("double", a.fp64_steps, max(2, a.warmup // 4))
+        # mixed vs FP64 at the final mixed configuration (same positions, fresh solves)
+        x = mdm.pos
+        rm = ffm.compute(x)
+        rd = ffd.compute(x)
+        Fm, Fd = rm.forces.cpu().numpy(), rd.forces.cpu().numpy()
+        mum, mud = rm.induction.mu.cpu().numpy(), rd.induction.mu.cpu().numpy()
+        errors = {
+            "energy_rel": abs(rm.energy["total"] - rd.energy["total"]) / abs(rd.energy["total"]),
+            "force_rms_rel": float(np.sqrt(np.mean((Fm - Fd) ** 2)) / np.sqrt(np.mean(Fd**2))),
+            "dipole_rms_rel": float(np.sqrt(np.mean((mum - mud) ** 2)) / np.sqrt(np.mean(mud**2))),
+            "cg_residual_mixed": rm.residual,
+            "cg_residual_fp64": rd.residual,
+        }
+        log("errors vs fp64:", json.dumps(errors))
+        fp64 = {"value": round(ns_per_day(ms_d, a.dt_fs), 3), "ms_per_step": round(ms_d, 4), "steps": a.fp64_steps,
+                "cg_iters_per_step": round(repd["cg_iterations"] / a.fp64_steps, 3),
+                "speedup_mixed_vs_fp64": round(ms_d / ms_m, 3)}
 
     n = 3 * a.waters
     K3 = a.grid**3
-    spread_bytes = n * 28 + K3 * 4  # positions 12 + q 4 + d 12 per atom, FP32 grid written once
-    spread_gbs = spread_bytes / (kern["spread"] * 1e-3) / 1e9
-    gather_bytes = K3 * 4 + n * 24
-    gather_gbs = gather_bytes / (kern["gather"] * 1e-3) / 1e9
-    roofline = {
-        "kernel": "k_spread (PME order-6 dipole spread, 98,304 atoms onto 128^3 FP32)",
-        "bound": "hbm",
-        "achieved": round(spread_gbs, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(spread_gbs / HBM_PEAK_GBS, 5),
-        "traffic": None,
-        "algorithmic_bytes": spread_bytes,
-        "ms_per_launch": round(kern["spread"], 5),
-        "gather": {"achieved": round(gather_gbs, 2), "frac": round(gather_gbs / HBM_PEAK_GBS, 5),
-                   "algorithmic_bytes": gather_bytes, "ms_per_launch": round(kern["gather"], 5)},
-    }
+    roofline = roofline_of(live, kern, entries, n, K3)
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,
@@ -190,10 +245,8 @@ def main() -> int:  # noqa: C901
             "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": "previous-step",
         },
         "cg_iters_per_step": round(repm["cg_iterations"] / a.steps, 3),
-        "fp64": {"value": round(ns_per_day(ms_d, a.dt_fs), 3), "ms_per_step": round(ms_d, 4), "steps": a.fp64_steps,
-                 "cg_iters_per_step": round(repd["cg_iterations"] / a.fp64_steps, 3)},
-        "speedup_vs_fp64": round(ms_d / ms_m, 3),
-        "errors_vs_fp64": {k: float(f"{v:.3e}") for k, v in errors.items()},
+        "fp64": fp64,
+        "errors_vs_fp64": None if errors is None else {k: float(f"{v:.3e}") for k, v in errors.items()},
         "energy_drift_check": {"etot_end_kJmol": repm["etot"], "temperature_K": round(repm["temperature"], 2)},
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "roofline": roofline,

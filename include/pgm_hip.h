@@ -61,6 +61,8 @@ typedef struct {
     int norm_refresh;   /* Induction.norm_refresh */
     double peek;        /* Induction.peek */
     int precision;      /* 0 double, 1 mixed (MDSettings.precision) */
+    int recip;          /* reciprocal space: 0 smooth PME, 1 exact Ewald sum (PeriodicPGM, ewald.py:186-470) */
+    double k_tol;       /* Ewald: k-vectors with exp(-k^2/(4 beta^2)) >= k_tol, fixed at the creation box */
 } pgm_settings;
 
 /* Induction state (pgm_jax/md/forcefield.py:679-713 InductionState), caller-owned:
@@ -119,6 +121,22 @@ int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy 
  * and forces at fixed induced dipoles mu (N,3) f64 device. */
 int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_energy *energy, double *forces);
 
+/* PGMForceField.strain_derivative (forcefield.py:2628-2695, with the tail impulse -E_lrc I when
+ * lj_lrc) and PeriodicModel.strain_derivative (periodic.py:236-251, which adds it back through
+ * virial_derivative): dE/d eps (3,3) row-major [kJ/mol] on the host at fixed induced dipoles mu.
+ * molecular != 0: molecules translated with their centres of mass (masses (N,) f64 device;
+ * molecules whole); 0: atomic scaling (masses may be NULL; the covalent dipole frames strain).
+ * Also returns the energy and forces of the same pass (forces may be NULL). */
+int pgm_strain_derivative(pgm_ctx *ctx, const double *pos, const double *mu, const double *masses, int molecular,
+                          pgm_energy *energy, double *forces, double *strain);
+
+/* solver.solve_linear_induction (pgm_jax/solver.py:29-62): mu (n,3) f64 device solving
+ * (T + diag(1/alpha)) mu = F for T (n,n,3,3) f64 device (zero diagonal blocks), alpha (n), F (n,3),
+ * by FP64 Jacobi-preconditioned CG from alpha F until |r| <= tol |F| (the reference solves it by
+ * dense LU).  No context; runs on `stream` (hipStream_t) and synchronises it. */
+int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu, double tol,
+                               int max_iter, void *stream, int *iterations, double *residual);
+
 /* MD state (caller-owned device buffers, float64 (N,3)) and the report of pgm_md_run. */
 typedef struct {
     double *pos, *vel, *forces;
@@ -133,6 +151,7 @@ typedef struct {
     int cg_iterations_max, cg_iterations_last;
     double cg_residual_max;
     int row_rebuilds;
+    double heat; /* Langevin: the O steps' change of kinetic energy [kJ/mol] over the call */
 } pgm_md_report;
 
 /* Rigid-molecule MD set-up (replaces the rigid engine's constraints, pgm_jax/md/constraints.py:254-299
@@ -145,6 +164,12 @@ int pgm_md_setup(pgm_ctx *ctx, const double *masses, const double *pos);
  * step, rows rebuilt when an atom moved by more than skin/2 (MDEngine's list update). */
 int pgm_md_run(pgm_ctx *ctx, pgm_md_state *state, int nsteps, double dt, pgm_md_report *report);
 
+/* Langevin thermostat (Langevin(friction).apply, pgm_jax/md/thermostats.py:195-242, in the g-BAOAB
+ * step of md/flexible.py:776-799): friction [1/ps] (0: NVE), temperature [K], seed of the
+ * counter-based noise (Philox4x32-10 keyed by seed, counter (molecule, step, k)).  Applies to the
+ * following pgm_md_run calls. */
+int pgm_md_set_thermostat(pgm_ctx *ctx, double friction, double temperature, unsigned long long seed);
+
 /* Velocity rescaling v *= factor (constraints are preserved): the velocity-rescaling thermostat
  * used to equilibrate (pgm_jax/md/thermostats.py's role for this engine). */
 int pgm_md_rescale(pgm_ctx *ctx, pgm_md_state *state, double factor);
@@ -154,6 +179,15 @@ int pgm_md_rescale(pgm_ctx *ctx, pgm_md_state *state, double factor);
  * ms[0] block merge of the tiled spread (grid clear without tiles), [1] spread, [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field,
  * [6] gather + combine, [7] one whole dipole-field matvec.  ms must hold 8 doubles. */
 int pgm_time_kernels(pgm_ctx *ctx, const double *pos, int reps, double *ms);
+
+/* Measurement (no reference counterpart): live timing of the CG matvec kernels.  With on != 0,
+ * every following dipole-field evaluation records HIP events around its row field (on the side
+ * stream), its reciprocal spread (tiled spread + merge, or the Ewald structure factors) and its
+ * gather; pgm_set_timing also clears the sums.  pgm_kernel_timing waits for the recorded events
+ * and returns the average duration [ms] and the number of launches of [0] row field,
+ * [1] spread, [2] gather. */
+int pgm_set_timing(pgm_ctx *ctx, int on);
+int pgm_kernel_timing(pgm_ctx *ctx, double *avg_ms, long long *launches);
 
 #ifdef __cplusplus
 }

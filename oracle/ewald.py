@@ -25,6 +25,8 @@ from .forcefield import KE, lj_long_range
 from .kernels import pair_kernels
 from .pairs import min_image
 
+BAR_PER_KJMOL_NM3 = 16.605390671738466  # 1 kJ/mol/nm^3 in bar (units.py:17)
+
 
 def kvector_indices(H: np.ndarray, kcut: float) -> np.ndarray:
     """Return half-space integer triples m with 0 < |2 pi m H^-T| < kcut (ewald.py:129-158)."""
@@ -194,3 +196,91 @@ class EwaldPGM:
         np.add.at(gr, t.cov_i, -tv)
         return -(gr + self.lj(pos)[1])
 
+
+    def strain_derivative(self, pos: np.ndarray, molecular: bool = True) -> np.ndarray:
+        """Return dE_total/d eps (3, 3) [kJ/mol] at mu* (periodic.py:43-81, :236-251), without the tail impulse.
+
+        Written out (the reference takes it by autodiff).  Molecular scaling moves atoms with their
+        molecule's centre of mass (offsets o = r - com fixed); atomic scaling moves every atom
+        (o = 0), so the covalent dipole frames strain too:
+          pairs        sum (KE de/dx + de_LJ/dx) (x) (x - o_i + o_j)
+          reciprocal   -KE sum (dU/dr (x) o + dU/dd (x) d)
+                       + KE sum_k f |S|^2 [2 k k^T (1/(4 b0^2) + 1/k^2) - I]
+          background   -E_bg I;  LJ tail (1/V) -E_lrc I
+          atomic only  sum_cov t (x) v,  t = c (g - (g.u) u)/|v|, g = dE/dd of the frame's owner.
+        """
+        t = self.top
+        pos = np.asarray(pos, float)
+        n = t.n
+        g = self._pairs(pos)
+        p = self.perm_dipoles(pos)
+        mu = self.induced_dipoles(pos)
+        d = p + mu
+        if molecular:
+            m = t.masses
+            com = np.zeros((t.nmol, 3))
+            np.add.at(com, t.mol, m[:, None] * pos)
+            com /= np.bincount(t.mol, weights=m, minlength=t.nmol)[:, None]
+            o = pos - com[t.mol]
+        else:
+            o = np.zeros_like(pos)
+        i, j, x = g["i"], g["j"], g["x"]
+        G0, G1, G2, G3 = g["G"]
+        q = t.q
+        qi, qk, di, dk = q[i], q[j], d[i], d[j]
+        dix, dkx, didk = np.sum(di * x, -1), np.sum(dk * x, -1), np.sum(di * dk, -1)
+        tt = qi * dkx - qk * dix
+        radial = -qi * qk * G1 - tt * G2 + G3 * dix * dkx - G2 * didk
+        gx = radial[:, None] * x + (qi * G1)[:, None] * dk - (qk * G1)[:, None] * di - G2[:, None] * (
+            di * dkx[:, None] + dk * dix[:, None])
+        W = KE * gx.T @ (x - o[i] + o[j])
+        # LJ pairs (intermolecular, within rc)
+        li, lj = np.triu_indices(n, k=1)
+        inter = t.mol[li] != t.mol[lj]
+        li, lj = li[inter], lj[inter]
+        lx = min_image(pos[li] - pos[lj], self.H)
+        lr = np.linalg.norm(lx, axis=-1)
+        keep = lr < self.rc
+        li, lj, lx, lr = li[keep], lj[keep], lx[keep], lr[keep]
+        s6 = ((t.lj_rmin_half[li] + t.lj_rmin_half[lj]) / lr) ** 6
+        eps = t.lj_sqrt_eps[li] * t.lj_sqrt_eps[lj]
+        glj = (eps * 12.0 * (s6 - s6 * s6) / (lr * lr))[:, None] * lx
+        W += glj.T @ (lx - o[li] + o[lj])
+        # reciprocal space
+        k, f, V = self._k()
+        ph = pos @ k.T
+        kd = d @ k.T
+        c, s = np.cos(ph), np.sin(ph)
+        A = np.sum(q[:, None] * c - kd * s, 0)
+        Bs = np.sum(q[:, None] * s + kd * c, 0)
+        _, gd, gr = self._rec(pos, q, d)
+        W -= KE * (gr.T @ o + gd.T @ d)
+        S2 = A * A + Bs * Bs
+        k2 = np.sum(k * k, 1)
+        fac = f * S2 * 2.0 * (1.0 / (4.0 * self.b0**2) + 1.0 / k2)
+        W += KE * (np.einsum("k,ka,kb->ab", fac, k, k) - np.sum(f * S2) * np.eye(3))
+        e_bg = -KE * np.pi * np.sum(q) ** 2 / (2.0 * V * self.b0**2)
+        W -= e_bg * np.eye(3)
+        if self.lj_lrc:
+            W -= lj_long_range(t.lj_rmin_half, t.lj_sqrt_eps, V, self.rc) * np.eye(3)
+        if not molecular:  # covalent frames strain with the atoms
+            _, gdd, _ = self.U(pos, q, d, g)
+            v = pos[t.cov_j] - pos[t.cov_i]
+            L = np.linalg.norm(v, axis=-1, keepdims=True)
+            u = v / L
+            gi = gdd[t.cov_i]
+            tv = t.cov_c[:, None] * (gi - np.sum(gi * u, -1, keepdims=True) * u) / L
+            W += tv.T @ v
+        return W
+
+    def virial_derivative(self, pos: np.ndarray, molecular: bool = True) -> np.ndarray:
+        """Return the strain derivative plus the LJ tail impulse -E_lrc I (periodic.py:253-268, lj.py:190-201)."""
+        W = self.strain_derivative(pos, molecular)
+        if self.lj_lrc:
+            W = W - lj_long_range(self.top.lj_rmin_half, self.top.lj_sqrt_eps, abs(np.linalg.det(self.H)), self.rc) * np.eye(3)
+        return W
+
+    def pressure(self, pos: np.ndarray, molecular: bool = True) -> float:
+        """Return the static pressure -tr(virial_derivative)/(3V) [bar] (periodic.py:84-106, :270-281)."""
+        V = abs(np.linalg.det(self.H))
+        return float(-np.trace(self.virial_derivative(pos, molecular)) / (3.0 * V) * BAR_PER_KJMOL_NM3)

@@ -304,8 +304,12 @@ class ForceField:
         energy, forces = self.energy_forces(pos, H, mu, g, S, Gk)
         return Result(energy, forces, mu, it, err, ind)
 
-    def strain_derivative(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> np.ndarray:
-        """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, molecular scaling, minus the tail impulse E_lrc I.
+    def strain_derivative(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, molecular: bool = True) -> np.ndarray:
+        """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, minus the tail impulse E_lrc I.
+
+        molecular=False: atomic scaling (every atom r -> (1 + eps) r): o = 0 below, and the covalent
+        dipole frames strain, adding sum_cov t (x) v with t = c (g - (g.u) u)/|v|, g = dE/dd of the
+        frame's owner atom (the pull-back of perm_dipoles_vjp, applied to eps v instead of dr).
 
         forcefield.py:2628-2695 takes it by autodiff of the whole energy (full_strain_derivative,
         :608-662); here it is written out.  Under x -> x + (com eps^T)[mol], H -> H (1 + eps)^T:
@@ -328,7 +332,7 @@ class ForceField:
         np.add.at(com, t.mol, m[:, None] * pos)
         msum = np.bincount(t.mol, weights=m, minlength=t.nmol)
         com /= msum[:, None]
-        o = pos - com[t.mol]
+        o = pos - com[t.mol] if molecular else np.zeros_like(pos)
         i, j, x, r = g["i"], g["j"], g["x"], g["r"]
         X = x - o[i] + o[j]
         # pair gradients de/dx (electrostatic in KE units, LJ)
@@ -360,6 +364,14 @@ class ForceField:
         W -= e_bg * np.eye(3)
         tail = self.vdw_tail(H)
         W -= 2.0 * tail * np.eye(3)
+        if not molecular:
+            gdd = KE * (self.row_field(g, t.q, d) + gd - self.c_self * d)
+            v = min_image(pos[t.cov_j] - pos[t.cov_i], H)
+            L = np.linalg.norm(v, axis=-1, keepdims=True)
+            u = v / L
+            gi = gdd[t.cov_i]
+            tv = t.cov_c[:, None] * (gi - np.sum(gi * u, -1, keepdims=True) * u) / L
+            W += tv.T @ v
         return W
 
     def energy_fixed_mu(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> float:

@@ -14,7 +14,8 @@ from .forcefield import InductionState, PGMForceField, Result
 from ._lib import PGMError
 
 __all__ = ["MDSettings", "grid_size", "Molecule", "System", "PGMForceField", "InductionState", "Result", "PGMError"]
-from .md import RigidWaterMD  # noqa: E402
-from . import models  # noqa: E402
+from .md import Langevin, RigidWaterMD  # noqa: E402
+from .periodic import PeriodicModel, PeriodicPGM, pressure_bar  # noqa: E402
+from . import models, solver  # noqa: E402
 
-__all__ += ["RigidWaterMD", "models"]
+__all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "models", "solver"]

@@ -31,10 +31,15 @@ EXPORTS = (
     "pgm_pme_energy",
     "pgm_compute",
     "pgm_energy_forces",
+    "pgm_strain_derivative",
+    "pgm_solve_linear_induction",
     "pgm_md_setup",
     "pgm_md_run",
+    "pgm_md_set_thermostat",
     "pgm_md_rescale",
     "pgm_time_kernels",
+    "pgm_set_timing",
+    "pgm_kernel_timing",
 )
 
 
@@ -73,6 +78,8 @@ class Settings(C.Structure):
         ("norm_refresh", C.c_int),
         ("peek", C.c_double),
         ("precision", C.c_int),
+        ("recip", C.c_int),
+        ("k_tol", C.c_double),
     ]
 
 
@@ -116,6 +123,7 @@ class MDReport(C.Structure):
         ("cg_iterations_last", C.c_int),
         ("cg_residual_max", C.c_double),
         ("row_rebuilds", C.c_int),
+        ("heat", C.c_double),
     ]
 
 
@@ -149,10 +157,15 @@ def lib() -> C.CDLL:
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
     L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
+    L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
+    L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]
     L.pgm_md_run.argtypes = [P, C.POINTER(MDState), I, D, C.POINTER(MDReport)]
+    L.pgm_md_set_thermostat.argtypes = [P, D, D, C.c_ulonglong]
     L.pgm_time_kernels.argtypes = [P, P, I, C.POINTER(D)]
     L.pgm_md_rescale.argtypes = [P, C.POINTER(MDState), D]
+    L.pgm_set_timing.argtypes = [P, I]
+    L.pgm_kernel_timing.argtypes = [P, C.POINTER(D), C.POINTER(C.c_longlong)]
     for name in EXPORTS:
         if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries"):
             getattr(L, name).restype = I

@@ -17,6 +17,9 @@
 #include "pairs.h"
 #include "md.h"
 #include "pme.h"
+#include "virial.h"
+#include "ewald.h"
+#include "dense.h"
 #include "pgm_hip.h"
 
 using namespace pgm;
@@ -118,9 +121,15 @@ struct pgm_ctx {
     // rows
     int cap_full = 0, cap = 0;
     bool rows_valid = false;
-    DevBuf rows_full, count_full, rows, count, count_near, maxcount, G1, G2;
+    DevBuf rows_full, count_full, rows, count, count_near, maxcount, jell, G1, G2;
     DevBuf key, atom, skey, satom, cell_start, cell_end, sort_tmp;
     long long entries = 0;
+    // reciprocal space: PME, or (ewald) the exact Ewald sum over the half-space index set m
+    // (fixed at the reference box) with kvec = (k, f_k) recomputed per box
+    bool ewald = false;
+    int nk = 0;
+    std::vector<int> h_m;
+    DevBuf kvec, sk;
     // PME
     size_t ngrid = 0, ncplx = 0;
     DevBuf grid, cgrid, Gc, fft_work;
@@ -143,9 +152,26 @@ struct pgm_ctx {
     DevBuf stats;                 // per-solve final residuals (ring of stats_cap)
     int stats_cap = 4096;
     long long stats_slot = 0;
+    // strain derivative: molecules as CSR (atoms of molecule m in index order), centres of mass,
+    // offsets from them, dU_rec/dd and per-block partials
+    int nmol_all = 0;
+    DevBuf mol_ptr, mol_atoms, com, off, rec_dd, vir_pair, vir_conv, vir_atoms, vir_cov, vir_out;
+    int ncov = 0;
+    // live kernel timing of the CG matvecs (pgm_set_timing): HIP event pairs around the row field
+    // (side stream), the spread (+ merge) and the gather (main stream)
+    struct Timed {
+        std::vector<hipEvent_t> ev;  // 2 per launch
+        int used = 0;
+        double sum_ms = 0.0;
+        long long count = 0;
+    };
+    bool timing = false;
+    Timed timed[3];
     // MD (rigid 3-site molecules)
-    DevBuf pos_build, minv, mass, d2, md_part, md_fail;
+    DevBuf pos_build, minv, mass, d2, md_part, md_fail, heat;
     int nmol = 0;
+    double friction = 0.0, kT = 0.0;  // Langevin O step (friction 0: NVE)
+    unsigned long long seed = 0;
     bool md_ready = false;
     int rebuilds = 0;
 };
@@ -282,6 +308,18 @@ static int set_box(pgm_ctx *c, const double *H) {
     } else {
         PGM_TRY(upload(c->Gc, G.data(), G.size(), c->stream));
     }
+    if (c->ewald) {  // k = 2 pi H^-1 m, f_k = (4 pi / V) exp(-k^2 / (4 beta^2)) / k^2 (pgm_jax/ewald.py:317-333)
+        std::vector<double4> kv((size_t)std::max(c->nk, 1));
+        for (int k = 0; k < c->nk; ++k) {
+            const double m0 = c->h_m[3 * k], m1 = c->h_m[3 * k + 1], m2 = c->h_m[3 * k + 2];
+            const double kx = 2.0 * M_PI * (Hi[0] * m0 + Hi[1] * m1 + Hi[2] * m2);
+            const double ky = 2.0 * M_PI * (Hi[3] * m0 + Hi[4] * m1 + Hi[5] * m2);
+            const double kz = 2.0 * M_PI * (Hi[6] * m0 + Hi[7] * m1 + Hi[8] * m2);
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            kv[k] = make_double4(kx, ky, kz, (4.0 * M_PI / c->volume) * std::exp(-k2 / (4.0 * beta * beta)) / k2);
+        }
+        PGM_TRY(upload(c->kvec, kv.data(), kv.size(), c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->rows_valid = false;
     return PGM_OK;
@@ -391,8 +429,10 @@ static int alloc_rows(pgm_ctx *c, int cap) {
     c->cap = cap;
     PGM_TRY(c->rows_full.ensure((size_t)c->n * cap * sizeof(int)));
     PGM_TRY(c->rows.ensure((size_t)c->n * cap * sizeof(int)));
-    PGM_TRY(c->G1.ensure((size_t)c->n * cap * c->rsize));
-    PGM_TRY(c->G2.ensure((size_t)c->n * cap * c->rsize));
+    const size_t ell_n = (size_t)((cap + RL - 1) / RL) * RL * c->n;
+    PGM_TRY(c->jell.ensure(ell_n * sizeof(int)));
+    PGM_TRY(c->G1.ensure(ell_n * c->rsize));
+    PGM_TRY(c->G2.ensure(ell_n * c->rsize));
     return PGM_OK;
 }
 
@@ -451,23 +491,69 @@ static int prepare(pgm_ctx *c, const double *pos) {
                                                c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>());
     LAUNCH_CHECK();
     if (c->mixed)
-        k_pair_tensors<float><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+        k_pair_tensors<float><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
-            (float)c->s.ewald_beta, c->G1.as<float>(), c->G2.as<float>());
+            (float)c->s.ewald_beta, c->jell.as<int>(), c->G1.as<float>(), c->G2.as<float>());
     else
-        k_pair_tensors<double><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
+        k_pair_tensors<double><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
-            c->s.ewald_beta, c->G1.as<double>(), c->G2.as<double>());
+            c->s.ewald_beta, c->jell.as<int>(), c->G1.as<double>(), c->G2.as<double>());
     LAUNCH_CHECK();
+    if (c->ewald) return PGM_OK;
     return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos);
+}
+
+// Live timing: fold the recorded event pairs into the sums (waits for them), then free the slots.
+// Launches queued past the CG's convergence return at once (a few microseconds); a launch counts
+// only if it took at least a fifth of the longest one of its batch.
+static int harvest_timing(pgm_ctx *c) {
+    for (auto &t : c->timed) {
+        std::vector<float> d((size_t)t.used);
+        float mx = 0.f;
+        for (int k = 0; k < t.used; ++k) {
+            HIP_TRY(hipEventSynchronize(t.ev[2 * k + 1]));
+            HIP_TRY(hipEventElapsedTime(&d[k], t.ev[2 * k], t.ev[2 * k + 1]));
+            mx = std::max(mx, d[k]);
+        }
+        for (int k = 0; k < t.used; ++k)
+            if (d[k] >= 0.2f * mx) {
+                t.sum_ms += d[k];
+                t.count += 1;
+            }
+        t.used = 0;
+    }
+    return PGM_OK;
+}
+
+static constexpr int TIMED_CAP = 1024;
+
+// Record the start event of kernel slot `which` on `st`; returns the pair index, or -1 when off.
+static int timing_begin(pgm_ctx *c, int which, hipStream_t st) {
+    if (!c->timing) return -1;
+    auto &t = c->timed[which];
+    if (t.ev.empty()) {
+        t.ev.resize(2 * TIMED_CAP);
+        for (auto &e : t.ev)
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+    }
+    if (t.used == TIMED_CAP && harvest_timing(c) < 0) return -1;
+    const int k = t.used++;
+    if (hipEventRecord(t.ev[2 * k], st) != hipSuccess) return -1;
+    return k;
+}
+
+static void timing_end(pgm_ctx *c, int which, int k, hipStream_t st) {
+    if (k >= 0) (void)hipEventRecord(c->timed[which].ev[2 * k + 1], st);
 }
 
 // ---------------------------------------------------------------------------- PME dispatch
 template <typename real>
-static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter) {
+static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter,
+                      double *vir_part = nullptr) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int P = c->s.order;
+    const int t_sp = energy ? -1 : timing_begin(c, 1, st);
     if (c->tiled) {
         const PMETiles &t = c->tiles;
         const size_t lds = c->rsize * t.bsize;
@@ -494,9 +580,19 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
     }
     }
     LAUNCH_CHECK();
+    timing_end(c, 1, t_sp, st);
     void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
     FFT_TRY(rocfft_execute(c->fwd, in, out, c->info));
     const int K3h = c->s.grid[2] / 2 + 1;
+    if (vir_part) {  // influence-function part of the strain derivative, from the unconvolved transform
+        RecipGeom rg{};
+        for (int k = 0; k < 9; ++k) rg.hinv[k] = c->pg.hinv[k];
+        for (int a = 0; a < 3; ++a) rg.K[a] = c->s.grid[a];
+        rg.pb2 = M_PI * M_PI / (c->s.ewald_beta * c->s.ewald_beta);
+        k_convolve_virial<real><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(c->ncplx, rg, c->cgrid.as<real>(),
+                                                                                  c->Gc.as<real>(), vir_part);
+        LAUNCH_CHECK();
+    }
     if (energy)
         k_convolve<real, true><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
             c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), c->conv_part.as<double>(), conv, iter);
@@ -520,23 +616,46 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     // fork: the row field on the side stream, concurrently with spread -> FFT -> convolve -> FFT
     HIP_TRY(hipEventRecord(c->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    const int t_row = timing_begin(c, 0, c->side);
     if (withq)
-        k_row_field<real, true><<<nblk((long long)n * ROWL, 256), 256, 0, c->side>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
-            c->xr.as<double4>(), dq, c->box, c->vrow.as<real4<real>>(), conv, iter);
+        k_row_field<real, true><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
+            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+            c->vrow.as<real4<real>>(), conv, iter);
     else
-        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, c->side>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
-            c->xr.as<double4>(), dq, c->box, c->vrow.as<real4<real>>(), conv, iter);
+        k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
+            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+            c->vrow.as<real4<real>>(), conv, iter);
     LAUNCH_CHECK();
+    timing_end(c, 0, t_row, c->side);
     HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
+    if (c->ewald) {
+        const int t_sp = timing_begin(c, 1, st);
+        if (withq)
+            k_ewald_sk<real, true><<<nblk(c->nk, EW_TILE), EW_TILE, 0, st>>>(c->nk, c->kvec.as<double4>(), n, c->pos, dq,
+                                                                             c->sk.as<double2>(), nullptr, conv, iter);
+        else
+            k_ewald_sk<real, false><<<nblk(c->nk, EW_TILE), EW_TILE, 0, st>>>(c->nk, c->kvec.as<double4>(), n, c->pos, dq,
+                                                                              c->sk.as<double2>(), nullptr, conv, iter);
+        LAUNCH_CHECK();
+        timing_end(c, 1, t_sp, st);
+        HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+        const int t_ga = timing_begin(c, 2, st);
+        k_ewald_gather_combine<real><<<nblk(n, EW_TILE), EW_TILE, 0, st>>>(
+            n, c->nk, c->kvec.as<double4>(), c->sk.as<double2>(), c->pos, dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+            extra, cself, mode, out, part, conv, iter);
+        LAUNCH_CHECK();
+        timing_end(c, 2, t_ga, st);
+        return PGM_OK;
+    }
     PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
     HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
-    const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
+    const int t_ga = timing_begin(c, 2, st);
     ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
                                  n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
                                  extra, cself, mode, out, part, conv, iter)));
     LAUNCH_CHECK();
+    timing_end(c, 2, t_ga, st);
     return PGM_OK;
 }
 
@@ -658,7 +777,8 @@ static int read_energy(pgm_ctx *c, pgm_energy *energy) {
 }
 
 template <typename real>
-static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces, bool quiet = false) {
+static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces, bool quiet = false,
+                              const double *masses = nullptr, double *strain = nullptr, bool molecular = true) {
     const int n = c->n;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
@@ -668,16 +788,77 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                             c->dd.as<double>(), dq, c->pack_part.as<double>());
     LAUNCH_CHECK();
     const int npair = nblk((long long)n * ROWL, 256);
-    k_pair_forces<real><<<npair, 256, 0, st>>>(n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-                                               (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(),
-                                               c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(),
-                                               c->pair_part.as<double>());
+    const int nconv = c->ewald ? nblk(c->nk, EW_TILE) : nblk((long long)c->ncplx, 256);
+    const bool vir = strain != nullptr;
+    if (vir) {
+        PGM_TRY(c->com.ensure(sizeof(double) * 3 * (size_t)c->nmol_all));
+        PGM_TRY(c->off.ensure(sizeof(double) * 3 * (size_t)n));
+        PGM_TRY(c->rec_dd.ensure(sizeof(double) * 3 * (size_t)n));
+        PGM_TRY(c->vir_pair.ensure(sizeof(double) * 9 * (size_t)npair));
+        PGM_TRY(c->vir_conv.ensure(sizeof(double) * 7 * (size_t)nconv));
+        PGM_TRY(c->vir_atoms.ensure(sizeof(double) * 9 * (size_t)nb));
+        PGM_TRY(c->vir_cov.ensure(sizeof(double) * 9 * (size_t)(nblk(c->ncov, 256) + 1)));
+        PGM_TRY(c->vir_out.ensure(sizeof(double) * 9));
+        if (molecular) {
+            k_com<<<nblk(c->nmol_all, 256), 256, 0, st>>>(c->nmol_all, c->mol_ptr.as<int>(), c->mol_atoms.as<int>(), c->pos,
+                                                           masses, c->com.as<double>());
+            LAUNCH_CHECK();
+            k_offsets<<<nb, 256, 0, st>>>(n, c->pos, c->mol.as<int>(), c->com.as<double>(), c->off.as<double>());
+        } else {  // atomic scaling: every atom its own centre
+            k_fill<double><<<nblk(3LL * n, 256), 256, 0, st>>>(3 * (size_t)n, c->off.as<double>(), 0.0);
+        }
+        LAUNCH_CHECK();
+        k_pair_forces<real, true><<<npair, 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+            (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
+            c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), c->off.as<double>(),
+            c->vir_pair.as<double>());
+    } else {
+        k_pair_forces<real, false><<<npair, 256, 0, st>>>(
+            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+            (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
+            c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
+    }
     LAUNCH_CHECK();
-    PGM_TRY(spread_fft<real>(c, dq, true, true, nullptr, -1));
-    ORDER_SWITCH(c->s.order, (k_gather_forces<real, PO><<<nb, 256, 0, st>>>(n, c->pos, c->pg, c->grid.as<real>(), dq,
-                                                                           c->dd.as<double>(), c->rowf.as<double>(), c_self(c),
-                                                                           c->rec_dr.as<double>(), c->gp.as<double>())));
+    if (c->ewald) {
+        k_ewald_sk<real, true><<<nconv, EW_TILE, 0, st>>>(c->nk, c->kvec.as<double4>(), n, c->pos, dq, c->sk.as<double2>(),
+                                                          c->conv_part.as<double>(), nullptr, -1);
+        LAUNCH_CHECK();
+        if (vir) {
+            k_ewald_virial<<<nconv, 256, 0, st>>>(c->nk, c->kvec.as<double4>(), c->sk.as<double2>(),
+                                                   0.25 / (c->s.ewald_beta * c->s.ewald_beta), c->vir_conv.as<double>());
+            LAUNCH_CHECK();
+        }
+        k_ewald_gather_forces<real><<<nblk(n, EW_TILE), EW_TILE, 0, st>>>(
+            n, c->nk, c->kvec.as<double4>(), c->sk.as<double2>(), c->pos, dq, c->dd.as<double>(), c->rowf.as<double>(),
+            c_self(c), c->rec_dr.as<double>(), c->gp.as<double>(), vir ? c->rec_dd.as<double>() : nullptr);
+    } else {
+        PGM_TRY(spread_fft<real>(c, dq, true, true, nullptr, -1, vir ? c->vir_conv.as<double>() : nullptr));
+        ORDER_SWITCH(c->s.order, (k_gather_forces<real, PO><<<nb, 256, 0, st>>>(
+                                     n, c->pos, c->pg, c->grid.as<real>(), dq, c->dd.as<double>(), c->rowf.as<double>(),
+                                     c_self(c), c->rec_dr.as<double>(), c->gp.as<double>(),
+                                     vir ? c->rec_dd.as<double>() : nullptr)));
+    }
     LAUNCH_CHECK();
+    if (vir) {
+        k_virial_atoms<<<nb, 256, 0, st>>>(n, c->rec_dr.as<double>(), c->off.as<double>(), c->rec_dd.as<double>(),
+                                           c->dd.as<double>(), c->vir_atoms.as<double>());
+        LAUNCH_CHECK();
+        const double b0 = c->s.ewald_beta;
+        const double e_bg = -KE * M_PI * c->sum_q * c->sum_q / (2.0 * c->volume * b0 * b0);
+        const double tail = c->s.lj_lrc ? lj_tail(c) : 0.0;
+        const int ncb = molecular ? 0 : nblk(c->ncov, 256);
+        if (ncb > 0) {
+            k_virial_cov<<<ncb, 256, 0, st>>>(c->ncov, c->pos, c->box, c->ci.as<int>(), c->cj.as<int>(), c->cc.as<double>(),
+                                               c->gp.as<double>(), c->vir_cov.as<double>());
+            LAUNCH_CHECK();
+        }
+        k_virial_final<<<1, 256, 0, st>>>(c->vir_pair.as<double>(), npair, c->vir_conv.as<double>(), nconv,
+                                          c->vir_atoms.as<double>(), nb, c->vir_cov.as<double>(), ncb, -e_bg - 2.0 * tail,
+                                          c->vir_out.as<double>());
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 4, c->vir_out.p, 9 * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     if (forces) {
         k_forces_final<<<nb, 256, 0, st>>>(n, c->pos, c->box, c->g_el.as<double>(), c->g_lj.as<double>(), c->rec_dr.as<double>(),
                                            c->gp.as<double>(), c->cstart.as<int>(), c->ci.as<int>(), c->cj.as<int>(),
@@ -688,10 +869,13 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     const double ke_const = KE * (-(b0 / SQRT_PI) * c->sum_q2 - M_PI * c->sum_q * c->sum_q / (2.0 * c->volume * b0 * b0));
     const double tail = c->s.lj_lrc ? lj_tail(c) : 0.0;
     k_energy_final<<<1, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), nb, c->conv_part.as<double>(),
-                                      nblk((long long)c->ncplx, 256), ke_const, tail, c_self(c), c->energy.as<double>());
+                                      nconv, ke_const, tail, c_self(c), c->energy.as<double>());
     LAUNCH_CHECK();
     if (quiet) return PGM_OK;  // energies stay in c->energy
-    return read_energy(c, energy);
+    PGM_TRY(read_energy(c, energy));
+    if (vir)
+        for (int k = 0; k < 9; ++k) strain[k] = c->h_pin_dbl[4 + k];
+    return PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- C ABI
@@ -708,9 +892,46 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         g_err = "pgm_ctx_create: invalid cutoff, skin, ewald_beta or max_iter";
         return PGM_ERR_ARG;
     }
+    if (s->recip != 0 && s->recip != 1) {
+        g_err = "pgm_ctx_create: recip must be 0 (PME) or 1 (Ewald)";
+        return PGM_ERR_ARG;
+    }
+    if (s->recip == 1 && !(s->k_tol > 0 && s->k_tol < 1)) {
+        g_err = "pgm_ctx_create: Ewald needs 0 < k_tol < 1";
+        return PGM_ERR_ARG;
+    }
     pgm_ctx *c = new pgm_ctx();
     c->n = top->n_atoms;
     c->s = *s;
+    c->ewald = s->recip == 1;
+    if (c->ewald) {  // half-space index set with 0 < |k| < kcut at the creation box (pgm_jax/ewald.py:129-158)
+        const double kcut = 2.0 * s->ewald_beta * std::sqrt(-std::log(s->k_tol));
+        double Hi[9], hmax = 0.0;
+        inv3(H, Hi);
+        for (int a = 0; a < 3; ++a) hmax = std::max(hmax, std::sqrt(H[3 * a] * H[3 * a] + H[3 * a + 1] * H[3 * a + 1] + H[3 * a + 2] * H[3 * a + 2]));
+        const int Mx = (int)std::ceil(kcut * hmax / (2.0 * M_PI)) + 1;
+        for (int a = -Mx; a <= Mx; ++a)
+            for (int b = -Mx; b <= Mx; ++b)
+                for (int d = -Mx; d <= Mx; ++d) {
+                    if (!(a > 0 || (a == 0 && b > 0) || (a == 0 && b == 0 && d > 0))) continue;
+                    double k2 = 0.0;
+                    for (int j = 0; j < 3; ++j) {
+                        const double kj = 2.0 * M_PI * (Hi[3 * j] * a + Hi[3 * j + 1] * b + Hi[3 * j + 2] * d);
+                        k2 += kj * kj;
+                    }
+                    if (k2 > 0.0 && std::sqrt(k2) < kcut) {
+                        c->h_m.push_back(a);
+                        c->h_m.push_back(b);
+                        c->h_m.push_back(d);
+                    }
+                }
+        c->nk = (int)(c->h_m.size() / 3);
+        if (c->nk == 0) {
+            g_err = "pgm_ctx_create: k_tol leaves no k-vector";
+            delete c;
+            return PGM_ERR_ARG;
+        }
+    }
     c->stream = (hipStream_t)stream;
     c->mixed = s->precision == 1;
     c->rsize = c->mixed ? sizeof(float) : sizeof(double);
@@ -743,6 +964,24 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = upload(c->se, c->h_se.data(), n, st)) < 0) return fail(r);
     if ((r = upload(c->mol, c->h_mol.data(), n, st)) < 0) return fail(r);
     if ((r = upload(c->R2, c->h_R2.data(), n, st)) < 0) return fail(r);
+    {  // molecules as CSR over atom indices (for centres of mass)
+        int mmax = -1;
+        for (int i = 0; i < n; ++i) {
+            if (c->h_mol[i] < 0 || c->h_mol[i] >= n) {
+                g_err = "pgm_ctx_create: molecule index out of range [0, n_atoms)";
+                return fail(PGM_ERR_ARG);
+            }
+            mmax = std::max(mmax, c->h_mol[i]);
+        }
+        c->nmol_all = mmax + 1;
+        std::vector<int> mp(c->nmol_all + 1, 0), ma(n);
+        for (int i = 0; i < n; ++i) mp[c->h_mol[i] + 1]++;
+        for (int m = 0; m < c->nmol_all; ++m) mp[m + 1] += mp[m];
+        std::vector<int> fill(mp.begin(), mp.end() - 1);
+        for (int i = 0; i < n; ++i) ma[fill[c->h_mol[i]]++] = i;
+        if ((r = upload(c->mol_ptr, mp.data(), mp.size(), st)) < 0) return fail(r);
+        if ((r = upload(c->mol_atoms, ma.data(), ma.size(), st)) < 0) return fail(r);
+    }
     if (c->mixed) {
         std::vector<float> af(c->h_alpha.begin(), c->h_alpha.end());
         if ((r = upload(c->alpha_r, af.data(), n, st)) < 0) return fail(r);
@@ -751,6 +990,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     }
     // covalent dipoles: CSR by owner atom (cstart over m sorted by cov_i) and by target (jstart/jlist)
     const int nc = top->n_cov;
+    c->ncov = nc;
     std::vector<int> order(nc), ci(nc), cj(nc), cs(n + 1, 0), js(n + 1, 0), jl(nc);
     std::vector<double> cc(nc);
     for (int m = 0; m < nc; ++m) {
@@ -824,7 +1064,9 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     c->ncplx = (size_t)s->grid[0] * s->grid[1] * (s->grid[2] / 2 + 1);
     if ((r = c->grid.ensure(c->rsize * c->ngrid)) < 0) return fail(r);
     if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx)) < 0) return fail(r);
-    if ((r = c->conv_part.ensure(sizeof(double) * (nblk((long long)c->ncplx, 256) + 16))) < 0) return fail(r);
+    if ((r = c->conv_part.ensure(sizeof(double) * (std::max(nblk((long long)c->ncplx, 256), nblk(c->nk, EW_TILE)) + 16))) < 0)
+        return fail(r);
+    if (c->ewald && (r = c->sk.ensure(sizeof(double2) * std::max(c->nk, 1))) < 0) return fail(r);
     if ((r = setup_tiles(c)) < 0) return fail(r);
     if ((r = set_box(c, H)) < 0) return fail(r);
     // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
@@ -845,19 +1087,23 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     for (int k = 0; k < 4; ++k)
         if (c->ev_read[k]) (void)hipEventDestroy(c->ev_read[k]);
+    for (auto &t : c->timed)
+        for (hipEvent_t e : t.ev)
+            if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fwd) rocfft_plan_destroy(c->fwd);
     if (c->bwd) rocfft_plan_destroy(c->bwd);
     if (c->info) rocfft_execution_info_destroy(c->info);
     DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
-                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->G1, &c->G2, &c->key,
+                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
                      &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy,
-                     &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail,
+                     &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->spl, &c->loc, &c->blocks,
-                     &c->pme_sort_tmp, &c->stats};
+                     &c->pme_sort_tmp, &c->stats, &c->mol_ptr, &c->mol_atoms, &c->com, &c->off, &c->rec_dd,
+                     &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
     if (c->h_pin_ring) (void)hipHostFree(c->h_pin_ring);
@@ -971,17 +1217,119 @@ extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pg
 }
 
 template <typename real>
-static int ef_api(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces) {
+static int ef_api(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces,
+                  const double *masses = nullptr, double *strain = nullptr, bool molecular = true) {
     PGM_TRY(prepare(c, pos));
     k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
                                                             c->cc.as<double>(), c->pperm.as<double>());
     LAUNCH_CHECK();
-    return energy_forces_impl<real>(c, mu, energy, forces);
+    return energy_forces_impl<real>(c, mu, energy, forces, false, masses, strain, molecular);
 }
 
 extern "C" int pgm_energy_forces(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces) {
     if (!c || !pos || !mu) return PGM_ERR_ARG;
     return c->mixed ? ef_api<float>(c, pos, mu, energy, forces) : ef_api<double>(c, pos, mu, energy, forces);
+}
+
+extern "C" int pgm_strain_derivative(pgm_ctx *c, const double *pos, const double *mu, const double *masses, int molecular,
+                                     pgm_energy *energy, double *forces, double *strain) {
+    if (!c || !pos || !mu || !strain || (molecular && !masses)) {
+        g_err = "pgm_strain_derivative: null argument";
+        return PGM_ERR_ARG;
+    }
+    return c->mixed ? ef_api<float>(c, pos, mu, energy, forces, masses, strain, molecular != 0)
+                    : ef_api<double>(c, pos, mu, energy, forces, masses, strain, molecular != 0);
+}
+
+// ---------------------------------------------------------------------------- live kernel timing
+extern "C" int pgm_set_timing(pgm_ctx *c, int on) {
+    if (!c) return PGM_ERR_ARG;
+    PGM_TRY(harvest_timing(c));
+    for (auto &t : c->timed) {
+        t.sum_ms = 0.0;
+        t.count = 0;
+    }
+    c->timing = on != 0;
+    return PGM_OK;
+}
+
+extern "C" int pgm_kernel_timing(pgm_ctx *c, double *avg_ms, long long *launches) {
+    if (!c || !avg_ms || !launches) return PGM_ERR_ARG;
+    PGM_TRY(harvest_timing(c));
+    for (int k = 0; k < 3; ++k) {
+        launches[k] = c->timed[k].count;
+        avg_ms[k] = c->timed[k].count ? c->timed[k].sum_ms / (double)c->timed[k].count : 0.0;
+    }
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- dense induction solve
+extern "C" int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu,
+                                          double tol, int max_iter, void *stream, int *iterations, double *residual) {
+    if (n <= 0 || !T || !alpha || !F || !mu || !(tol > 0) || max_iter < 0) {
+        g_err = "pgm_solve_linear_induction: invalid argument";
+        return PGM_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int n3 = 3 * n, nb = nblk(n3, 256);
+    DevBuf y, r, z, p, part, out;
+    double *h = nullptr;
+    auto cleanup = [&](int rc) {
+        if (h) (void)hipHostFree(h);
+        for (DevBuf *b : {&y, &r, &z, &p, &part, &out}) b->release();
+        return rc;
+    };
+    int rc;
+    for (DevBuf *b : {&y, &r, &z, &p})
+        if ((rc = b->ensure(sizeof(double) * n3)) < 0) return cleanup(rc);
+    if ((rc = part.ensure(sizeof(double) * 2 * std::max(n, nb))) < 0) return cleanup(rc);
+    if ((rc = out.ensure(sizeof(double) * 2)) < 0) return cleanup(rc);
+    if (hipHostMalloc((void **)&h, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        return cleanup(PGM_ERR_HIP);
+    }
+    auto scalars = [&](int np, int stride) -> int {
+        k_dense_reduce<<<1, 256, 0, st>>>(part.as<double>(), np, stride, out.as<double>());
+        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h, out.p, stride * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            g_err = "pgm_solve_linear_induction: device error";
+            return PGM_ERR_HIP;
+        }
+        return PGM_OK;
+    };
+    double *x = mu;
+    // x0 = alpha F, r = F - A x0, z = alpha r, p = z
+    k_dense_update<<<nb, 256, 0, st>>>(n3, 3, 0.0, 0.0, F, alpha, nullptr, x, nullptr, nullptr, nullptr, part.as<double>());
+    if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
+    const double fnorm = std::sqrt(h[0]);
+    k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, x, y.as<double>(), part.as<double>());
+    k_dense_update<<<nb, 256, 0, st>>>(n3, 0, 0.0, 0.0, F, alpha, y.as<double>(), x, r.as<double>(), z.as<double>(), p.as<double>(),
+                                       part.as<double>());
+    if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
+    double rz = h[0], rr = h[1];
+    int it = 0;
+    // stopping rule of jax.scipy.sparse.linalg.cg: |r| <= tol |F|
+    while (it < max_iter && std::sqrt(rr) > tol * fnorm) {
+        k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, p.as<double>(), y.as<double>(), part.as<double>());
+        if ((rc = scalars(n, 1)) < 0) return cleanup(rc);
+        const double al = rz / std::max(h[0], 1e-300);
+        k_dense_update<<<nb, 256, 0, st>>>(n3, 1, al, 0.0, F, alpha, y.as<double>(), x, r.as<double>(), z.as<double>(),
+                                           p.as<double>(), part.as<double>());
+        if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
+        const double be = h[0] / std::max(rz, 1e-300);
+        rz = h[0];
+        rr = h[1];
+        k_dense_update<<<nb, 256, 0, st>>>(n3, 2, 0.0, be, F, alpha, nullptr, x, nullptr, z.as<double>(), p.as<double>(),
+                                           nullptr);
+        ++it;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        g_err = "pgm_solve_linear_induction: device error";
+        return cleanup(PGM_ERR_HIP);
+    }
+    if (iterations) *iterations = it;
+    if (residual) *residual = fnorm > 0 ? std::sqrt(rr) / fnorm : 0.0;
+    return cleanup(std::sqrt(rr) > tol * fnorm ? PGM_NOT_CONVERGED : PGM_OK);
 }
 
 // ---------------------------------------------------------------------------- MD
@@ -1036,6 +1384,36 @@ static int md_kinetic(pgm_ctx *c, double *ke) {
     return PGM_OK;
 }
 
+// First half of a step: NVE (half kick, drift, SHAKE) or, with a Langevin friction, the g-BAOAB
+// half kick / drift / O / drift with SHAKE and RATTLE.
+static int md_first(pgm_ctx *c, pgm_md_state *s, double dt) {
+    hipStream_t st = c->stream;
+    if (c->friction > 0.0) {
+        const int nb = nblk(c->nmol, 256);
+        k_md_first_langevin<<<nb, 256, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(), c->d2.as<double>(),
+                                                dt, std::exp(-c->friction * dt), c->kT, c->seed, (uint32_t)s->step,
+                                                c->md_fail.as<int>(), c->md_part.as<double>());
+        LAUNCH_CHECK();
+        k_accumulate_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nb, c->heat.as<double>());
+    } else {
+        k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
+                                                       c->d2.as<double>(), dt, c->md_fail.as<int>());
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+extern "C" int pgm_md_set_thermostat(pgm_ctx *c, double friction, double temperature, unsigned long long seed) {
+    if (!c || !(friction >= 0.0) || !(temperature >= 0.0)) {
+        g_err = "pgm_md_set_thermostat: friction and temperature must be >= 0";
+        return PGM_ERR_ARG;
+    }
+    c->friction = friction;
+    c->kT = 0.0083144626181532 * temperature;
+    c->seed = seed;
+    return PGM_OK;
+}
+
 extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pgm_md_report *rep) {
     if (!c || !s || !s->pos || !s->vel || !s->forces || nsteps < 0 || !(dt > 0)) return PGM_ERR_ARG;
     if (!c->md_ready) {
@@ -1051,6 +1429,9 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     int it_max = 0;
     const int rebuilds0 = c->rebuilds;
     const long long slot0 = c->stats_slot;
+    PGM_TRY(c->heat.ensure(sizeof(double)));
+    k_fill<double><<<1, 64, 0, st>>>(1, c->heat.as<double>(), 0.0);
+    LAUNCH_CHECK();
     if (!s->have_forces) {
         const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true)
                                : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true);
@@ -1065,16 +1446,12 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     for (int k = 0; k < nsteps; ++k) {
         if (disp_pending) {  // the read-back queued last step is complete: the CG waited on later work
             if (!c->rows_valid || c->h_pin_dbl[9] > trigger) {
-                k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
-                                                               c->d2.as<double>(), dt, c->md_fail.as<int>());
-                LAUNCH_CHECK();
+                PGM_TRY(md_first(c, s, dt));
                 PGM_TRY(build_rows(c, s->pos));
                 goto forces;
             }
         }
-        k_md_first<<<nblk(c->nmol, 128), 128, 0, st>>>(c->nmol, s->pos, s->vel, s->forces, c->minv.as<double>(),
-                                                       c->d2.as<double>(), dt, c->md_fail.as<int>());
-        LAUNCH_CHECK();
+        PGM_TRY(md_first(c, s, dt));
     forces:
         k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
         k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, c->energy.as<double>() + 3);
@@ -1107,6 +1484,7 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     }
     HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &c->st.as<CGState>()->norm, sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(c->h_pin_int + 4, c->md_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 2, c->heat.p, sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     s->ind.norm = c->h_pin_dbl[1];
     if (c->h_pin_int[4] != 0) {
@@ -1126,6 +1504,7 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         rep->cg_iterations_last = it;
         rep->cg_residual_max = res_max;
         rep->row_rebuilds = c->rebuilds - rebuilds0;
+        rep->heat = c->h_pin_dbl[2];
     }
     return status;
 }
@@ -1143,6 +1522,10 @@ extern "C" int pgm_md_rescale(pgm_ctx *c, pgm_md_state *s, double factor) {
 // (the CG matvec's pair part), [6] gather + combine, [7] one whole CG matvec (all of the above).
 template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int reps, double *ms) {
     using R4 = real4<real>;
+    if (c->ewald) {
+        g_err = "pgm_time_kernels: PME kernels only (this context uses the Ewald sum)";
+        return PGM_ERR_STATE;
+    }
     PGM_TRY(prepare(c, pos));
     hipStream_t st = c->stream;
     const int n = c->n;
@@ -1204,9 +1587,9 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     rc = timed(4, [&] { rocfft_execute(c->bwd, in2, out2, c->info); });
     if (rc < 0) return rc;
     rc = timed(5, [&] {
-        k_row_field<real, false><<<nblk((long long)n * ROWL, 256), 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->G1.as<real>(), c->G2.as<real>(),
-            c->xr.as<double4>(), dq, c->box, c->vrow.as<R4>(), nullptr, -1);
+        k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
+            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+            c->vrow.as<R4>(), nullptr, -1);
     });
     if (rc < 0) return rc;
     ORDER_SWITCH(c->s.order, rc = timed(6, [&] {

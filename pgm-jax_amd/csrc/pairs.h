@@ -148,39 +148,42 @@ __global__ __launch_bounds__(256) void k_compact_rows(int n, const double4 *__re
     }
 }
 
-// Pair tensors of the compacted rows, once per geometry: G1 and G2 of every entry (the dipole
-// field needs only these), in the layout of the rows.  ROWL lanes per atom, near part with the
-// series branches, far part with the closed form.
+// Sliced-ELL layout of the rows used by the CG matvec: entry e of atom i lives at
+// ell(e, i) = (e / RL) * (n * RL) + i * RL + e % RL, so RL lanes per atom, 64 / RL atoms per wave,
+// read 64 consecutive words per load.
+constexpr int RL = 4;
+PGM_DEV size_t ell(int e, int i, int n) { return (size_t)(e / RL) * ((size_t)n * RL) + (size_t)i * RL + (e % RL); }
+
+// Pair tensors of the compacted rows, once per geometry: partner index, G1 and G2 of every entry
+// (the dipole field needs only these) in the ELL layout, near part (series branches) first, then
+// the far part (closed form).  RL lanes per atom.
 template <typename real>
 __global__ __launch_bounds__(256) void k_pair_tensors(int n, const int *__restrict__ rows, const int *__restrict__ count,
                                                       const int *__restrict__ count_near, int cap,
                                                       const double4 *__restrict__ xr, Box box, real beta,
-                                                      real *__restrict__ G1, real *__restrict__ G2) {
-    const int lane = threadIdx.x % ROWL;
-    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+                                                      int *__restrict__ jell, real *__restrict__ G1, real *__restrict__ G2) {
+    const int lane = threadIdx.x % RL;
+    const int i = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
     if (i >= n) return;
     const double4 a = xr[i];
     const int c = count[i], cn = count_near[i];
     const size_t o = (size_t)i * cap;
-    for (int e = lane; e < cn; e += ROWL) {
-        const double4 b = xr[rows[o + e]];
+    for (int e = lane; e < c; e += RL) {
+        const int j = e < cn ? rows[o + e] : rows[o + cap - 1 - (e - cn)];
+        const double4 b = xr[j];
         double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
         min_image(dx, dy, dz, box);
         const real x = (real)dx, y = (real)dy, z = (real)dz;
+        const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
         real G[3];
-        pair_G<real, 3>(rsqrt_(real(2) * (real)(a.w + b.w)), beta, x * x + y * y + z * z, G);
-        G1[o + e] = G[1];
-        G2[o + e] = G[2];
-    }
-    for (int e = cap - (c - cn) + lane; e < cap; e += ROWL) {
-        const double4 b = xr[rows[o + e]];
-        double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-        min_image(dx, dy, dz, box);
-        const real x = (real)dx, y = (real)dy, z = (real)dz;
-        real G[3];
-        pair_G_far<real, 3>(rsqrt_(real(2) * (real)(a.w + b.w)), beta, x * x + y * y + z * z, G);
-        G1[o + e] = G[1];
-        G2[o + e] = G[2];
+        if (e < cn)
+            pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
+        else
+            pair_G_far<real, 3>(aij, beta, x * x + y * y + z * z, G);
+        const size_t q = ell(e, i, n);
+        jell[q] = j;
+        G1[q] = G[1];
+        G2[q] = G[2];
     }
 }
 
@@ -203,34 +206,29 @@ PGM_DEV void row_field_entry(const double4 &a, const double4 &b, const real4<rea
 }
 
 // out_i = sum_k de_ik/dd_i = sum_k (-q_k G1 - G2 (d_k . x)) x + G1 d_k, x = r_i - r_k
-// (minus the direct-space field at atom i; pgm_jax/md/forcefield.py:1395-1402), with G1, G2 of the
-// rows precomputed by k_pair_tensors.  dq holds (d_x, d_y, d_z, q) per atom; WITHQ adds the
-// charge term.
+// (minus the direct-space field at atom i; pgm_jax/md/forcefield.py:1395-1402), with the partner
+// indices and G1, G2 precomputed by k_pair_tensors (ELL layout, RL lanes per atom).  dq holds
+// (d_x, d_y, d_z, q) per atom; WITHQ adds the charge term.
 template <typename real, bool WITHQ>
-__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ rows, const int *__restrict__ count,
-                                                   const int *__restrict__ count_near, int cap,
+__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ jell, const int *__restrict__ count,
                                                    const real *__restrict__ G1, const real *__restrict__ G2,
                                                    const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
                                                    real4<real> *__restrict__ out, const int *__restrict__ conv_it, int iter) {
     if (conv_it && iter >= *conv_it) return;
-    const int lane = threadIdx.x % ROWL;
-    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    const int lane = threadIdx.x % RL;
+    const int i = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
     if (i >= n) return;
     const double4 a = xr[i];
-    const int c = count[i], cn = count_near[i];
-    const size_t o = (size_t)i * cap;
+    const int c = count[i];
     real ex = 0, ey = 0, ez = 0;
-    for (int e = lane; e < cn; e += ROWL) {
-        const int j = rows[o + e];
-        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[o + e], G2[o + e], ex, ey, ez);
+    for (int e = lane; e < c; e += RL) {
+        const size_t q = ell(e, i, n);
+        const int j = jell[q];
+        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[q], G2[q], ex, ey, ez);
     }
-    for (int e = cap - (c - cn) + lane; e < cap; e += ROWL) {
-        const int j = rows[o + e];
-        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[o + e], G2[o + e], ex, ey, ez);
-    }
-    ex = group_sum<ROWL>(ex);
-    ey = group_sum<ROWL>(ey);
-    ez = group_sum<ROWL>(ez);
+    ex = group_sum<RL>(ex);
+    ey = group_sum<RL>(ey);
+    ez = group_sum<RL>(ez);
     if (lane == 0) {
         real4<real> v;
         v.x = ex;
@@ -248,7 +246,9 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
 //           - G2 (d_i (d_k.x) + d_k (d_i.x)),   t = q_i d_k.x - q_k d_i.x
 //   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2.
 // Outputs (FP64): g_el = sum de/dx, g_lj, the row field sum de/dd_i, and per-block energy sums
-// (each pair counted in both rows: the caller halves them).
+// (each pair counted in both rows: the caller halves them).  With VIRIAL, also the per-block sums
+// of (KE de/dx + de_LJ/dx) (x) X, X = x - o_i + o_k with o the offsets of the atoms from their
+// molecules' centres of mass: the pair part of the molecular strain derivative (again twice).
 #define PAIR_FORCE_ENTRY(GFUN) \
             const double4 b = xr[j]; \
             double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z; \
@@ -267,15 +267,19 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
             const real ep = qi * qk * G[0] + t * G[1] - G[2] * dix * dkx + G[1] * didk; \
             const real radial = -qi * qk * G[1] - t * G[2] + G[3] * dix * dkx - G[2] * didk; \
             const real qiG1 = qi * G[1], qkG1 = qk * G[1]; \
-            gx += (double)(radial * x + qiG1 * dk.x - qkG1 * di.x - G[2] * (di.x * dkx + dk.x * dix)); \
-            gy += (double)(radial * y + qiG1 * dk.y - qkG1 * di.y - G[2] * (di.y * dkx + dk.y * dix)); \
-            gz += (double)(radial * z + qiG1 * dk.z - qkG1 * di.z - G[2] * (di.z * dkx + dk.z * dix)); \
+            const real ge0 = radial * x + qiG1 * dk.x - qkG1 * di.x - G[2] * (di.x * dkx + dk.x * dix); \
+            const real ge1 = radial * y + qiG1 * dk.y - qkG1 * di.y - G[2] * (di.y * dkx + dk.y * dix); \
+            const real ge2 = radial * z + qiG1 * dk.z - qkG1 * di.z - G[2] * (di.z * dkx + dk.z * dix); \
+            gx += (double)ge0; \
+            gy += (double)ge1; \
+            gz += (double)ge2; \
             ee += (double)ep; \
             const real cfield = -qk * G[1] - G[2] * dkx; \
             fx += (double)(cfield * x + G[1] * dk.x); \
             fy += (double)(cfield * y + G[1] * dk.y); \
             fz += (double)(cfield * z + G[1] * dk.z); \
             const double eps = sei * se[j]; \
+            double l0 = 0.0, l1 = 0.0, l2 = 0.0; \
             if (mol[j] != mi && eps != 0.0) { \
                 const double rmin = rhi + rh[j]; \
                 const double rr2 = dx * dx + dy * dy + dz * dz; \
@@ -283,24 +287,34 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
                 const double s6 = s2 * s2 * s2; \
                 el += eps * (s6 * s6 - 2.0 * s6); \
                 const double dl = eps * 12.0 * (s6 - s6 * s6) / rr2; \
-                lx += dl * dx; \
-                ly += dl * dy; \
-                lz += dl * dz; \
+                l0 = dl * dx; \
+                l1 = dl * dy; \
+                l2 = dl * dz; \
             } \
-        
+            lx += l0; \
+            ly += l1; \
+            lz += l2; \
+            if (VIRIAL) { \
+                const double X[3] = {dx - oi[0] + off[3 * j], dy - oi[1] + off[3 * j + 1], dz - oi[2] + off[3 * j + 2]}; \
+                const double v[3] = {KE * (double)ge0 + l0, KE * (double)ge1 + l1, KE * (double)ge2 + l2}; \
+                _Pragma("unroll") for (int p_ = 0; p_ < 3; ++p_) \
+                _Pragma("unroll") for (int q_ = 0; q_ < 3; ++q_) w[3 * p_ + q_] += v[p_] * X[q_]; \
+            }
 
-template <typename real>
+template <typename real, bool VIRIAL>
 __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restrict__ rows, const int *__restrict__ count,
                                                      const int *__restrict__ count_near, int cap,
                                                      const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
                                                      real beta, const double *__restrict__ rh, const double *__restrict__ se,
                                                      const int *__restrict__ mol, double *__restrict__ g_el,
                                                      double *__restrict__ g_lj, double *__restrict__ rowf,
-                                                     double *__restrict__ epart) {
-    __shared__ double lds[2 * 16];
+                                                     double *__restrict__ epart, const double *__restrict__ off,
+                                                     double *__restrict__ wpart) {
+    __shared__ double lds[9 * 16];
     const int lane = threadIdx.x % ROWL;
     const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
     double esum[2] = {0.0, 0.0};
+    double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < n) {
         const double4 a = xr[i];
         const real4<real> di = dq[i];
@@ -309,6 +323,12 @@ __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restric
         const int mi = mol[i];
         const int c = count[i];
         const int *row = rows + (size_t)i * cap;
+        double oi[3] = {0.0, 0.0, 0.0};
+        if (VIRIAL) {
+            oi[0] = off[3 * i];
+            oi[1] = off[3 * i + 1];
+            oi[2] = off[3 * i + 2];
+        }
         double gx = 0, gy = 0, gz = 0, lx = 0, ly = 0, lz = 0, fx = 0, fy = 0, fz = 0, ee = 0, el = 0;
         const int cn = count_near[i];
         for (int e = lane; e < cn; e += ROWL) {
@@ -346,6 +366,11 @@ __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restric
     if (threadIdx.x == 0) {
         epart[2 * blockIdx.x] = esum[0];
         epart[2 * blockIdx.x + 1] = esum[1];
+    }
+    if (VIRIAL) {
+        block_sum<9>(w, lds);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 9; ++k) wpart[9 * blockIdx.x + k] = w[k];
     }
 }
 

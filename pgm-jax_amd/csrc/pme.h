@@ -263,6 +263,56 @@ __global__ __launch_bounds__(256) void k_convolve(size_t nc, int K3h, int K3even
     }
 }
 
+// Reciprocal-space part of the strain derivative that comes from the influence function: for
+// G(m*, V) = exp(-pi^2 |m*|^2 / beta^2) / (2 pi V |m*|^2) B(m), with m* = H^-1 m moving as
+// -eps^T m* and V as (1 + tr eps), dG/d eps_ab = G [2 (pi^2/beta^2 + 1/|m*|^2) m*_a m*_b - delta_ab].
+// Per block: the six sums of w Gc |F Q|^2 2 (pi^2/beta^2 + 1/|m*|^2) m*_a m*_b (xx yy zz xy xz yz)
+// and E_rec = sum w Gc |F Q|^2, from the unconvolved transform (w = 1/2 on the k3 = 0 and Nyquist
+// planes of the half spectrum, 1 elsewhere).  cgrid is only read.
+struct RecipGeom {
+    double hinv[9];   // H^-1: m* = H^-1 (m1, m2, m3)
+    int K[3];
+    double pb2;       // pi^2 / beta^2
+};
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_convolve_virial(size_t nc, RecipGeom rg, const real *__restrict__ cgrid,
+                                                         const real *__restrict__ Gc, double *__restrict__ vpart) {
+    __shared__ double lds[7 * 16];
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (s < nc) {
+        const int K1 = rg.K[0], K2 = rg.K[1], K3 = rg.K[2], K3h = K3 / 2 + 1;
+        const int k3 = (int)(s % (size_t)K3h);
+        const int k2 = (int)((s / (size_t)K3h) % (size_t)K2);
+        const int k1 = (int)(s / ((size_t)K3h * K2));
+        const double f1 = k1 <= (K1 - 1) / 2 ? k1 : k1 - K1;
+        const double f2 = k2 <= (K2 - 1) / 2 ? k2 : k2 - K2;
+        const double f3 = k3;
+        const double *h = rg.hinv;
+        const double mx = h[0] * f1 + h[1] * f2 + h[2] * f3;
+        const double my = h[3] * f1 + h[4] * f2 + h[5] * f3;
+        const double mz = h[6] * f1 + h[7] * f2 + h[8] * f3;
+        const double msq = mx * mx + my * my + mz * mz;
+        const double re = (double)cgrid[2 * s], im = (double)cgrid[2 * s + 1];
+        const double w = (k3 == 0 || (K3 % 2 == 0 && k3 == K3h - 1)) ? 0.5 : 1.0;
+        const double e = w * (double)Gc[s] * (re * re + im * im);
+        if (msq > 0.0) {
+            const double c = 2.0 * e * (rg.pb2 + 1.0 / msq);
+            acc[0] = c * mx * mx;
+            acc[1] = c * my * my;
+            acc[2] = c * mz * mz;
+            acc[3] = c * mx * my;
+            acc[4] = c * mx * mz;
+            acc[5] = c * my * mz;
+            acc[6] = e;
+        }
+    }
+    block_sum<7>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) vpart[7 * blockIdx.x + k] = acc[k];
+}
+
 // Gather of the dipole gradient (pme.py:378-388): g_a = sum phi * (derivative spline along a),
 // dU/dd = escale . g; then combined with the row field according to `mode`:
 //   0 FIELD   out = -(row + rec - c_self d)                          [+ sum |alpha out| per block]
@@ -271,6 +321,49 @@ __global__ __launch_bounds__(256) void k_convolve(size_t nc, int K3h, int K3even
 //   3 PLAIN   out = b - (d / alpha + row + rec - c_self d)           (d = x0, extra = b)
 //   4 REC     out = dU_rec/dd alone
 enum { MODE_FIELD = 0, MODE_AV = 1, MODE_FUSED = 2, MODE_PLAIN = 3, MODE_REC = 4 };
+
+// out[i] from rec = dU_rec/dd (rx, ry, rz) and the row field, by `mode` (see above); acc gets the
+// per-atom contribution to the block partial of MODE_FIELD / MODE_AV.
+template <typename real>
+PGM_DEV void combine_out(int mode, int i, real rx, real ry, real rz, const real4<real> *__restrict__ dq,
+                         const real4<real> *__restrict__ row, const real *__restrict__ alpha,
+                         const real4<real> *__restrict__ extra, real cself, real4<real> *__restrict__ out, double &acc) {
+    const real4<real> d = dq[i];
+    const real4<real> rw = row[i];
+    const real tx = rw.x + rx - cself * d.x;  // dU/dd (direct + reciprocal + self)
+    const real ty = rw.y + ry - cself * d.y;
+    const real tz = rw.z + rz - cself * d.z;
+    real4<real> o;
+    o.w = 0;
+    const real ia = real(1) / alpha[i];
+    if (mode == MODE_FIELD) {
+        o.x = -tx;
+        o.y = -ty;
+        o.z = -tz;
+        const real a = alpha[i];
+        acc = fabs((double)(a * o.x)) + fabs((double)(a * o.y)) + fabs((double)(a * o.z));
+    } else if (mode == MODE_AV) {
+        o.x = d.x * ia + tx;
+        o.y = d.y * ia + ty;
+        o.z = d.z * ia + tz;
+        acc = (double)(d.x * o.x) + (double)(d.y * o.y) + (double)(d.z * o.z);
+    } else if (mode == MODE_FUSED) {
+        const real4<real> x0 = extra[i];
+        o.x = -tx - x0.x * ia;
+        o.y = -ty - x0.y * ia;
+        o.z = -tz - x0.z * ia;
+    } else if (mode == MODE_REC) {
+        o.x = rx;
+        o.y = ry;
+        o.z = rz;
+    } else {
+        const real4<real> b = extra[i];
+        o.x = b.x - (d.x * ia + tx);
+        o.y = b.y - (d.y * ia + ty);
+        o.z = b.z - (d.z * ia + tz);
+    }
+    out[i] = o;
+}
 
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_combine(int n, const double *__restrict__ pos, PMEGeom g,
@@ -324,41 +417,7 @@ __global__ __launch_bounds__(256) void k_gather_combine(int n, const double *__r
         const real rx = (real)es[0] * g1 + (real)es[1] * g2 + (real)es[2] * g3;
         const real ry = (real)es[3] * g1 + (real)es[4] * g2 + (real)es[5] * g3;
         const real rz = (real)es[6] * g1 + (real)es[7] * g2 + (real)es[8] * g3;
-        const real4<real> d = dq[i];
-        const real4<real> rw = row[i];
-        const real tx = rw.x + rx - cself * d.x;  // dU/dd (direct + reciprocal + self)
-        const real ty = rw.y + ry - cself * d.y;
-        const real tz = rw.z + rz - cself * d.z;
-        real4<real> o;
-        o.w = 0;
-        const real ia = real(1) / alpha[i];
-        if (mode == MODE_FIELD) {
-            o.x = -tx;
-            o.y = -ty;
-            o.z = -tz;
-            const real a = alpha[i];
-            acc[0] = fabs((double)(a * o.x)) + fabs((double)(a * o.y)) + fabs((double)(a * o.z));
-        } else if (mode == MODE_AV) {
-            o.x = d.x * ia + tx;
-            o.y = d.y * ia + ty;
-            o.z = d.z * ia + tz;
-            acc[0] = (double)(d.x * o.x) + (double)(d.y * o.y) + (double)(d.z * o.z);
-        } else if (mode == MODE_FUSED) {
-            const real4<real> x0 = extra[i];
-            o.x = -tx - x0.x * ia;
-            o.y = -ty - x0.y * ia;
-            o.z = -tz - x0.z * ia;
-        } else if (mode == MODE_REC) {
-            o.x = rx;
-            o.y = ry;
-            o.z = rz;
-        } else {
-            const real4<real> b = extra[i];
-            o.x = b.x - (d.x * ia + tx);
-            o.y = b.y - (d.y * ia + ty);
-            o.z = b.z - (d.z * ia + tz);
-        }
-        out[i] = o;
+        combine_out<real>(mode, i, rx, ry, rz, dq, row, alpha, extra, cself, out, acc[0]);
     }
     if (part) {
         block_sum<1>(acc, lds);
@@ -368,12 +427,14 @@ __global__ __launch_bounds__(256) void k_gather_combine(int n, const double *__r
 
 // Force gather: dU_rec/dd and dU_rec/dr at fixed q, d (the autodiff of pme.py:390-411 written
 // out, with the second spline derivative), FP64 outputs; also gp = KE (rowf + dU_rec/dd - c_self d),
-// the total dE/dd of the atom for the covalent-frame pull-back.
+// the total dE/dd of the atom for the covalent-frame pull-back.  rec_dd (optional) receives
+// dU_rec/dd itself (for the strain derivative).
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_forces(int n, const double *__restrict__ pos, PMEGeom g,
                                                        const real *__restrict__ phi, const real4<real> *__restrict__ dq,
                                                        const double *__restrict__ dd, const double *__restrict__ rowf,
-                                                       double cself, double *__restrict__ rec_dr, double *__restrict__ gp) {
+                                                       double cself, double *__restrict__ rec_dr, double *__restrict__ gp,
+                                                       double *__restrict__ rec_dd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int base[3];
@@ -437,6 +498,7 @@ __global__ __launch_bounds__(256) void k_gather_forces(int n, const double *__re
         const double rdd = es[3 * b] * gd[0] + es[3 * b + 1] * gd[1] + es[3 * b + 2] * gd[2];
         rec_dr[3 * i + b] = es[3 * b] * gw[0] + es[3 * b + 1] * gw[1] + es[3 * b + 2] * gw[2];
         gp[3 * i + b] = KE * (rowf[3 * i + b] + rdd - cself * dd[3 * i + b]);
+        if (rec_dd) rec_dd[3 * i + b] = rdd;
     }
 }
 

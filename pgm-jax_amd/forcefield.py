@@ -93,6 +93,8 @@ class PGMForceField:
         cs.predictor = PREDICTORS[settings.predictor]
         cs.fused, cs.norm_refresh, cs.peek = int(settings.fused), settings.norm_refresh, settings.peek
         cs.precision = 1 if settings.precision == "mixed" else 0
+        cs.recip = 1 if settings.recip == "ewald" else 0
+        cs.k_tol = settings.k_tol
         self._stream = torch.cuda.current_stream()
         ctx = C.c_void_p()
         _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),
@@ -178,6 +180,29 @@ class PGMForceField:
         """Return the total energy [kJ/mol] with the induced dipoles held at mu (forcefield.py:2249-2292)."""
         return self.energy_forces(pos, mu, H)[0]["total"]
 
+    def strain_derivative(self, pos, H, mu, molecular: bool = True):
+        """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, plus the tail impulse -E_lrc I with lj_lrc
+        (forcefield.py:2628-2695), as a float64 numpy array.
+
+        molecular=True: molecules are translated with their centres of mass (the system's masses;
+        molecules whole in `pos`).  molecular=False: atomic scaling, every atom r -> (1 + eps) r (the
+        covalent dipole frames strain with them).
+        """
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        mu = _dev(mu, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        if getattr(self, "_masses_dev", None) is None:
+            self._masses_dev = torch.as_tensor(np.asarray(self.sys.masses, float), device="cuda")
+        W = (C.c_double * 9)()
+        e = _lib.Energy()
+        _lib.check(_lib.lib().pgm_strain_derivative(self._ctx, C.c_void_p(pos.data_ptr()), C.c_void_p(mu.data_ptr()),
+                                                    C.c_void_p(self._masses_dev.data_ptr()), int(bool(molecular)), C.byref(e),
+                                                    None, W),
+                   "pgm_strain_derivative")
+        return np.array(list(W), float).reshape(3, 3)
+
     def perm_dipoles(self, pos, H=None):
         """Return the permanent dipoles p (N,3) [e nm] (forcefield.py:1032-1042)."""
         torch = _torch()
@@ -212,6 +237,17 @@ class PGMForceField:
         _lib.check(_lib.lib().pgm_pme_energy(self._ctx, C.c_void_p(pos.data_ptr()), int(with_charges), C.c_void_p(d.data_ptr()),
                                              C.byref(e), C.c_void_p(g.data_ptr())), "pgm_pme_energy")
         return e.value, g
+
+    def set_timing(self, on: bool = True) -> None:
+        """Record HIP events around the CG matvec kernels from now on (pgm_set_timing); clears the sums."""
+        _lib.check(_lib.lib().pgm_set_timing(self._ctx, int(bool(on))), "pgm_set_timing")
+
+    def kernel_timing(self) -> dict:
+        """Average duration [ms] and launch count of the row field, spread and gather since set_timing."""
+        ms = (C.c_double * 3)()
+        cnt = (C.c_longlong * 3)()
+        _lib.check(_lib.lib().pgm_kernel_timing(self._ctx, ms, cnt), "pgm_kernel_timing")
+        return {k: {"ms": ms[i], "launches": cnt[i]} for i, k in enumerate(("row_field", "spread", "gather"))}
 
     def row_entries(self) -> int:
         """Number of stored row entries within the cutoff at the last evaluation (pairs counted twice)."""

@@ -1,8 +1,9 @@
-"""Rigid-water MD on the GPU: NVE velocity Verlet with SHAKE/RATTLE in FP64 around the force field.
+"""Rigid-water MD on the GPU: NVE velocity Verlet or Langevin g-BAOAB with SHAKE/RATTLE in FP64.
 
 Mirrors the MD engine around pgm_jax/md/forcefield.py's compute (pgm_jax/md/integrate.py:743-761,
-the constrained step of md/flexible.py:776-799 with md/constraints.py:254-299): the whole step
-runs in the HIP library (pgm_md_run), the host only launches blocks of steps and reads the report.
+the constrained step of md/flexible.py:776-799 with md/constraints.py:254-299, the Langevin O step
+of md/thermostats.py:195-242): the whole step runs in the HIP library (pgm_md_run), the host only
+launches blocks of steps and reads the report.
 """
 
 from __future__ import annotations
@@ -15,10 +16,32 @@ from . import _lib
 from .forcefield import PGMForceField, _dev, _torch
 
 
-class RigidWaterMD:
-    """NVE MD of consecutive 3-site rigid molecules (constraint lengths from the initial geometry)."""
+class Langevin:
+    """Langevin thermostat (pgm_jax/md/thermostats.py:165-242): friction gamma [1/ps].
 
-    def __init__(self, ff: PGMForceField, pos, vel, masses=None) -> None:
+    Raises ValueError for a negative friction, as the reference does.
+    """
+
+    name = "langevin"
+
+    def __init__(self, friction: float = 1.0) -> None:
+        if float(friction) < 0.0:
+            raise ValueError(f"Langevin: friction must be >= 0 ({friction!r} 1/ps)")
+        self.friction = float(friction)
+
+    def describe(self) -> str:
+        return f"Langevin {self.friction:g}/ps"
+
+
+class RigidWaterMD:
+    """MD of consecutive 3-site rigid molecules (constraint lengths from the initial geometry).
+
+    NVE without a thermostat; with `thermostat=Langevin(gamma)` and `temperature` [K], the g-BAOAB
+    step (noise from Philox4x32-10 keyed by `seed`).
+    """
+
+    def __init__(self, ff: PGMForceField, pos, vel, masses=None, thermostat: Langevin | None = None,
+                 temperature: float = 298.0, seed: int = 0) -> None:
         torch = _torch()
         self.ff = ff
         n = ff.n
@@ -35,6 +58,15 @@ class RigidWaterMD:
                                    _lib.Induction(self.ind.mu.data_ptr(), self.ind.hist.data_ptr(), 0, 1.0), 0, 0)
         self.cg_total = 0
         self.steps = 0
+        self.heat = 0.0
+        self.set_thermostat(thermostat, temperature, seed)
+
+    def set_thermostat(self, thermostat: Langevin | None, temperature: float = 298.0, seed: int = 0) -> None:
+        """Use `thermostat` (None: NVE) at `temperature` [K] for the following steps."""
+        gamma = 0.0 if thermostat is None else thermostat.friction
+        _lib.check(_lib.lib().pgm_md_set_thermostat(self.ff._ctx, gamma, float(temperature), int(seed) & (2**64 - 1)),
+                   "pgm_md_set_thermostat")
+        self.thermostat, self.temperature = thermostat, float(temperature)
 
     def run(self, nsteps: int, dt_fs: float = 1.0) -> dict:
         """Advance nsteps of dt_fs femtoseconds; return the report of the last step (energies in kJ/mol)."""
@@ -45,11 +77,12 @@ class RigidWaterMD:
         self.steps += nsteps
         self.ind.count = self._state.ind.count
         self.ind.norm = self._state.ind.norm
+        self.heat += rep.heat
         return {
             "epot": rep.epot, "elec": rep.elec, "vdw": rep.vdw, "ekin": rep.ekin, "etot": rep.etot,
             "temperature": rep.temperature, "cg_iterations": rep.cg_iterations_total,
             "cg_iterations_max": rep.cg_iterations_max, "cg_residual_max": rep.cg_residual_max,
-            "row_rebuilds": rep.row_rebuilds, "not_converged": st == _lib.PGM_NOT_CONVERGED,
+            "row_rebuilds": rep.row_rebuilds, "not_converged": st == _lib.PGM_NOT_CONVERGED, "heat": rep.heat,
         }
 
     def equilibrate(self, nsteps: int, temperature: float, dt_fs: float = 1.0, every: int = 10) -> dict:

@@ -19,6 +19,8 @@ class MDSettings:
     cutoff [nm], skin [nm], ewald_beta [1/nm], pme_grid (K1, K2, K3) or None (from pme_spacing [nm]),
     pme_order, lj_lrc, dipole_tol (max|alpha r| / mean|alpha b|), max_iter, predictor
     ("mu4", "mu3", "previous", "none"), fused, norm_refresh, peek, precision ("mixed" | "double").
+    Beyond the reference's MD settings: recip ("pme" | "ewald", the exact Ewald sum of the
+    PeriodicPGM reference model) and k_tol (its k-vector truncation).
     """
 
     elec: str = "qpi"
@@ -37,6 +39,8 @@ class MDSettings:
     norm_refresh: int = 1000
     peek: float = 0.65
     precision: str = "mixed"
+    recip: str = "pme"
+    k_tol: float = 1e-12
 
     def __post_init__(self) -> None:
         """Validate like PGMForceField.__init__ (forcefield.py:917-1000)."""
@@ -50,6 +54,10 @@ class MDSettings:
             raise ValueError("cutoff and ewald_beta must be positive")
         if not 4 <= int(self.pme_order) <= 8:
             raise ValueError("pme_order must be in 4..8")
+        if self.recip not in ("pme", "ewald"):
+            raise ValueError(f"recip must be 'pme' or 'ewald', got {self.recip!r}")
+        if not 0.0 < self.k_tol < 1.0:
+            raise ValueError("k_tol must be in (0, 1)")
 
     def replace(self, **changes) -> MDSettings:
         """Return a copy with some settings changed (TypeError for an unknown name, as the reference)."""

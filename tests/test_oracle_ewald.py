@@ -38,3 +38,14 @@ def test_min_image_is_the_nearest_of_27_images():
     imgs = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]) @ H
     r27 = np.min(np.linalg.norm((pos[:, None, None] - pos[None, :, None]) + imgs[None, None], axis=-1), axis=-1)
     assert np.allclose(np.minimum(r, 1.0), np.minimum(r27, 1.0))
+
+
+def test_periodic_model_strain_and_pressure_match_golden():
+    """Molecular and atomic strain derivatives (full 3x3) and the pressure vs periodic_model.npz."""
+    top, pos, H = systems.small_box(1)
+    ew = EwaldPGM(top, H, ewald_beta=3.8, cutoff=0.6, lj_lrc=True)
+    with np.load(GOLD) as g:
+        for mol, key in ((True, "strain_mol"), (False, "strain_atom")):
+            W = ew.strain_derivative(pos, molecular=mol)
+            assert np.abs(W - g[key]).max() < 1e-10 * np.abs(g[key]).max(), key
+        assert abs(ew.pressure(pos) - float(g["pressure"])) < 1e-10 * abs(float(g["pressure"]))

@@ -107,3 +107,29 @@ def test_strain_derivative_trace_by_isotropic_scaling():
     h = 1e-6
     fd = (e(h) - e(-h)) / (2 * h)
     assert abs(fd - (np.trace(W) + 3 * ff.vdw_tail(H))) < 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("molecular", [True, False])
+def test_strain_derivative_by_finite_differences(molecular):
+    """W_ab (a <= b: the box stays lower triangular) + E_lrc delta_ab equals the central difference of the
+    fixed-mu energy under the strain eps = h e_a e_b^T, molecular or atomic scaling."""
+    top, pos, H = systems.small_box(2)
+    ff = ForceField(top, H, SMALL)
+    res = ff.compute(pos, H)
+    W = ff.strain_derivative(pos, H, res.mu, molecular=molecular)
+    m = top.masses
+    com = np.array([np.average(pos[top.mol == k], 0, weights=m[top.mol == k]) for k in range(top.nmol)])
+
+    def e(eps):
+        F = np.eye(3) + eps
+        x = pos + (com @ eps.T)[top.mol] if molecular else pos @ F.T
+        return ff.energy_fixed_mu(x, H @ F.T, res.mu)
+
+    h = 1e-6
+    tail = ff.vdw_tail(H)
+    for a in range(3):
+        for b in range(a, 3):
+            E = np.zeros((3, 3))
+            E[a, b] = h
+            fd = (e(E) - e(-E)) / (2 * h)
+            assert abs(fd - (W[a, b] + (tail if a == b else 0.0))) < 2e-6 * np.abs(W).max(), (a, b, fd, W[a, b])

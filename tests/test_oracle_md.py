@@ -49,3 +49,48 @@ def test_energy_error_is_second_order_in_dt():
 
     e1, e2 = spread(2e-3), spread(1e-3)
     assert 3.0 < e1 / e2 < 5.0, (e1, e2)
+
+
+# Known-answer vectors of Philox4x32-10 from the Random123 distribution (kat_vectors):
+# (counter, key) -> output.
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0), (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+def test_philox_known_answers():
+    for ctr, key, want in PHILOX_KAT:
+        assert tuple(int(x) for x in md.philox4x32_10(np.array(ctr), key)) == want
+
+
+def test_langevin_normals_are_standard():
+    g = np.concatenate([md.langevin_normals(20000, s, seed=7).ravel() for s in range(3)])
+    assert abs(g.mean()) < 5e-3 and abs(g.std() - 1.0) < 5e-3
+    assert abs(np.mean(g**4) - 3.0) < 0.05  # Gaussian kurtosis
+    a, b = md.langevin_normals(1000, 0, 7), md.langevin_normals(1000, 1, 7)
+    assert abs(np.corrcoef(a.ravel(), b.ravel())[0, 1]) < 0.03  # steps independent
+
+
+def test_langevin_thermalises_free_rigid_water():
+    """Force-free rigid water under g-BAOAB: constraints hold, temperature -> target over 6 dof per molecule,
+    and the heat booked by the O steps equals the kinetic-energy change."""
+    top, pos, H = systems.water_box(216, 1.86, seed=4)
+    M = len(pos) // 3
+    vel = np.zeros_like(pos)
+    d2 = md.constraint_lengths2(pos)
+    kT = 0.0083144626181532 * 300.0
+    zero = lambda x: np.zeros_like(x)  # noqa: E731
+    F = zero(pos)
+    temps, heat = [], 0.0
+    ke0 = 0.0
+    for s in range(600):
+        pos, vel, F, q = md.langevin_step(pos, vel, F, top.masses, d2, 2e-3, 10.0, kT, 11, s, zero)
+        heat += q
+        assert np.abs(md.constraint_lengths2(pos) / d2 - 1).max() < 1e-12
+        ke = 0.5 * np.sum(top.masses[:, None] * vel**2)
+        if s >= 200:
+            temps.append(2 * ke / (6 * M * 0.0083144626181532))
+    assert abs(np.mean(temps) - 300.0) < 12.0, np.mean(temps)
+    assert abs(heat - (ke - ke0)) < 1e-9 * abs(heat)  # no forces: only the O steps change the kinetic energy
