@@ -56,7 +56,7 @@ typedef struct {
     int lj_lrc;         /* LJ long-range correction (Terms.lj_lrc) */
     double dipole_tol;  /* max|alpha r| / mean|alpha b| (Induction.tol) */
     int max_iter;       /* Induction.max_iter */
-    int predictor;      /* 0 none, 1 mu3, 2 mu4, 3 previous step (Induction.predictor) */
+    int predictor;      /* 0 none, 1 mu3, 2 mu4, 3 previous step (Induction.predictor), 4 learned (pgm_guess_*) */
     int fused;          /* fused initial residual (Induction.fused) */
     int norm_refresh;   /* Induction.norm_refresh */
     double peek;        /* Induction.peek */
@@ -179,6 +179,31 @@ int pgm_md_rescale(pgm_ctx *ctx, pgm_md_state *state, double factor);
  * ms[0] block merge of the tiled spread (grid clear without tiles), [1] spread, [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field,
  * [6] gather + combine, [7] one whole dipole-field matvec.  ms must hold 8 doubles. */
 int pgm_time_kernels(pgm_ctx *ctx, const double *pos, int reps, double *ms);
+
+/* Learned CG initial guess (no reference counterpart; MDSettings.predictor "learned" = 4): a
+ * per-atom MLP (16 features: the last four converged dipoles in a local frame of the molecule and
+ * an oxygen flag; hidden_layers x 128 SiLU; 3 outputs) predicts the correction to the mu4
+ * extrapolation; BF16 inference on the matrix cores, FP32 training (Adam) on rows harvested from
+ * the solves every harvest_every solves into a ring of `capacity` rows.  The guess is used only
+ * while its error on the latest harvest is below 0.8 x that of mu4 (else mu4); the CG always runs
+ * to the same tolerance.  Needs consecutive 3-site molecules.
+ *   pgm_guess_config: set up (weights LeCun normal from seed); dipole_scale [e nm] scales inputs.
+ *   pgm_guess_params: get (set = 0) or set (set = 1) the flat FP32 parameters (per layer W (in x
+ *     out) then b), host array of pgm_guess_nparams() floats.
+ *   pgm_guess_forward: y (n x 3) of feature rows x (n x 16), device FP32; bf16 != 0: the matrix-core
+ *     inference kernel, else the FP32 training forward.
+ *   pgm_guess_grad: gradients (host, flat) and loss of the mean squared error of rows x vs t (n x 3).
+ *   pgm_guess_train: `steps` Adam steps on minibatches of `batch` harvested rows; mean loss.
+ *   pgm_guess_status: out[6] = rows harvested, sigma, mu4 and network error^2 at the last harvest
+ *     (per component, sigma units), enabled, trained. */
+int pgm_guess_config(pgm_ctx *ctx, int hidden_layers, double dipole_scale, int harvest_every, int capacity,
+                     unsigned long long seed);
+long long pgm_guess_nparams(pgm_ctx *ctx);
+int pgm_guess_params(pgm_ctx *ctx, float *host, long long count, int set);
+int pgm_guess_forward(pgm_ctx *ctx, const float *x, int n, float *y, int bf16);
+int pgm_guess_grad(pgm_ctx *ctx, const float *x, const float *t, int n, float *grads, double *loss);
+int pgm_guess_train(pgm_ctx *ctx, int steps, int batch, double lr, double *loss);
+int pgm_guess_status(pgm_ctx *ctx, double *out);
 
 /* Measurement (no reference counterpart): live timing of the CG matvec kernels.  With on != 0,
  * every following dipole-field evaluation records HIP events around its row field (on the side

@@ -40,6 +40,13 @@ EXPORTS = (
     "pgm_time_kernels",
     "pgm_set_timing",
     "pgm_kernel_timing",
+    "pgm_guess_config",
+    "pgm_guess_nparams",
+    "pgm_guess_params",
+    "pgm_guess_forward",
+    "pgm_guess_grad",
+    "pgm_guess_train",
+    "pgm_guess_status",
 )
 
 
@@ -165,9 +172,17 @@ def lib() -> C.CDLL:
     L.pgm_time_kernels.argtypes = [P, P, I, C.POINTER(D)]
     L.pgm_md_rescale.argtypes = [P, C.POINTER(MDState), D]
     L.pgm_set_timing.argtypes = [P, I]
+    L.pgm_guess_config.argtypes = [P, I, D, I, I, C.c_ulonglong]
+    L.pgm_guess_nparams.argtypes = [P]
+    L.pgm_guess_nparams.restype = C.c_longlong
+    L.pgm_guess_params.argtypes = [P, P, C.c_longlong, I]
+    L.pgm_guess_forward.argtypes = [P, P, I, P, I]
+    L.pgm_guess_grad.argtypes = [P, P, P, I, P, C.POINTER(D)]
+    L.pgm_guess_train.argtypes = [P, I, I, D, C.POINTER(D)]
+    L.pgm_guess_status.argtypes = [P, C.POINTER(D)]
     L.pgm_kernel_timing.argtypes = [P, C.POINTER(D), C.POINTER(C.c_longlong)]
     for name in EXPORTS:
-        if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries"):
+        if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries", "pgm_guess_nparams"):
             getattr(L, name).restype = I
     _lib = L
     return L

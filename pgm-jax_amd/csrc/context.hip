@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <random>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,8 @@
 #include "virial.h"
 #include "ewald.h"
 #include "dense.h"
+#include "mlp.h"
+#include "guess.h"
 #include "pgm_hip.h"
 
 using namespace pgm;
@@ -167,6 +170,19 @@ struct pgm_ctx {
     };
     bool timing = false;
     Timed timed[3];
+    // learned CG initial guess (predictor 4): network GUESS_D -> 128 x nh -> 3, harvest ring, Adam
+    struct Guess {
+        bool on = false;        // configured
+        bool trained = false;   // at least one training call
+        bool enabled = false;   // the judge found it better than mu4 at the last harvest
+        int nh = 3, every = 10, cap = 0;
+        long long nparams = 0, rows = 0;
+        unsigned long long seed = 0;
+        int adam = 0;
+        double scale = 1.0, sigma = 0.0, err_mu4 = 0.0, err_net = 0.0;
+        bool have_y = false;    // y holds the network output of the current solve
+        DevBuf params, grads, m, v, packed, bias, feat, frames, y, bx, by, idx, xb, yb, acts, zs, delta, delta2, part, scal;
+    } guess;
     // MD (rigid 3-site molecules)
     DevBuf pos_build, minv, mass, d2, md_part, md_fail, heat;
     int nmol = 0;
@@ -327,6 +343,32 @@ static int set_box(pgm_ctx *c, const double *H) {
 
 // Tile widths of the spread: per axis the smallest T in 8..16 (else 7..P-1) with at least two tiles
 // and a last tile at least P-1 wide; no such T: the atomic spread is used.
+static size_t spread_lds_bytes(const pgm_ctx *c) {
+    const int P = c->s.order;
+    return (size_t)SPREAD_CHUNK * (6 * P + 4) * c->rsize + SPREAD_CHUNK * sizeof(int) + c->rsize * (size_t)c->tiles.bsize;
+}
+
+// Raise the dynamic-LDS limit of the tiled spread / gather instantiations of this context.
+template <typename real, int P> static int allow_lds_p(size_t spread, size_t gather) {
+    HIP_TRY(hipFuncSetAttribute((const void *)k_spread_tiles<real, P, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spread));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_spread_tiles<real, P, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spread));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_gather_tiles<real, P>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)gather));
+    return PGM_OK;
+}
+static int allow_lds(pgm_ctx *c, size_t spread, size_t gather) {
+    if (spread > 160 * 1024 || gather > 160 * 1024) {
+        g_err = "PME tile block exceeds the LDS";
+        return PGM_ERR_ARG;
+    }
+    int r = PGM_OK;
+    if (c->mixed) {
+        ORDER_SWITCH(c->s.order, r = (allow_lds_p<float, PO>(spread, gather)));
+    } else {
+        ORDER_SWITCH(c->s.order, r = (allow_lds_p<double, PO>(spread, gather)));
+    }
+    return r;
+}
+
 static int setup_tiles(pgm_ctx *c) {
     const int P = c->s.order;
     PMETiles t{};
@@ -335,7 +377,7 @@ static int setup_tiles(pgm_ctx *c) {
         const int K = c->s.grid[a];
         int best = 0;
         std::vector<int> cand;
-        for (int T = 8; T <= 16; ++T) cand.push_back(T);
+        for (int T = 16; T >= 8; --T) cand.push_back(T);
         for (int T = 7; T >= P - 1; --T) cand.push_back(T);
         for (int T : cand) {
             const int nt = (K + T - 1) / T;
@@ -368,6 +410,11 @@ static int setup_tiles(pgm_ctx *c) {
     PGM_TRY(c->tend.ensure(sizeof(int) * t.ntiles));
     PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
     PGM_TRY(c->blocks.ensure(c->rsize * (size_t)t.ntiles * t.bsize));
+    // gather partials come per tile: the partial buffers must hold ntiles entries
+    for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * (size_t)t.ntiles)));
+    // dynamic LDS of the tiled spread and gather beyond the default limit
+    const size_t lds_spread = spread_lds_bytes(c), lds_gather = c->rsize * (size_t)t.bsize;
+    PGM_TRY(allow_lds(c, lds_spread, lds_gather));
     return PGM_OK;
 }
 
@@ -556,13 +603,13 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
     const int t_sp = energy ? -1 : timing_begin(c, 1, st);
     if (c->tiled) {
         const PMETiles &t = c->tiles;
-        const size_t lds = c->rsize * t.bsize;
+        const size_t lds = spread_lds_bytes(c);
         if (withq) {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, WAVE, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, 256, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         } else {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, 256, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
@@ -607,6 +654,36 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
 
 // One field evaluation of the dipoles (and charges) dq: spread -> FFT -> convolve -> FFT, the row
 // field, then the gather combined with the row field according to `mode` (pme.h).
+static double c_self(const pgm_ctx *c) { return 4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI); }
+
+// Number of per-block partials a field evaluation writes (MODE_FIELD / MODE_AV).
+static int gather_parts(const pgm_ctx *c) {
+    if (c->ewald) return nblk(c->n, EW_TILE);
+    return c->tiled ? c->tiles.ntiles : nblk(c->n, 256);
+}
+
+// PME gather + combine (tiled through LDS when the grid is tiled).
+template <typename real>
+static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const real4<real> *extra, real4<real> *out, double *part,
+                          const int *conv, int iter) {
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const real cself = (real)c_self(c);
+    if (c->tiled) {
+        const PMETiles &t = c->tiles;
+        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<t.ntiles, 256, c->rsize * t.bsize, st>>>(
+                                     t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                     c->loc.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                                     extra, cself, mode, out, part, conv, iter)));
+    } else {
+        ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
+                                     n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                                     extra, cself, mode, out, part, conv, iter)));
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
 template <typename real>
 static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, const real4<real> *extra, real4<real> *out,
                       double *part, int iter) {
@@ -651,15 +728,195 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
     HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     const int t_ga = timing_begin(c, 2, st);
-    ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
-                                 n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
-                                 extra, cself, mode, out, part, conv, iter)));
-    LAUNCH_CHECK();
+    PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter));
     timing_end(c, 2, t_ga, st);
     return PGM_OK;
 }
 
-static double c_self(const pgm_ctx *c) { return 4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI); }
+
+// ---------------------------------------------------------------------------- learned guess (host)
+// Layer sizes GUESS_D -> 128 (x nh) -> GUESS_OUT; flat FP32 parameters: per layer W (in x out, row
+// major) then b (out).
+static std::vector<int> guess_sizes(const pgm_ctx *c) {
+    std::vector<int> sz{GUESS_D};
+    for (int l = 0; l < c->guess.nh; ++l) sz.push_back(MLP_W);
+    sz.push_back(GUESS_OUT);
+    return sz;
+}
+
+static int guess_pack(pgm_ctx *c) {
+    auto &G = c->guess;
+    const std::vector<int> sz = guess_sizes(c);
+    hipStream_t st = c->stream;
+    long long off = 0, poff = 0, boff = 0;
+    for (size_t l = 0; l + 1 < sz.size(); ++l) {
+        const int in = sz[l], out = sz[l + 1], S = (in + 15) / 16, T = (out + 15) / 16;
+        k_mlp_pack<<<nblk((long long)T * S * 256, 256), 256, 0, st>>>(in, out, S, T, G.params.as<float>() + off,
+                                                                       G.packed.as<short>() + poff);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(G.bias.as<float>() + boff, G.params.as<float>() + off + (long long)in * out, sizeof(float) * out,
+                               hipMemcpyDeviceToDevice, st));
+        off += (long long)in * out + out;
+        poff += (long long)T * S * 256;
+        boff += out;
+    }
+    return PGM_OK;
+}
+
+template <int NH> static int guess_infer_nh(pgm_ctx *c, const float *x, int n, float *y) {
+    constexpr size_t lds = sizeof(short) * mlp_packed_shorts<1, NH>();
+    static bool attr = false;
+    if (!attr) {
+        HIP_TRY(hipFuncSetAttribute((const void *)k_mlp_infer<1, NH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = true;
+    }
+    const int tiles = (n + 15) / 16;
+    const int grid = std::max(1, std::min(512, (tiles + 3) / 4));
+    k_mlp_infer<1, NH><<<grid, 256, lds, c->stream>>>(n, GUESS_D, GUESS_OUT, c->guess.packed.as<short>(), c->guess.bias.as<float>(),
+                                                     x, y);
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// BF16 inference of n feature rows x (device) into y (device, n x GUESS_OUT).
+static int guess_infer(pgm_ctx *c, const float *x, int n, float *y) {
+    switch (c->guess.nh) {
+        case 1: return guess_infer_nh<1>(c, x, n, y);
+        case 2: return guess_infer_nh<2>(c, x, n, y);
+        case 3: return guess_infer_nh<3>(c, x, n, y);
+        case 4: return guess_infer_nh<4>(c, x, n, y);
+        default: g_err = "learned guess: 1..4 hidden layers"; return PGM_ERR_ARG;
+    }
+}
+
+static void gemm(hipStream_t st, int M, int N, int K, const float *A, long sam, long sak, const float *B, long sbk, long sbn,
+                 const float *bias, float *C, float *Z, int act) {
+    k_gemm<<<dim3((N + 15) / 16, (M + 15) / 16), 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, bias, C, Z, act);
+}
+
+// FP32 forward of B rows x (B x GUESS_D) keeping pre-activations (zs) and activations (acts) of
+// the hidden layers; output y (B x GUESS_OUT).
+static int guess_forward(pgm_ctx *c, const float *x, int B, float *y) {
+    auto &G = c->guess;
+    const std::vector<int> sz = guess_sizes(c);
+    hipStream_t st = c->stream;
+    const float *h = x;
+    long long off = 0;
+    for (size_t l = 0; l + 1 < sz.size(); ++l) {
+        const int in = sz[l], out = sz[l + 1];
+        const float *W = G.params.as<float>() + off, *b = W + (long long)in * out;
+        const bool last = l + 2 == sz.size();
+        float *Z = last ? nullptr : G.zs.as<float>() + (size_t)l * B * MLP_W;
+        float *A = last ? y : G.acts.as<float>() + (size_t)l * B * MLP_W;
+        gemm(st, B, out, in, h, in, 1, W, out, 1, b, A, Z, last ? 0 : 1);
+        h = A;
+        off += (long long)in * out + out;
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// Gradients of the mean squared error of the last forward (rows x, targets t) into grads; per-block
+// partial sums of the squared error into part.
+static int guess_backward(pgm_ctx *c, const float *x, const float *t, int B, const float *y) {
+    auto &G = c->guess;
+    const std::vector<int> sz = guess_sizes(c);
+    const int L = (int)sz.size() - 1;
+    hipStream_t st = c->stream;
+    std::vector<long long> off(L + 1, 0);
+    for (int l = 0; l < L; ++l) off[l + 1] = off[l] + (long long)sz[l] * sz[l + 1] + sz[l + 1];
+    const size_t cnt = (size_t)B * GUESS_OUT;
+    float *dcur = G.delta.as<float>(), *dnext = G.delta2.as<float>();
+    k_mse_grad<<<nblk((long long)cnt, 256), 256, 0, st>>>(cnt, y, t, dcur, G.part.as<double>());
+    for (int l = L - 1; l >= 0; --l) {
+        const int in = sz[l], out = sz[l + 1];
+        const float *hin = l == 0 ? x : G.acts.as<float>() + (size_t)(l - 1) * B * MLP_W;
+        float *gW = G.grads.as<float>() + off[l], *gb = gW + (long long)in * out;
+        gemm(st, in, out, B, hin, 1, in, dcur, out, 1, nullptr, gW, nullptr, 0);        // dW = h^T delta
+        k_colsum<<<nblk(out, 128), 128, 0, st>>>(B, out, dcur, gb);                      // db = sum delta
+        if (l > 0) {
+            const float *W = G.params.as<float>() + off[l];
+            gemm(st, B, in, out, dcur, out, 1, W, 1, out, nullptr, dnext, nullptr, 0);   // delta W^T
+            k_dsilu_mul<<<nblk((long long)B * in, 256), 256, 0, st>>>((size_t)B * in, G.zs.as<float>() + (size_t)(l - 1) * B * MLP_W,
+                                                                     dnext);
+            std::swap(dcur, dnext);
+        }
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+static int guess_alloc_batch(pgm_ctx *c, int B) {
+    auto &G = c->guess;
+    PGM_TRY(G.xb.ensure(sizeof(float) * (size_t)B * GUESS_D));
+    PGM_TRY(G.yb.ensure(sizeof(float) * (size_t)B * GUESS_OUT));
+    PGM_TRY(G.idx.ensure(sizeof(int) * (size_t)B));
+    PGM_TRY(G.acts.ensure(sizeof(float) * (size_t)G.nh * B * MLP_W));
+    PGM_TRY(G.zs.ensure(sizeof(float) * (size_t)G.nh * B * MLP_W));
+    PGM_TRY(G.delta.ensure(sizeof(float) * (size_t)B * MLP_W));
+    PGM_TRY(G.delta2.ensure(sizeof(float) * (size_t)B * MLP_W));
+    PGM_TRY(G.y.ensure(sizeof(float) * (size_t)std::max(B, c->n) * GUESS_OUT));
+    PGM_TRY(G.part.ensure(sizeof(double) * (nblk((long long)std::max(B * GUESS_OUT, c->n), 256) + 16)));
+    return PGM_OK;
+}
+
+// Before the solve (predictor 4, history of 4): features of every atom, the network's output when
+// trained, and x0 = the learned guess when the judge enabled it, else the mu4 extrapolation.
+template <typename real> static int guess_x0(pgm_ctx *c, const pgm_induction *ind, const double *pos, real4<real> *x0) {
+    auto &G = c->guess;
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    k_guess_features<<<nblk(n, 256), 256, 0, st>>>(n, pos, ind->hist, 1.0 / G.scale, G.feat.as<float>(), G.frames.as<double>());
+    LAUNCH_CHECK();
+    G.have_y = false;
+    if (G.trained) {
+        PGM_TRY(guess_infer(c, G.feat.as<float>(), n, G.y.as<float>()));
+        G.have_y = true;
+    }
+    if (G.enabled && G.have_y) {
+        k_guess_apply<real><<<nblk(n, 256), 256, 0, st>>>(n, ind->hist, G.frames.as<double>(), G.y.as<float>(), G.sigma, x0);
+    } else {
+        k_predict<real><<<nblk(n, 256), 256, 0, st>>>(n, ind->hist, 4.0, -6.0, 4.0, -1.0, x0);
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// After the solve, before the history push, every `every` solves: append (features, target) rows,
+// set sigma at the first harvest, and judge the network against mu4 on these rows (one sync).
+static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
+    auto &G = c->guess;
+    const int n = c->n;
+    hipStream_t st = c->stream;
+    const int nb = nblk(n, 256);
+    const bool first = G.sigma <= 0.0;
+    k_guess_harvest<<<nb, 256, 0, st>>>(n, G.feat.as<float>(), G.frames.as<double>(), ind->hist, ind->mu,
+                                        first ? 1.0 : 1.0 / G.sigma, G.rows, G.cap, G.bx.as<float>(), G.by.as<float>(),
+                                        G.part.as<double>());
+    k_reduce_partials<<<1, 256, 0, st>>>(G.part.as<double>(), nb, 0, G.scal.as<double>());
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 20, G.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double e_mu4 = c->h_pin_dbl[20] / (3.0 * n);
+    if (first) {  // sigma = the RMS error of mu4 per component; rescale the rows just written
+        G.sigma = std::sqrt(std::max(e_mu4, 1e-300));
+        k_guess_scale_rows<<<nb, 256, 0, st>>>(n, G.rows, G.cap, (float)(1.0 / G.sigma), G.by.as<float>());
+        LAUNCH_CHECK();
+        e_mu4 = 1.0;
+    }
+    G.err_mu4 = e_mu4;
+    if (G.have_y) {
+        k_guess_err<<<nb, 256, 0, st>>>(n, G.y.as<float>(), G.by.as<float>(), G.rows, G.cap, G.part.as<double>());
+        k_reduce_partials<<<1, 256, 0, st>>>(G.part.as<double>(), nb, 0, G.scal.as<double>() + 1);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 21, G.scal.as<double>() + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        G.err_net = c->h_pin_dbl[21] / (3.0 * n);
+        G.enabled = G.err_net < 0.8 * G.err_mu4;  // the judge: a clear gain over mu4 or back to mu4
+    }
+    G.rows += n;
+    return PGM_OK;
+}
 
 // ---------------------------------------------------------------------------- induced dipoles
 // Predictor, initial residual (fused or plain) and CG (pgm_jax/md/forcefield.py:1917-2008, :1714-1821).
@@ -688,8 +945,23 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         case 3: cf[0] = 1; K = 1; break;
         default: K = 0;
     }
+    const bool learned = c->s.predictor == 4 && c->guess.on;
+    if (c->s.predictor == 4) {  // learned guess; mu4 / previous-step until the history is there
+        if (ind->count >= 4) {
+            K = 4;
+        } else {
+            cf[0] = 1;
+            K = 1;
+        }
+    }
     const bool have = K > 0 && ind->count >= K;
-    if (have) {
+    const bool use_net = learned && ind->count >= 4;
+    if (use_net) {
+        PGM_TRY(guess_x0<real>(c, ind, c->pos, x0));
+    } else if (have) {
+        if (c->s.predictor == 4 && K == 4) {
+            cf[0] = 4, cf[1] = -6, cf[2] = 4, cf[3] = -1;
+        }
         k_predict<real><<<nb, 256, 0, st>>>(n, ind->hist, cf[0], cf[1], cf[2], cf[3], x0);
         LAUNCH_CHECK();
     }
@@ -703,7 +975,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, q, dq);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, dq, true, MODE_FIELD, nullptr, b, c->partA.as<double>(), -1));
-        k_set_norm<<<1, 256, 0, st>>>(c->partA.as<double>(), nb, 1.0 / (3.0 * n), S);
+        k_set_norm<<<1, 256, 0, st>>>(c->partA.as<double>(), gather_parts(c), 1.0 / (3.0 * n), S);
         if (!have) k_scale_alpha<real><<<nb, 256, 0, st>>>(n, b, c->alpha_r.as<real>(), x0);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, x0, false, MODE_PLAIN, b, r, nullptr, -1));
@@ -727,7 +999,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     while (launched < max_iter) {
         const int k = launched;
         PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
-        k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), nb, rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
+        k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), gather_parts(c), rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
                                                c->partB.as<double>(), S);
         k_cg_update2<real><<<nb, 256, 0, st>>>(n, c->partB.as<double>(), nb, rz, err, k, z, p, S, c->s.dipole_tol, max_iter);
         LAUNCH_CHECK();
@@ -747,6 +1019,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     const int it = std::min(conv, launched);
     c->last_iters = std::max(1, it);
     k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
+    if (use_net && c->guess.every > 0 && ind->count % c->guess.every == 0) PGM_TRY(guess_harvest(c, ind));
     k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
     k_record_residual<<<1, 64, 0, st>>>(err, it, c->stats.as<double>() + (c->stats_slot % c->stats_cap));
     LAUNCH_CHECK();
@@ -1055,7 +1328,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
             return fail(PGM_ERR_HIP);
         }
     if (hipHostMalloc((void **)&c->h_pin_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_pin_dbl, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pin_dbl, 32 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_pin_ring, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         return fail(PGM_ERR_HIP);
@@ -1103,7 +1376,11 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->spl, &c->loc, &c->blocks,
                      &c->pme_sort_tmp, &c->stats, &c->mol_ptr, &c->mol_atoms, &c->com, &c->off, &c->rec_dd,
-                     &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk};
+                     &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk,
+                     &c->guess.params, &c->guess.grads, &c->guess.m, &c->guess.v, &c->guess.packed, &c->guess.bias,
+                     &c->guess.feat, &c->guess.frames, &c->guess.y, &c->guess.bx, &c->guess.by, &c->guess.idx, &c->guess.xb,
+                     &c->guess.yb, &c->guess.acts, &c->guess.zs, &c->guess.delta, &c->guess.delta2, &c->guess.part,
+                     &c->guess.scal};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
     if (c->h_pin_ring) (void)hipHostFree(c->h_pin_ring);
@@ -1239,6 +1516,166 @@ extern "C" int pgm_strain_derivative(pgm_ctx *c, const double *pos, const double
     }
     return c->mixed ? ef_api<float>(c, pos, mu, energy, forces, masses, strain, molecular != 0)
                     : ef_api<double>(c, pos, mu, energy, forces, masses, strain, molecular != 0);
+}
+
+// ---------------------------------------------------------------------------- learned guess (ABI)
+extern "C" int pgm_guess_config(pgm_ctx *c, int hidden_layers, double dipole_scale, int harvest_every, int capacity,
+                                unsigned long long seed) {
+    if (!c || hidden_layers < 1 || hidden_layers > 4 || !(dipole_scale > 0) || harvest_every < 1 || capacity < c->n) {
+        g_err = "pgm_guess_config: 1..4 hidden layers, dipole_scale > 0, harvest_every >= 1, capacity >= n_atoms";
+        return PGM_ERR_ARG;
+    }
+    if (c->n % 3 != 0) {
+        g_err = "pgm_guess_config: the learned guess needs consecutive 3-site molecules";
+        return PGM_ERR_ARG;
+    }
+    for (int i = 0; i < c->n; ++i)
+        if (c->h_mol[i] != i / 3) {
+            g_err = "pgm_guess_config: the learned guess needs consecutive 3-site molecules";
+            return PGM_ERR_ARG;
+        }
+    auto &G = c->guess;
+    G = pgm_ctx::Guess();
+    G.nh = hidden_layers;
+    G.scale = dipole_scale;
+    G.every = harvest_every;
+    G.cap = capacity;
+    G.seed = seed;
+    const std::vector<int> sz = guess_sizes(c);
+    long long np = 0, packed = 0, nbias = 0;
+    for (size_t l = 0; l + 1 < sz.size(); ++l) {
+        np += (long long)sz[l] * sz[l + 1] + sz[l + 1];
+        packed += (long long)((sz[l + 1] + 15) / 16) * ((sz[l] + 15) / 16) * 256;
+        nbias += sz[l + 1];
+    }
+    G.nparams = np;
+    for (DevBuf *b : {&G.params, &G.grads, &G.m, &G.v}) PGM_TRY(b->ensure(sizeof(float) * np));
+    PGM_TRY(G.packed.ensure(sizeof(short) * packed));
+    PGM_TRY(G.bias.ensure(sizeof(float) * (nbias + 16)));
+    PGM_TRY(G.feat.ensure(sizeof(float) * (size_t)c->n * GUESS_D));
+    PGM_TRY(G.frames.ensure(sizeof(double) * (size_t)c->n * 9));
+    PGM_TRY(G.bx.ensure(sizeof(float) * (size_t)capacity * GUESS_D));
+    PGM_TRY(G.by.ensure(sizeof(float) * (size_t)capacity * GUESS_OUT));
+    PGM_TRY(G.scal.ensure(sizeof(double) * 4));
+    PGM_TRY(guess_alloc_batch(c, 1024));
+    // LeCun-normal weights (Box-Muller on a 64-bit Mersenne twister keyed by seed), zero biases
+    std::vector<float> h((size_t)np, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto normal = [&]() {
+        const double u1 = ((double)(rng() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = ((double)(rng() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    long long off = 0;
+    for (size_t l = 0; l + 1 < sz.size(); ++l) {
+        const double sd = 1.0 / std::sqrt((double)sz[l]);
+        for (long long k = 0; k < (long long)sz[l] * sz[l + 1]; ++k) h[(size_t)(off + k)] = (float)(sd * normal());
+        off += (long long)sz[l] * sz[l + 1] + sz[l + 1];
+    }
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(G.params.p, h.data(), sizeof(float) * np, hipMemcpyHostToDevice, st));
+    k_fill<float><<<nblk(np, 256), 256, 0, st>>>((size_t)np, G.m.as<float>(), 0.0f);
+    k_fill<float><<<nblk(np, 256), 256, 0, st>>>((size_t)np, G.v.as<float>(), 0.0f);
+    LAUNCH_CHECK();
+    PGM_TRY(guess_pack(c));
+    HIP_TRY(hipStreamSynchronize(st));
+    G.on = true;
+    return PGM_OK;
+}
+
+extern "C" int pgm_guess_params(pgm_ctx *c, float *host, long long count, int set) {
+    if (!c || !c->guess.on || !host || count != c->guess.nparams) {
+        g_err = "pgm_guess_params: configure first; count must equal the number of parameters";
+        return PGM_ERR_ARG;
+    }
+    hipStream_t st = c->stream;
+    if (set) {
+        HIP_TRY(hipMemcpyAsync(c->guess.params.p, host, sizeof(float) * count, hipMemcpyHostToDevice, st));
+        PGM_TRY(guess_pack(c));
+        c->guess.trained = true;
+    } else {
+        HIP_TRY(hipMemcpyAsync(host, c->guess.params.p, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return PGM_OK;
+}
+
+extern "C" long long pgm_guess_nparams(pgm_ctx *c) { return c && c->guess.on ? c->guess.nparams : -1; }
+
+extern "C" int pgm_guess_forward(pgm_ctx *c, const float *x, int n, float *y, int bf16) {
+    if (!c || !c->guess.on || !x || !y || n <= 0) return PGM_ERR_ARG;
+    if (bf16) {
+        PGM_TRY(guess_infer(c, x, n, y));
+    } else {
+        PGM_TRY(guess_alloc_batch(c, n));
+        PGM_TRY(guess_forward(c, x, n, y));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PGM_OK;
+}
+
+extern "C" int pgm_guess_grad(pgm_ctx *c, const float *x, const float *t, int n, float *grads, double *loss) {
+    if (!c || !c->guess.on || !x || !t || !grads || n <= 0) return PGM_ERR_ARG;
+    auto &G = c->guess;
+    hipStream_t st = c->stream;
+    PGM_TRY(guess_alloc_batch(c, n));
+    PGM_TRY(guess_forward(c, x, n, G.y.as<float>()));
+    PGM_TRY(guess_backward(c, x, t, n, G.y.as<float>()));
+    const int np = nblk((long long)n * GUESS_OUT, 256);
+    k_reduce_partials<<<1, 256, 0, st>>>(G.part.as<double>(), np, 0, G.scal.as<double>() + 2);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(grads, G.grads.p, sizeof(float) * G.nparams, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 22, G.scal.as<double>() + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (loss) *loss = c->h_pin_dbl[22] / ((double)n * GUESS_OUT);
+    return PGM_OK;
+}
+
+extern "C" int pgm_guess_train(pgm_ctx *c, int steps, int batch, double lr, double *loss) {
+    if (!c || !c->guess.on || steps < 0 || batch < 16 || !(lr > 0)) return PGM_ERR_ARG;
+    auto &G = c->guess;
+    const long long filled = std::min<long long>(G.rows, G.cap);
+    if (filled <= 0) {
+        g_err = "pgm_guess_train: no harvested rows yet";
+        return PGM_ERR_STATE;
+    }
+    hipStream_t st = c->stream;
+    PGM_TRY(guess_alloc_batch(c, batch));
+    k_fill<double><<<1, 64, 0, st>>>(1, G.scal.as<double>() + 3, 0.0);
+    const double b1 = 0.9, b2 = 0.999;
+    for (int k = 0; k < steps; ++k) {
+        k_sample_rows<<<nblk(batch, 256), 256, 0, st>>>(batch, filled, G.seed, (uint32_t)G.adam, G.idx.as<int>());
+        k_guess_batch<<<nblk(batch, 256), 256, 0, st>>>(batch, G.idx.as<int>(), G.bx.as<float>(), G.by.as<float>(),
+                                                         G.xb.as<float>(), G.yb.as<float>());
+        PGM_TRY(guess_forward(c, G.xb.as<float>(), batch, G.y.as<float>()));
+        PGM_TRY(guess_backward(c, G.xb.as<float>(), G.yb.as<float>(), batch, G.y.as<float>()));
+        k_accumulate_partials<<<1, 256, 0, st>>>(G.part.as<double>(), nblk((long long)batch * GUESS_OUT, 256), G.scal.as<double>() + 3);
+        G.adam += 1;
+        const float c1 = (float)(1.0 - std::pow(b1, G.adam)), c2 = (float)(1.0 - std::pow(b2, G.adam));
+        k_adam<<<nblk(G.nparams, 256), 256, 0, st>>>((size_t)G.nparams, G.params.as<float>(), G.grads.as<float>(), G.m.as<float>(),
+                                                     G.v.as<float>(), (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+        LAUNCH_CHECK();
+    }
+    PGM_TRY(guess_pack(c));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 23, G.scal.as<double>() + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (loss) *loss = steps > 0 ? c->h_pin_dbl[23] / ((double)steps * batch * GUESS_OUT) : 0.0;
+    G.trained = true;
+    return PGM_OK;
+}
+
+// out: rows harvested, sigma, mu4 error^2 and network error^2 (per component, sigma units) at the
+// last harvest, enabled (1/0), trained (1/0).
+extern "C" int pgm_guess_status(pgm_ctx *c, double *out) {
+    if (!c || !out) return PGM_ERR_ARG;
+    const auto &G = c->guess;
+    out[0] = (double)G.rows;
+    out[1] = G.sigma;
+    out[2] = G.err_mu4;
+    out[3] = G.err_net;
+    out[4] = G.enabled ? 1.0 : 0.0;
+    out[5] = G.trained ? 1.0 : 0.0;
+    return PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- live kernel timing
@@ -1554,9 +1991,9 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     int rc = PGM_OK;
     if (c->tiled) {
         const PMETiles &t = c->tiles;
-        const size_t lds = c->rsize * t.bsize;
+        const size_t lds = spread_lds_bytes(c);
         ORDER_SWITCH(c->s.order, rc = timed(1, [&] {
-                                     k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+                                     k_spread_tiles<real, PO, false><<<t.ntiles, 256, lds, st>>>(
                                          t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                          c->loc.as<int>(), dq, c->blocks.as<real>(), nullptr, -1);
                                  }));
@@ -1592,11 +2029,7 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
             c->vrow.as<R4>(), nullptr, -1);
     });
     if (rc < 0) return rc;
-    ORDER_SWITCH(c->s.order, rc = timed(6, [&] {
-                                 k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
-                                     n, pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<R4>(), c->alpha_r.as<real>(), nullptr, cself,
-                                     MODE_AV, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1);
-                             }));
+    rc = timed(6, [&] { (void)gather_combine<real>(c, dq, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1); });
     if (rc < 0) return rc;
     rc = timed(7, [&] { field_eval<real>(c, dq, false, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), -1); });
     if (rc < 0) return rc;

@@ -152,6 +152,15 @@ PGM_DEV void normals4(uint32_t a, uint32_t b, uint32_t c, uint64_t seed, double 
     g[3] = r1 * sin(t1);
 }
 
+// Minibatch rows for training: idx[r] = Philox(r, counter, 0x5eed) mod filled.
+__global__ void k_sample_rows(int rows, long long filled, uint64_t seed, uint32_t counter, int *__restrict__ idx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const U4 u = philox4x32_10(U4{(uint32_t)r, counter, 0x5eedu, 0u}, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t w = ((uint64_t)u.x << 32) | u.y;
+    idx[r] = (int)(w % (uint64_t)filled);
+}
+
 // Velocity Verlet, first half (NVE): half kick, drift by dt, SHAKE; velocities corrected by the
 // constraint displacement / dt (the closing kick's RATTLE projects them).  d2[3 m + c] = squared
 // constraint lengths.

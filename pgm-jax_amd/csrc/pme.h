@@ -154,49 +154,75 @@ __global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, 
     loc[s] = l[0] | (l[1] << 10) | (l[2] << 20);
 }
 
+// Tiled spread: one 256-thread workgroup per grid tile accumulates the stencils of the atoms
+// binned to the tile into an LDS block of (T + P - 1)^3 points, written to `blocks` (merged by
+// k_merge_blocks).  The atoms' spline weights, scaled dipoles, charges and local offsets are staged
+// through LDS in chunks of SPREAD_CHUNK (one parallel load instead of a dependent chain per atom).
+// Ownership keeps it deterministic and free of atomics: wave w writes only the block x-planes
+// x = w (mod 4), so two waves never touch one point, and each wave adds the atoms in index order.
+constexpr int SPREAD_CHUNK = 64;
+
+template <typename real, int P>
+PGM_DEV constexpr int spread_lds_atoms() {
+    return SPREAD_CHUNK * (6 * P + 4) * (int)sizeof(real) + SPREAD_CHUNK * (int)sizeof(int);
+}
+
 template <typename real, int P, bool WITHQ>
-__global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
-                                                     const int *__restrict__ tend, const int *__restrict__ sidx,
-                                                     const real *__restrict__ spl, const int *__restrict__ loc,
-                                                     const real4<real> *__restrict__ dq, real *__restrict__ blocks,
-                                                     const int *__restrict__ conv_it, int iter) {
+__global__ __launch_bounds__(256) void k_spread_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
+                                                      const int *__restrict__ tend, const int *__restrict__ sidx,
+                                                      const real *__restrict__ spl, const int *__restrict__ loc,
+                                                      const real4<real> *__restrict__ dq, real *__restrict__ blocks,
+                                                      const int *__restrict__ conv_it, int iter) {
     if (conv_it && iter >= *conv_it) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    real *blk = reinterpret_cast<real *>(lds_raw);
-    const int lane = threadIdx.x;
+    constexpr int SW = 6 * P + 4;  // staged values per atom: splines, e0 e1 e2 q
+    real *sv = reinterpret_cast<real *>(lds_raw);
+    int *sl = reinterpret_cast<int *>(sv + SPREAD_CHUNK * SW);
+    real *blk = reinterpret_cast<real *>(lds_raw + spread_lds_atoms<real, P>());
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const int tile = blockIdx.x;
     const int BY = t.B[1], BZ = t.B[2];
-    for (int k = lane; k < t.bsize; k += WAVE) blk[k] = real(0);
-    __syncthreads();
+    for (int k = tid; k < t.bsize; k += blockDim.x) blk[k] = real(0);
     const int s0 = tstart[tile], s1 = tend[tile];
-    constexpr int NP = P * P * P;
-    constexpr int NCH = (NP + WAVE - 1) / WAVE;
-    for (int s = s0; s < s1; ++s) {
-        const real v = lane < 6 * P ? spl[(size_t)s * 6 * P + lane] : real(0);
-        const real4<real> d = dq[sidx[s]];
-        real e[3];
-        scaled_dipole<real>(d, g, e);
-        const real q = WITHQ ? d.w : real(0);
-        const int lc = loc[s];
-        const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            const int p = c * WAVE + lane;
-            const int pp = p < NP ? p : NP - 1;
-            const int ix = pp / (P * P), iy = (pp / P) % P, iz = pp % P;
-            const real tx = __shfl(v, ix, WAVE), dx = __shfl(v, P + ix, WAVE);
-            const real ty = __shfl(v, 2 * P + iy, WAVE), dy = __shfl(v, 3 * P + iy, WAVE);
-            const real tz = __shfl(v, 4 * P + iz, WAVE), dz = __shfl(v, 5 * P + iz, WAVE);
-            const real w = (q * tx + e[0] * dx) * ty * tz + tx * (e[1] * dy * tz + e[2] * ty * dz);
-            if (p < NP) {
-                const int a = ((lx + ix) * BY + (ly + iy)) * BZ + (lz + iz);
-                blk[a] += w;
+    for (int c0 = s0; c0 < s1; c0 += SPREAD_CHUNK) {
+        const int m = min(SPREAD_CHUNK, s1 - c0);
+        __syncthreads();  // the previous chunk is consumed (and the block is zeroed)
+        for (int k = tid; k < m * 6 * P; k += blockDim.x) {
+            const int a = k / (6 * P), v = k - a * 6 * P;
+            sv[a * SW + v] = spl[(size_t)(c0 + a) * 6 * P + v];
+        }
+        if (tid < m) {
+            const real4<real> d = dq[sidx[c0 + tid]];
+            real e[3];
+            scaled_dipole<real>(d, g, e);
+            sv[tid * SW + 6 * P] = e[0];
+            sv[tid * SW + 6 * P + 1] = e[1];
+            sv[tid * SW + 6 * P + 2] = e[2];
+            sv[tid * SW + 6 * P + 3] = WITHQ ? d.w : real(0);
+            sl[tid] = loc[c0 + tid];
+        }
+        __syncthreads();
+        for (int a = 0; a < m; ++a) {
+            const real *v = sv + a * SW;
+            const int lc = sl[a];
+            const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
+            const int ix0 = (wave - lx) & 3;         // first stencil x-plane this wave owns
+            const int np = ix0 + 4 < P ? 2 : (ix0 < P ? 1 : 0);
+            const real e0 = v[6 * P], e1 = v[6 * P + 1], e2 = v[6 * P + 2], q = v[6 * P + 3];
+            for (int p = lane; p < np * P * P; p += WAVE) {
+                const int ix = ix0 + 4 * (p / (P * P));
+                const int iy = (p / P) % P, iz = p % P;
+                const real tx = v[ix], dx = v[P + ix];
+                const real ty = v[2 * P + iy], dy = v[3 * P + iy];
+                const real tz = v[4 * P + iz], dz = v[5 * P + iz];
+                const real w = (q * tx + e0 * dx) * ty * tz + tx * (e1 * dy * tz + e2 * ty * dz);
+                blk[((lx + ix) * BY + (ly + iy)) * BZ + (lz + iz)] += w;
             }
         }
     }
     __syncthreads();
     real *out = blocks + (size_t)tile * t.bsize;
-    for (int k = lane; k < t.bsize; k += WAVE) out[k] = blk[k];
+    for (int k = tid; k < t.bsize; k += blockDim.x) out[k] = blk[k];
 }
 
 // Grid point value = sum of the blocks covering it: along each axis its own tile (local l) and,
@@ -363,6 +389,91 @@ PGM_DEV void combine_out(int mode, int i, real rx, real ry, real rz, const real4
         o.z = b.z - (d.z * ia + tz);
     }
     out[i] = o;
+}
+
+// Tiled gather + combine: one 256-thread workgroup per grid tile loads the potential on the
+// tile's (T + P - 1)^3 block (periodic wrap) into LDS once, then every atom binned to the tile
+// (thread per atom, sorted order s, atom sidx[s]) sums its 216 stencil values from LDS with the
+// spline weights stored by k_pme_splines, and combine_out writes its result.  Per-block partials
+// of MODE_FIELD / MODE_AV go to part[tile].
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
+                                                      const int *__restrict__ tend, const int *__restrict__ sidx,
+                                                      const real *__restrict__ spl, const int *__restrict__ loc,
+                                                      const real *__restrict__ phi, const real4<real> *__restrict__ dq,
+                                                      const real4<real> *__restrict__ row, const real *__restrict__ alpha,
+                                                      const real4<real> *__restrict__ extra, real cself, int mode,
+                                                      real4<real> *__restrict__ out, double *__restrict__ part,
+                                                      const int *__restrict__ conv_it, int iter) {
+    if (conv_it && iter >= *conv_it) return;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    real *blk = reinterpret_cast<real *>(lds_raw);
+    __shared__ double lds[16];
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int BX = t.B[0], BY = t.B[1], BZ = t.B[2];
+    const int u2 = tile % t.nt[2], u1 = (tile / t.nt[2]) % t.nt[1], u0 = tile / (t.nt[2] * t.nt[1]);
+    const int o0 = u0 * t.T[0], o1 = u1 * t.T[1], o2 = u2 * t.T[2];
+    const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    for (int k = tid; k < t.bsize; k += blockDim.x) {
+        const int bz = k % BZ, by = (k / BZ) % BY, bx = k / (BZ * BY);
+        int gx = o0 + bx, gy = o1 + by, gz = o2 + bz;
+        gx -= gx >= K1 ? K1 : 0;
+        gy -= gy >= K2 ? K2 : 0;
+        gz -= gz >= K3 ? K3 : 0;
+        blk[k] = phi[((size_t)gx * K2 + gy) * K3 + gz];
+    }
+    (void)BX;
+    __syncthreads();
+    const int s0 = tstart[tile], s1 = tend[tile];
+    double acc[1] = {0.0};
+    for (int s = s0 + tid; s < s1; s += blockDim.x) {
+        const real *v = spl + (size_t)s * 6 * P;
+        real t1[P], d1[P], t2[P], d2[P], t3[P], d3[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            t1[k] = v[k];
+            d1[k] = v[P + k];
+            t2[k] = v[2 * P + k];
+            d2[k] = v[3 * P + k];
+            t3[k] = v[4 * P + k];
+            d3[k] = v[5 * P + k];
+        }
+        const int lc = loc[s];
+        const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
+        real g1 = 0, g2 = 0, g3 = 0;
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) {
+            real s_tt = 0, s_dt = 0, s_td = 0;
+#pragma unroll
+            for (int iy = 0; iy < P; ++iy) {
+                const real *line = blk + ((lx + ix) * BY + (ly + iy)) * BZ + lz;
+                real lt = 0, ld = 0;
+#pragma unroll
+                for (int iz = 0; iz < P; ++iz) {
+                    const real w = line[iz];
+                    lt += w * t3[iz];
+                    ld += w * d3[iz];
+                }
+                s_tt += lt * t2[iy];
+                s_dt += lt * d2[iy];
+                s_td += ld * t2[iy];
+            }
+            g1 += s_tt * d1[ix];
+            g2 += s_dt * t1[ix];
+            g3 += s_td * t1[ix];
+        }
+        const double *es = g.escale;
+        const real rx = (real)es[0] * g1 + (real)es[1] * g2 + (real)es[2] * g3;
+        const real ry = (real)es[3] * g1 + (real)es[4] * g2 + (real)es[5] * g3;
+        const real rz = (real)es[6] * g1 + (real)es[7] * g2 + (real)es[8] * g3;
+        double a = 0.0;
+        combine_out<real>(mode, sidx[s], rx, ry, rz, dq, row, alpha, extra, cself, out, a);
+        acc[0] += a;
+    }
+    if (part) {
+        block_sum<1>(acc, lds);
+        if (tid == 0) part[tile] = acc[0];
+    }
 }
 
 template <typename real, int P>

@@ -238,6 +238,60 @@ class PGMForceField:
                                              C.byref(e), C.c_void_p(g.data_ptr())), "pgm_pme_energy")
         return e.value, g
 
+    # ------------------------------------------------------------------ learned CG guess
+    def guess_config(self, hidden_layers: int = 3, dipole_scale: float | None = None, harvest_every: int = 10,
+                     capacity: int | None = None, seed: int = 0) -> None:
+        """Set up the learned initial guess (used with predictor "learned"): pgm_guess_config.
+
+        dipole_scale [e nm] defaults to alpha_O x 100 e/nm^2 (a typical induced dipole); capacity
+        defaults to the larger of 200,000 rows and two harvests.
+        """
+        scale = float(dipole_scale) if dipole_scale else float(np.max(self.sys.alpha) * 100.0)
+        cap = int(capacity) if capacity else max(200_000, 2 * self.n)
+        _lib.check(_lib.lib().pgm_guess_config(self._ctx, int(hidden_layers), scale, int(harvest_every), cap, int(seed)),
+                   "pgm_guess_config")
+        self.guess_nparams = int(_lib.lib().pgm_guess_nparams(self._ctx))
+
+    def guess_params(self, params: np.ndarray | None = None) -> np.ndarray:
+        """Return (params None) or set the flat FP32 network parameters (per layer W (in x out), b)."""
+        buf = np.zeros(self.guess_nparams, np.float32) if params is None else np.ascontiguousarray(params, np.float32)
+        _lib.check(_lib.lib().pgm_guess_params(self._ctx, buf.ctypes.data_as(C.c_void_p), buf.size, int(params is not None)),
+                   "pgm_guess_params")
+        return buf
+
+    def guess_forward(self, x, bf16: bool = True):
+        """Network outputs (n, 3) of feature rows x (n, 16): matrix-core BF16 inference or the FP32 forward."""
+        torch = _torch()
+        x = torch.as_tensor(np.asarray(x, np.float32) if not isinstance(x, torch.Tensor) else x).to("cuda", torch.float32).contiguous()
+        y = torch.empty((x.shape[0], 3), dtype=torch.float32, device="cuda")
+        _lib.check(_lib.lib().pgm_guess_forward(self._ctx, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr()),
+                                                int(bool(bf16))), "pgm_guess_forward")
+        return y
+
+    def guess_grad(self, x, t) -> tuple[np.ndarray, float]:
+        """Return (flat gradients, loss) of the mean squared error of rows x (n, 16) against t (n, 3)."""
+        torch = _torch()
+        x = torch.as_tensor(np.asarray(x, np.float32)).to("cuda").contiguous()
+        t = torch.as_tensor(np.asarray(t, np.float32)).to("cuda").contiguous()
+        g = np.zeros(self.guess_nparams, np.float32)
+        loss = C.c_double()
+        _lib.check(_lib.lib().pgm_guess_grad(self._ctx, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+                                             g.ctypes.data_as(C.c_void_p), C.byref(loss)), "pgm_guess_grad")
+        return g, loss.value
+
+    def guess_train(self, steps: int = 200, batch: int = 4096, lr: float = 1e-3) -> float:
+        """Run Adam steps on the harvested rows; return the mean loss."""
+        loss = C.c_double()
+        _lib.check(_lib.lib().pgm_guess_train(self._ctx, int(steps), int(batch), float(lr), C.byref(loss)), "pgm_guess_train")
+        return loss.value
+
+    def guess_status(self) -> dict:
+        """Rows harvested, sigma, mu4 / network squared errors at the last harvest, enabled, trained."""
+        out = (C.c_double * 6)()
+        _lib.check(_lib.lib().pgm_guess_status(self._ctx, out), "pgm_guess_status")
+        return {"rows": int(out[0]), "sigma": out[1], "err_mu4": out[2], "err_net": out[3], "enabled": bool(out[4]),
+                "trained": bool(out[5])}
+
     def set_timing(self, on: bool = True) -> None:
         """Record HIP events around the CG matvec kernels from now on (pgm_set_timing); clears the sums."""
         _lib.check(_lib.lib().pgm_set_timing(self._ctx, int(bool(on))), "pgm_set_timing")

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import dataclasses
 
-PREDICTORS = {"none": 0, "mu3": 1, "mu4": 2, "previous": 3}
+PREDICTORS = {"none": 0, "mu3": 1, "mu4": 2, "previous": 3, "learned": 4}
 
 
 @dataclasses.dataclass(frozen=True)
@@ -18,7 +18,8 @@ class MDSettings:
 
     cutoff [nm], skin [nm], ewald_beta [1/nm], pme_grid (K1, K2, K3) or None (from pme_spacing [nm]),
     pme_order, lj_lrc, dipole_tol (max|alpha r| / mean|alpha b|), max_iter, predictor
-    ("mu4", "mu3", "previous", "none"), fused, norm_refresh, peek, precision ("mixed" | "double").
+    ("mu4", "mu3", "previous", "none"; "learned": the MLP guess of PGMForceField.guess_config, not in
+    the reference), fused, norm_refresh, peek, precision ("mixed" | "double").
     Beyond the reference's MD settings: recip ("pme" | "ewald", the exact Ewald sum of the
     PeriodicPGM reference model) and k_tol (its k-vector truncation).
     """
