@@ -4,16 +4,20 @@
 Workload (one GPU): 32,768 rigid pGM3P-25 waters (98,304 atoms) in a 9.93 nm cubic box (1.0
 g/cm^3), seeded jittered lattice with random orientations, Maxwell-Boltzmann rigid-body velocities
 at 298 K; qpi + LJ (long-range correction), cutoff 0.9 nm, Ewald beta 4.0/nm, PME 128^3 order 6,
-CG tolerance 1e-7 from the previous step's dipoles, NVE velocity Verlet + SHAKE/RATTLE (FP64), dt 1
-fs.  A step is one MD step (force field with the induced-dipole solve, constraints, integrator);
-inputs are resident in GPU memory.  The lattice start is first equilibrated (untimed, --equil
-steps with the velocities rescaled to 298 K every 10 steps).  `value` is the mixed-precision path (FP32 pair/PME kernels,
-FP64 accumulation, FP64 CG reductions); the plain FP64 path is timed beside it ("fp64"), as are
-the energy / force / dipole errors of mixed vs FP64 at the final configuration, the kernel
-roofline of the PME spread (HIP events on the library's stream) and the float64 numpy oracle on
-the host cores (cpu_baseline).
+CG tolerance 1e-7, NVE velocity Verlet + SHAKE/RATTLE (FP64), dt 1 fs.  A step is one MD step
+(force field with the induced-dipole solve, constraints, integrator); inputs are resident in GPU
+memory.  The lattice start is first equilibrated (untimed, --equil steps with the velocities
+rescaled to 298 K every 10 steps); with the learned guess (--guess learned, the default) the
+equilibration also harvests converged dipoles and trains the network.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+`value` is the mixed-precision path (FP32 pair/PME kernels, FP64 accumulation, FP64 CG reductions)
+with the learned CG initial guess; beside it: the same mixed path from the previous-step guess,
+the plain FP64 path from the previous-step guess ("fp64", the baseline of the metric), the energy /
+force / dipole errors of mixed vs FP64 at the final configuration, the HBM roofline of the
+dominant kernel (the CG matvec's row field; HIP events around every launch of the timed steps)
+with the PME spread / gather beside it, and the float64 numpy oracle on the host (cpu_baseline).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--guess learned|mu4|previous]
 """
 
 from __future__ import annotations
@@ -127,7 +131,7 @@ def roofline_of(live: dict, kern: dict, entries: int, n: int, K3: int) -> dict:
         "spread": {"kernels": "k_spread_tiles + k_merge_blocks", "achieved": round(gbs(spread_bytes, spread_ms), 2),
                    "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
                    "ms_per_launch": round(spread_ms, 5)},
-        "gather": {"kernels": "k_gather_combine", "achieved": round(gbs(gather_bytes, gather_ms), 2),
+        "gather": {"kernels": "k_gather_tiles", "achieved": round(gbs(gather_bytes, gather_ms), 2),
                    "frac": round(gbs(gather_bytes, gather_ms) / HBM_PEAK_GBS, 5),
                    "algorithmic_bytes": gather_bytes, "ms_per_launch": round(gather_ms, 5)},
         "isolated_ms": {k: round(v, 5) for k, v in kern.items()},
@@ -146,6 +150,10 @@ def main() -> int:  # noqa: C901
     ap.add_argument("--dt-fs", type=float, default=1.0)
     ap.add_argument("--equil", type=int, default=300, help="equilibration steps (velocity rescaling to 298 K), untimed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--guess", default="learned", choices=["learned", "mu4", "previous"],
+                    help="CG initial guess of the measured mixed path")
+    ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
+    ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build has no multi-GPU path", file=sys.stderr)
@@ -161,26 +169,38 @@ def main() -> int:  # noqa: C901
     sys_, pos, H = P.models.water_box(a.waters, a.edge, seed=3)
     vel = P.models.maxwell_boltzmann_rigid(pos, sys_.masses, 298.0, seed=4)
     base = dict(cutoff=0.9, skin=0.1, ewald_beta=4.0, pme_grid=(a.grid,) * 3, pme_order=6, lj_lrc=True,
-                dipole_tol=1e-7, max_iter=200, predictor="previous")
+                dipole_tol=1e-7, max_iter=200)
 
-    # equilibrate once (mixed precision, velocity rescaling to 298 K) from the lattice start
-    ffe = P.PGMForceField(sys_, H, P.MDSettings(precision="mixed", **base))
+    # Equilibrate once (mixed precision, velocity rescaling to 298 K) from the lattice start.  With the
+    # learned guess, this run harvests (features, converged dipoles) every 10 solves and trains the
+    # network between blocks of steps (untimed), so the timed run starts with a trained guess.
+    ffe = P.PGMForceField(sys_, H, P.MDSettings(precision="mixed", predictor=a.guess, **base))
+    if a.guess == "learned":
+        ffe.guess_config(hidden_layers=3, harvest_every=10, seed=11)
     mde = P.RigidWaterMD(ffe, pos, vel)
     if a.equil > 0:
         log("equilibrating", a.equil, "steps at 298 K")
-        r = mde.equilibrate(a.equil, 298.0, a.dt_fs)
+        done = 0
+        while done < a.equil:
+            k = min(100, a.equil - done)
+            r = mde.equilibrate(k, 298.0, a.dt_fs)
+            done += k
+            if a.guess == "learned" and ffe.guess_status()["rows"] > 0:
+                loss = ffe.guess_train(steps=a.train_steps, batch=4096, lr=1e-3)
+                log(f"  step {done}: guess training loss {loss:.4f}, status {json.dumps(ffe.guess_status())}")
         log(f"equilibrated: T {r['temperature']:.1f} K, Epot {r['epot']:.1f}")
     pos = mde.pos.cpu().numpy()
     vel = mde.vel.cpu().numpy()
-    del mde, ffe
 
-    def run(precision: str, steps: int, warmup: int, live_timing: bool = False):
-        ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, **base))
-        md = P.RigidWaterMD(ff, pos, vel)
-        log(precision, "warmup", warmup)
+    def run(precision: str, steps: int, warmup: int, live_timing: bool = False, predictor: str = "previous",
+            ff=None, md=None):
+        if ff is None:
+            ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, predictor=predictor, **base))
+            md = P.RigidWaterMD(ff, pos, vel)
+        log(precision, predictor, "warmup", warmup)
         md.run(warmup, a.dt_fs)
         torch.cuda.synchronize()
-        log(precision, "timed", steps)
+        log(precision, predictor, "timed", steps)
         if live_timing:
             ff.set_timing(True)
         t0 = time.perf_counter()
@@ -194,12 +214,21 @@ def main() -> int:  # noqa: C901
             f"Etot {rep['etot']:.3f}, rebuilds {rep['row_rebuilds']}")
         return ff, md, rep, ms
 
-    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, live_timing=True)
+    # the measured path: mixed precision with the guess (the equilibrated context keeps the trained network)
+    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, live_timing=True, predictor=a.guess, ff=ffe, md=mde)
+    guess_status = ffm.guess_status() if a.guess == "learned" else None
+    if guess_status:
+        log("learned guess after the timed steps:", json.dumps(guess_status))
     live = repm["kernel_timing"]
     log("live kernel timing over the timed steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
     entries = ffm.row_entries()
     log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}), "row entries", entries)
+    prev = None
+    if a.guess != "previous" and a.prev_steps > 0:  # the same mixed path from the previous-step guess
+        _, _, repp, ms_p = run("mixed", a.prev_steps, max(2, a.warmup // 4), predictor="previous")
+        prev = {"ms_per_step": round(ms_p, 4), "value": round(ns_per_day(ms_p, a.dt_fs), 3), "steps": a.prev_steps,
+                "cg_iters_per_step": round(repp["cg_iterations"] / a.prev_steps, 3)}
     errors, fp64 = None, None
     if a.fp64_steps > 0:
         ffd, mdd, repd, ms_d = run("double", a.fp64_steps, max(2, a.warmup // 4))
@@ -240,11 +269,14 @@ def main() -> int:  # noqa: C901
         "data": "synthetic",
         "config": {
             "workload": (f"{a.waters} rigid pGM3P-25 waters ({n} atoms), {a.edge} nm cubic box, qpi+LJ(lrc), rc 0.9 nm, "
-                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the previous-step dipoles, NVE "
-                         f"dt {a.dt_fs} fs, FP32 kernels with FP64 accumulation"),
-            "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": "previous-step",
+                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the {a.guess} guess, NVE "
+                         f"dt {a.dt_fs} fs, FP32 kernels with FP64 accumulation; baseline: the FP64 path from the "
+                         f"previous-step guess"),
+            "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": a.guess,
         },
         "cg_iters_per_step": round(repm["cg_iterations"] / a.steps, 3),
+        "learned_guess": guess_status,
+        "mixed_previous_guess": prev,
         "fp64": fp64,
         "errors_vs_fp64": None if errors is None else {k: float(f"{v:.3e}") for k, v in errors.items()},
         "energy_drift_check": {"etot_end_kJmol": repm["etot"], "temperature_K": round(repm["temperature"], 2)},

@@ -79,7 +79,8 @@ def water512(ref: str) -> dict:
     cov = [(i, catm[k], cdip[k]) for i in range(3) for k in range(start[i], start[i + 1])]
     nc = netcdf_file(os.path.join(ref, "examples/water512/pgm3p25_512.rst7"), "r", mmap=False)
     xyz = np.array(nc.variables["coordinates"].data, float)
-    vel = np.array(nc.variables["velocities"].data, float)
+    # Amber stores velocities in A / (1/20.455 ps): the file's scale_factor (default 20.455) gives A/ps
+    vel = np.array(nc.variables["velocities"].data, float) * float(getattr(nc.variables["velocities"], "scale_factor", 20.455))
     lengths = np.array(nc.variables["cell_lengths"].data, float)
     angles = np.array(nc.variables["cell_angles"].data, float)
     nc.close()

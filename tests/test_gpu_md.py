@@ -81,3 +81,24 @@ def test_langevin_holds_temperature():
     assert abs(np.mean(temps) - 298.0) < 0.04 * 298.0
     x = dyn.pos.cpu().numpy()
     assert np.abs(omd.constraint_lengths2(x) / omd.constraint_lengths2(pos) - 1).max() < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_water512_ten_steps_known_answer(precision):
+    """examples/water512, 10 NVE steps of 1 fs with the reference's MD defaults on the GPU: Epot(10) and T(10)
+    against the reference's recorded run (-2114884.256504 kJ/mol, 291.356783 K; tests/test_water512.py:26-27).
+    SHAKE/RATTLE vs the reference's rigid-body integrator: 1e-6 relative in Epot, 1e-4 in T."""
+    import pgm_jax_amd as P
+
+    top, pos, H, vel = systems.water512()
+    print(f"water512 md {precision}", flush=True)
+    sys_ = P.System(q=top.q, radius=top.radius, alpha=top.alpha, lj_rmin_half=top.lj_rmin_half, lj_sqrt_eps=top.lj_sqrt_eps,
+                    masses=top.masses, mol=top.mol, cov_i=top.cov_i, cov_j=top.cov_j, cov_c=top.cov_c)
+    ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision))
+    dyn = P.RigidWaterMD(ff, pos, vel)
+    r = dyn.run(10, 1.0)
+    print(r, flush=True)
+    M = len(pos) // 3
+    T = 2 * r["ekin"] / ((6 * M - 3) * 0.0083144626181532)
+    assert abs(r["epot"] - (-2114884.256504)) < 1e-6 * 2114884.256504, r["epot"]
+    assert abs(T - 291.356783) < 1e-4 * 291.356783, T

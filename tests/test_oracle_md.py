@@ -94,3 +94,32 @@ def test_langevin_thermalises_free_rigid_water():
             temps.append(2 * ke / (6 * M * 0.0083144626181532))
     assert abs(np.mean(temps) - 300.0) < 12.0, np.mean(temps)
     assert abs(heat - (ke - ke0)) < 1e-9 * abs(heat)  # no forces: only the O steps change the kinetic energy
+
+
+def test_water512_ten_steps_known_answer():
+    """examples/water512, 10 NVE steps of 1 fs with the reference's MD defaults: Epot(10) and T(10) against the
+    reference's recorded run (tests/test_water512.py:26-27: -2114884.256504 kJ/mol, 291.356783 K).
+
+    The reference integrates rigid bodies (NO_SQUISH); this is SHAKE/RATTLE, so agreement is to the
+    integrator difference: 5e-7 relative in Epot, 1e-4 in T (measured 1.5e-7 and 4e-5)."""
+    from oracle.forcefield import ForceField, Settings
+
+    top, pos, H, vel = systems.water512()
+    ff = ForceField(top, H, Settings())
+    state = {"ind": ff.init_induction()}
+
+    def fn(x):
+        r = ff.compute(x, H, state["ind"])
+        state["ind"], state["e"] = r.induction, r.energy["total"]
+        return r.forces
+
+    M = len(pos) // 3
+    minv = (1 / top.masses).reshape(M, 3)
+    v = md.rattle(pos.reshape(M, 3, 3), vel.reshape(M, 3, 3), minv).reshape(-1, 3)
+    d2 = md.constraint_lengths2(pos)
+    F = fn(pos)
+    for _ in range(10):
+        pos, v, F = md.step(pos, v, F, top.masses, d2, 1e-3, fn)
+    T = 2 * 0.5 * np.sum(top.masses[:, None] * v**2) / ((6 * M - 3) * 0.0083144626181532)
+    assert abs(state["e"] - (-2114884.256504)) < 5e-7 * 2114884.256504, state["e"]
+    assert abs(T - 291.356783) < 1e-4 * 291.356783, T
