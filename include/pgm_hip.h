@@ -180,22 +180,25 @@ int pgm_md_rescale(pgm_ctx *ctx, pgm_md_state *state, double factor);
  * [6] gather + combine, [7] one whole dipole-field matvec.  ms must hold 8 doubles. */
 int pgm_time_kernels(pgm_ctx *ctx, const double *pos, int reps, double *ms);
 
-/* Learned CG initial guess (no reference counterpart; MDSettings.predictor "learned" = 4): a
- * per-atom MLP (16 features: the last four converged dipoles in a local frame of the molecule and
- * an oxygen flag; hidden_layers x 128 SiLU; 3 outputs) predicts the correction to the mu4
- * extrapolation; BF16 inference on the matrix cores, FP32 training (Adam) on rows harvested from
- * the solves every harvest_every solves into a ring of `capacity` rows.  The guess is used only
- * while its error on the latest harvest is below 0.8 x that of mu4 (else mu4); the CG always runs
- * to the same tolerance.  Needs consecutive 3-site molecules.
+/* Learned CG initial guess (no reference counterpart; MDSettings.predictor "learned" = 4): a linear
+ * predictor over the last 8 converged dipoles whose coefficients are fitted by least squares to the
+ * dipoles the solves produce (refitted every harvest_every solves), plus a per-atom MLP correction
+ * (16 features: the last four dipoles in a local frame of the molecule and an oxygen flag;
+ * hidden_layers x 128 SiLU; 3 outputs) trained on the linear predictor's errors harvested into a
+ * ring of `capacity` rows.  BF16 inference on the matrix cores, FP32 training (Adam).  The judge,
+ * at every harvest: the linear predictor is used while its error is below 0.8 x that of mu4, the
+ * network on top of it while the sum's error is below 0.8 x the linear predictor's; else mu4.  The
+ * CG always runs to the same tolerance.  Needs consecutive 3-site molecules.
  *   pgm_guess_config: set up (weights LeCun normal from seed); dipole_scale [e nm] scales inputs.
- *   pgm_guess_params: get (set = 0) or set (set = 1) the flat FP32 parameters (per layer W (in x
- *     out) then b), host array of pgm_guess_nparams() floats.
+ *   pgm_guess_params: get (set = 0) or set (set = 1) the flat FP32 network parameters (per layer W
+ *     (in x out) then b), host array of pgm_guess_nparams() floats.
  *   pgm_guess_forward: y (n x 3) of feature rows x (n x 16), device FP32; bf16 != 0: the matrix-core
  *     inference kernel, else the FP32 training forward.
  *   pgm_guess_grad: gradients (host, flat) and loss of the mean squared error of rows x vs t (n x 3).
  *   pgm_guess_train: `steps` Adam steps on minibatches of `batch` harvested rows; mean loss.
- *   pgm_guess_status: out[6] = rows harvested, sigma, mu4 and network error^2 at the last harvest
- *     (per component, sigma units), enabled, trained. */
+ *   pgm_guess_status: out[16] = rows harvested, sigma, squared errors per component at the last
+ *     harvest of mu4, the linear predictor and linear + network, mode (0 mu4, 1 linear, 2 linear
+ *     + network), trained, number of fits, the eight linear coefficients (newest first). */
 int pgm_guess_config(pgm_ctx *ctx, int hidden_layers, double dipole_scale, int harvest_every, int capacity,
                      unsigned long long seed);
 long long pgm_guess_nparams(pgm_ctx *ctx);

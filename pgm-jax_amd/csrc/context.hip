@@ -174,14 +174,16 @@ struct pgm_ctx {
     struct Guess {
         bool on = false;        // configured
         bool trained = false;   // at least one training call
-        bool enabled = false;   // the judge found it better than mu4 at the last harvest
+        int mode = 0;           // the judge's choice: 0 mu4, 1 fitted linear predictor, 2 linear + network
         int nh = 3, every = 10, cap = 0;
         long long nparams = 0, rows = 0;
         unsigned long long seed = 0;
-        int adam = 0;
-        double scale = 1.0, sigma = 0.0, err_mu4 = 0.0, err_net = 0.0;
+        int adam = 0, head = 0, nring = 0, nfit = 0;
+        double scale = 1.0, sigma = 0.0, err_mu4 = 0.0, err_lin = 0.0, err_net = 0.0;
+        double gram[GUESS_K * GUESS_K] = {0}, rhs[GUESS_K] = {0};
+        GuessCoef lin{{4.0, -6.0, 4.0, -1.0, 0.0, 0.0, 0.0, 0.0}};
         bool have_y = false;    // y holds the network output of the current solve
-        DevBuf params, grads, m, v, packed, bias, feat, frames, y, bx, by, idx, xb, yb, acts, zs, delta, delta2, part, scal;
+        DevBuf params, grads, m, v, packed, bias, feat, frames, y, bx, by, idx, xb, yb, acts, zs, delta, delta2, part, scal, ring;
     } guess;
     // MD (rigid 3-site molecules)
     DevBuf pos_build, minv, mass, d2, md_part, md_fail, heat;
@@ -833,7 +835,7 @@ static int guess_backward(pgm_ctx *c, const float *x, const float *t, int B, con
         const float *hin = l == 0 ? x : G.acts.as<float>() + (size_t)(l - 1) * B * MLP_W;
         float *gW = G.grads.as<float>() + off[l], *gb = gW + (long long)in * out;
         gemm(st, in, out, B, hin, 1, in, dcur, out, 1, nullptr, gW, nullptr, 0);        // dW = h^T delta
-        k_colsum<<<nblk(out, 128), 128, 0, st>>>(B, out, dcur, gb);                      // db = sum delta
+        k_colsum<<<out, 256, 0, st>>>(B, out, dcur, gb);                                 // db = sum delta
         if (l > 0) {
             const float *W = G.params.as<float>() + off[l];
             gemm(st, B, in, out, dcur, out, 1, W, 1, out, nullptr, dnext, nullptr, 0);   // delta W^T
@@ -860,61 +862,101 @@ static int guess_alloc_batch(pgm_ctx *c, int B) {
     return PGM_OK;
 }
 
-// Before the solve (predictor 4, history of 4): features of every atom, the network's output when
-// trained, and x0 = the learned guess when the judge enabled it, else the mu4 extrapolation.
-template <typename real> static int guess_x0(pgm_ctx *c, const pgm_induction *ind, const double *pos, real4<real> *x0) {
+static const GuessCoef MU4_COEF{{4.0, -6.0, 4.0, -1.0, 0.0, 0.0, 0.0, 0.0}};
+
+// Before the solve (predictor 4, ring full): features of every atom, the network's output when
+// trained, and x0 by the judge's mode: the fitted linear predictor (+ the network), or mu4.
+template <typename real> static int guess_x0(pgm_ctx *c, const double *pos, real4<real> *x0) {
     auto &G = c->guess;
     const int n = c->n;
     hipStream_t st = c->stream;
-    k_guess_features<<<nblk(n, 256), 256, 0, st>>>(n, pos, ind->hist, 1.0 / G.scale, G.feat.as<float>(), G.frames.as<double>());
+    k_guess_features<<<nblk(n, 256), 256, 0, st>>>(n, pos, G.ring.as<double>(), G.head, 1.0 / G.scale, G.feat.as<float>(),
+                                                   G.frames.as<double>());
     LAUNCH_CHECK();
     G.have_y = false;
     if (G.trained) {
         PGM_TRY(guess_infer(c, G.feat.as<float>(), n, G.y.as<float>()));
         G.have_y = true;
     }
-    if (G.enabled && G.have_y) {
-        k_guess_apply<real><<<nblk(n, 256), 256, 0, st>>>(n, ind->hist, G.frames.as<double>(), G.y.as<float>(), G.sigma, x0);
-    } else {
-        k_predict<real><<<nblk(n, 256), 256, 0, st>>>(n, ind->hist, 4.0, -6.0, 4.0, -1.0, x0);
-    }
+    const GuessCoef &cf = G.mode >= 1 ? G.lin : MU4_COEF;
+    k_guess_x0<real><<<nblk(n, 256), 256, 0, st>>>(n, G.ring.as<double>(), G.head, cf, G.frames.as<double>(), G.y.as<float>(),
+                                                  G.sigma, G.mode == 2 && G.have_y ? 1 : 0, x0);
     LAUNCH_CHECK();
     return PGM_OK;
 }
 
-// After the solve, before the history push, every `every` solves: append (features, target) rows,
-// set sigma at the first harvest, and judge the network against mu4 on these rows (one sync).
+// Least squares (Gram + ridge) for the linear predictor; Gaussian elimination with pivoting.
+static bool solve_small(int K, const double *A0, const double *b0, double *x) {
+    std::vector<double> A(A0, A0 + K * K), b(b0, b0 + K);
+    double tr = 0.0;
+    for (int k = 0; k < K; ++k) tr += A[k * K + k];
+    for (int k = 0; k < K; ++k) A[k * K + k] += 1e-13 * tr / K;
+    for (int col = 0; col < K; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < K; ++r)
+            if (std::fabs(A[r * K + col]) > std::fabs(A[piv * K + col])) piv = r;
+        if (std::fabs(A[piv * K + col]) < 1e-300) return false;
+        if (piv != col) {
+            for (int k = 0; k < K; ++k) std::swap(A[col * K + k], A[piv * K + k]);
+            std::swap(b[col], b[piv]);
+        }
+        for (int r = col + 1; r < K; ++r) {
+            const double f = A[r * K + col] / A[col * K + col];
+            for (int k = col; k < K; ++k) A[r * K + k] -= f * A[col * K + k];
+            b[r] -= f * b[col];
+        }
+    }
+    for (int r = K - 1; r >= 0; --r) {
+        double v = b[r];
+        for (int k = r + 1; k < K; ++k) v -= A[r * K + k] * x[k];
+        x[r] = v / A[r * K + r];
+    }
+    return true;
+}
+
+// After the solve of a harvest step (ring full, before the ring push): statistics of the ring
+// against mu_t (one sync), the refit of the linear predictor, the network's training rows, and the
+// judge's choice for the following steps.
 static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
     auto &G = c->guess;
     const int n = c->n;
     hipStream_t st = c->stream;
     const int nb = nblk(n, 256);
-    const bool first = G.sigma <= 0.0;
-    k_guess_harvest<<<nb, 256, 0, st>>>(n, G.feat.as<float>(), G.frames.as<double>(), ind->hist, ind->mu,
-                                        first ? 1.0 : 1.0 / G.sigma, G.rows, G.cap, G.bx.as<float>(), G.by.as<float>(),
-                                        G.part.as<double>());
-    k_reduce_partials<<<1, 256, 0, st>>>(G.part.as<double>(), nb, 0, G.scal.as<double>());
+    PGM_TRY(G.part.ensure(sizeof(double) * (size_t)GUESS_STATS * nb));
+    PGM_TRY(G.scal.ensure(sizeof(double) * GUESS_STATS));
+    const long long row0 = G.sigma > 0.0 ? G.rows : -1;
+    k_guess_stats<<<nb, 256, 0, st>>>(n, G.ring.as<double>(), G.head, G.lin, MU4_COEF, ind->mu, G.frames.as<double>(),
+                                      G.y.as<float>(), G.sigma, G.have_y ? 1 : 0, G.feat.as<float>(), row0, G.cap, G.bx.as<float>(),
+                                      G.by.as<float>(), G.part.as<double>());
+    k_dense_reduce<<<1, 256, 0, st>>>(G.part.as<double>(), nb, GUESS_STATS, G.scal.as<double>());
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 20, G.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    double h[GUESS_STATS];
+    HIP_TRY(hipMemcpyAsync(h, G.scal.p, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    double e_mu4 = c->h_pin_dbl[20] / (3.0 * n);
-    if (first) {  // sigma = the RMS error of mu4 per component; rescale the rows just written
-        G.sigma = std::sqrt(std::max(e_mu4, 1e-300));
-        k_guess_scale_rows<<<nb, 256, 0, st>>>(n, G.rows, G.cap, (float)(1.0 / G.sigma), G.by.as<float>());
-        LAUNCH_CHECK();
-        e_mu4 = 1.0;
+    const double inv = 1.0 / (3.0 * n);
+    G.err_lin = h[44] * inv;
+    G.err_mu4 = h[45] * inv;
+    G.err_net = G.have_y ? h[46] * inv : 0.0;
+    // judge for the following steps (errors of this harvest, measured before the refit)
+    int mode = 0;
+    if (G.nfit > 0 && G.err_lin < 0.8 * G.err_mu4) mode = 1;
+    if (mode == 1 && G.have_y && G.err_net < 0.8 * G.err_lin) mode = 2;
+    G.mode = mode;
+    if (row0 >= 0) G.rows += n;
+    if (G.sigma <= 0.0) G.sigma = std::sqrt(std::max(G.err_lin, 1e-300));
+    // refit: exponentially weighted normal equations over the harvests
+    int q = 0;
+    for (int j = 0; j < GUESS_K; ++j)
+        for (int k = j; k < GUESS_K; ++k, ++q) {
+            G.gram[j * GUESS_K + k] = 0.9 * G.gram[j * GUESS_K + k] + h[q];
+            G.gram[k * GUESS_K + j] = G.gram[j * GUESS_K + k];
+        }
+    for (int j = 0; j < GUESS_K; ++j) G.rhs[j] = 0.9 * G.rhs[j] + h[36 + j];
+    double w[GUESS_K];
+    if (solve_small(GUESS_K, G.gram, G.rhs, w)) {
+        for (int k = 0; k < GUESS_K; ++k) G.lin.w[k] = w[k];
+        G.nfit += 1;
     }
-    G.err_mu4 = e_mu4;
-    if (G.have_y) {
-        k_guess_err<<<nb, 256, 0, st>>>(n, G.y.as<float>(), G.by.as<float>(), G.rows, G.cap, G.part.as<double>());
-        k_reduce_partials<<<1, 256, 0, st>>>(G.part.as<double>(), nb, 0, G.scal.as<double>() + 1);
-        LAUNCH_CHECK();
-        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 21, G.scal.as<double>() + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        G.err_net = c->h_pin_dbl[21] / (3.0 * n);
-        G.enabled = G.err_net < 0.8 * G.err_mu4;  // the judge: a clear gain over mu4 or back to mu4
-    }
-    G.rows += n;
     return PGM_OK;
 }
 
@@ -946,7 +988,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         default: K = 0;
     }
     const bool learned = c->s.predictor == 4 && c->guess.on;
-    if (c->s.predictor == 4) {  // learned guess; mu4 / previous-step until the history is there
+    if (c->s.predictor == 4) {  // learned guess; mu4 / previous-step until the histories are there
         if (ind->count >= 4) {
             K = 4;
         } else {
@@ -955,9 +997,9 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         }
     }
     const bool have = K > 0 && ind->count >= K;
-    const bool use_net = learned && ind->count >= 4;
+    const bool use_net = learned && c->guess.nring >= GUESS_K;
     if (use_net) {
-        PGM_TRY(guess_x0<real>(c, ind, c->pos, x0));
+        PGM_TRY(guess_x0<real>(c, c->pos, x0));
     } else if (have) {
         if (c->s.predictor == 4 && K == 4) {
             cf[0] = 4, cf[1] = -6, cf[2] = 4, cf[3] = -1;
@@ -1019,7 +1061,13 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     const int it = std::min(conv, launched);
     c->last_iters = std::max(1, it);
     k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
-    if (use_net && c->guess.every > 0 && ind->count % c->guess.every == 0) PGM_TRY(guess_harvest(c, ind));
+    if (use_net && ind->count % c->guess.every == 0) PGM_TRY(guess_harvest(c, ind));
+    if (learned) {  // the guess's own history ring
+        auto &G = c->guess;
+        k_ring_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, G.ring.as<double>(), G.head);
+        G.head = (G.head + 1) % GUESS_K;
+        G.nring = std::min(G.nring + 1, GUESS_K);
+    }
     k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
     k_record_residual<<<1, 64, 0, st>>>(err, it, c->stats.as<double>() + (c->stats_slot % c->stats_cap));
     LAUNCH_CHECK();
@@ -1380,7 +1428,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->guess.params, &c->guess.grads, &c->guess.m, &c->guess.v, &c->guess.packed, &c->guess.bias,
                      &c->guess.feat, &c->guess.frames, &c->guess.y, &c->guess.bx, &c->guess.by, &c->guess.idx, &c->guess.xb,
                      &c->guess.yb, &c->guess.acts, &c->guess.zs, &c->guess.delta, &c->guess.delta2, &c->guess.part,
-                     &c->guess.scal};
+                     &c->guess.scal, &c->guess.ring};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
     if (c->h_pin_ring) (void)hipHostFree(c->h_pin_ring);
@@ -1535,6 +1583,9 @@ extern "C" int pgm_guess_config(pgm_ctx *c, int hidden_layers, double dipole_sca
             return PGM_ERR_ARG;
         }
     auto &G = c->guess;
+    for (DevBuf *b : {&G.params, &G.grads, &G.m, &G.v, &G.packed, &G.bias, &G.feat, &G.frames, &G.y, &G.bx, &G.by, &G.idx,
+                      &G.xb, &G.yb, &G.acts, &G.zs, &G.delta, &G.delta2, &G.part, &G.scal, &G.ring})
+        b->release();
     G = pgm_ctx::Guess();
     G.nh = hidden_layers;
     G.scale = dipole_scale;
@@ -1554,6 +1605,7 @@ extern "C" int pgm_guess_config(pgm_ctx *c, int hidden_layers, double dipole_sca
     PGM_TRY(G.bias.ensure(sizeof(float) * (nbias + 16)));
     PGM_TRY(G.feat.ensure(sizeof(float) * (size_t)c->n * GUESS_D));
     PGM_TRY(G.frames.ensure(sizeof(double) * (size_t)c->n * 9));
+    PGM_TRY(G.ring.ensure(sizeof(double) * (size_t)GUESS_K * 3 * c->n));
     PGM_TRY(G.bx.ensure(sizeof(float) * (size_t)capacity * GUESS_D));
     PGM_TRY(G.by.ensure(sizeof(float) * (size_t)capacity * GUESS_OUT));
     PGM_TRY(G.scal.ensure(sizeof(double) * 4));
@@ -1664,17 +1716,21 @@ extern "C" int pgm_guess_train(pgm_ctx *c, int steps, int batch, double lr, doub
     return PGM_OK;
 }
 
-// out: rows harvested, sigma, mu4 error^2 and network error^2 (per component, sigma units) at the
-// last harvest, enabled (1/0), trained (1/0).
+// out[16]: rows harvested, sigma, squared errors per component at the last harvest of mu4, the
+// linear predictor and linear + network, mode (0 mu4, 1 linear, 2 linear + network), trained,
+// fits, and the eight linear coefficients (newest first).
 extern "C" int pgm_guess_status(pgm_ctx *c, double *out) {
     if (!c || !out) return PGM_ERR_ARG;
     const auto &G = c->guess;
     out[0] = (double)G.rows;
     out[1] = G.sigma;
     out[2] = G.err_mu4;
-    out[3] = G.err_net;
-    out[4] = G.enabled ? 1.0 : 0.0;
-    out[5] = G.trained ? 1.0 : 0.0;
+    out[3] = G.err_lin;
+    out[4] = G.err_net;
+    out[5] = G.mode;
+    out[6] = G.trained ? 1.0 : 0.0;
+    out[7] = G.nfit;
+    for (int k = 0; k < GUESS_K; ++k) out[8 + k] = G.lin.w[k];
     return PGM_OK;
 }
 

@@ -170,13 +170,15 @@ __global__ __launch_bounds__(256) void k_mse_grad(size_t count, const float *__r
     if (threadIdx.x == 0) part[blockIdx.x] = e[0];
 }
 
-// db[n] = sum_m delta[m][n] (one thread per column, fixed order).
-__global__ void k_colsum(int M, int N, const float *__restrict__ delta, float *__restrict__ db) {
-    const int nn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nn >= N) return;
-    float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += delta[(size_t)m * N + nn];
-    db[nn] = s;
+// db[n] = sum_m delta[m][n]: one 256-thread block per column n, strided rows, FP64 block sum
+// (fixed order).
+__global__ __launch_bounds__(256) void k_colsum(int M, int N, const float *__restrict__ delta, float *__restrict__ db) {
+    __shared__ double lds[16];
+    const int nn = blockIdx.x;
+    double v[1] = {0.0};
+    for (int m = threadIdx.x; m < M; m += blockDim.x) v[0] += (double)delta[(size_t)m * N + nn];
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) db[nn] = (float)v[0];
 }
 
 // Adam (Kingma and Ba, 2015) on `count` parameters: c1 = 1 - b1^step, c2 = 1 - b2^step.

@@ -286,11 +286,14 @@ class PGMForceField:
         return loss.value
 
     def guess_status(self) -> dict:
-        """Rows harvested, sigma, mu4 / network squared errors at the last harvest, enabled, trained."""
-        out = (C.c_double * 6)()
+        """Rows harvested, sigma, squared errors of mu4 / the linear predictor / linear + network at the last
+        harvest, the judge's mode, trained, number of fits, the linear coefficients."""
+        out = (C.c_double * 16)()
         _lib.check(_lib.lib().pgm_guess_status(self._ctx, out), "pgm_guess_status")
-        return {"rows": int(out[0]), "sigma": out[1], "err_mu4": out[2], "err_net": out[3], "enabled": bool(out[4]),
-                "trained": bool(out[5])}
+        modes = ("mu4", "linear", "linear+network")
+        return {"rows": int(out[0]), "sigma": out[1], "err_mu4": out[2], "err_linear": out[3], "err_net": out[4],
+                "mode": modes[int(out[5])], "trained": bool(out[6]), "fits": int(out[7]),
+                "coefficients": [round(v, 6) for v in out[8:16]]}
 
     def set_timing(self, on: bool = True) -> None:
         """Record HIP events around the CG matvec kernels from now on (pgm_set_timing); clears the sums."""
