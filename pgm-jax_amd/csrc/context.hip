@@ -116,6 +116,7 @@ struct pgm_ctx {
     std::vector<double> h_q, h_R2, h_alpha, h_rh, h_se;
     std::vector<int> h_mol;
     double sum_q = 0, sum_q2 = 0;
+    double lj_sb = 0;  // sum_{i,j} 2 eps_ij rmin_ij^6 (LJ tail)
     DevBuf q, alpha64, alpha_r, rh, se, mol, R2;
     DevBuf cstart, ci, cj, cc, jstart, jlist;
     // geometry: the positions of the current evaluation (caller's device buffer) and xr
@@ -243,16 +244,26 @@ static void inv3(const double *H, double *Hi) {
     Hi[8] = (a * e - b * d) / det;
 }
 
-static double lj_tail(const pgm_ctx *c) {
-    // -2 pi / (3 V rc^3) sum_{i,j} 2 eps_ij rmin_ij^6 by moments (pgm_jax/lj.py:87-114)
+// sum_{i,j} 2 eps_ij rmin_ij^6 by the binomial moments of rmin/2 (pgm_jax/lj.py:87-114): fixed by the
+// topology, computed once at creation.
+static double lj_sum_b(const pgm_ctx *c) {
     double M[7] = {0};
-    for (int i = 0; i < c->n; ++i)
-        for (int k = 0; k < 7; ++k) M[k] += c->h_se[i] * std::pow(c->h_rh[i], k);
+    for (int i = 0; i < c->n; ++i) {
+        double p = c->h_se[i];
+        for (int k = 0; k < 7; ++k) {
+            M[k] += p;
+            p *= c->h_rh[i];
+        }
+    }
     static const double C6[7] = {1, 6, 15, 20, 15, 6, 1};
     double SB = 0;
     for (int k = 0; k < 7; ++k) SB += C6[k] * M[k] * M[6 - k];
-    SB *= 2.0;
-    return -2.0 * M_PI * SB / (3.0 * c->volume * std::pow(c->s.cutoff, 3));
+    return 2.0 * SB;
+}
+
+// E_lrc = -2 pi / (3 V rc^3) sum_{i,j} 2 eps_ij rmin_ij^6
+static double lj_tail(const pgm_ctx *c) {
+    return -2.0 * M_PI * c->lj_sb / (3.0 * c->volume * c->s.cutoff * c->s.cutoff * c->s.cutoff);
 }
 
 template <typename T> static int upload(DevBuf &b, const T *h, size_t count, hipStream_t st) {
@@ -1273,6 +1284,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
             return PGM_ERR_ARG;
         }
     }
+    c->lj_sb = lj_sum_b(c);
     auto fail = [&](int r) {
         pgm_ctx_destroy(c);
         return r;
