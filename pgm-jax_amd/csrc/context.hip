@@ -140,9 +140,11 @@ struct pgm_ctx {
     rocfft_plan fwd = nullptr, bwd = nullptr;
     rocfft_execution_info info = nullptr;
     // tiled spread (atoms sorted by grid tile)
+    // spread tiles of T8 points per axis (tiles8, one fine key each) inside gather tiles of 2 T8
+    // (tiles); tstart/tend by fine key, tstart16/tend16 by gather tile
     bool tiled = false;
-    PMETiles tiles{};
-    DevBuf pkey, pidx, pskey, psidx, tstart, tend, spl, loc, blocks, pme_sort_tmp;
+    PMETiles tiles{}, tiles8{};
+    DevBuf pkey, pidx, pskey, psidx, tstart, tend, tstart16, tend16, spl, loc, loc16, blocks, pme_sort_tmp;
     // vectors (real4 N) and FP64 (N, 3) work arrays
     DevBuf vx, vr, vz, vp, vAp, vb, vx0, vrow, vdq, vtmp;
     DevBuf pperm, dd, g_el, g_lj, rowf, rec_dr, gp;
@@ -358,7 +360,7 @@ static int set_box(pgm_ctx *c, const double *H) {
 // and a last tile at least P-1 wide; no such T: the atomic spread is used.
 static size_t spread_lds_bytes(const pgm_ctx *c) {
     const int P = c->s.order;
-    return (size_t)SPREAD_CHUNK * (6 * P + 4) * c->rsize + SPREAD_CHUNK * sizeof(int) + c->rsize * (size_t)c->tiles.bsize;
+    return (size_t)SPREAD_CHUNK * (6 * P + 4) * c->rsize + SPREAD_CHUNK * sizeof(int) + c->rsize * (size_t)c->tiles8.bsize;
 }
 
 // Raise the dynamic-LDS limit of the tiled spread / gather instantiations of this context.
@@ -382,77 +384,87 @@ static int allow_lds(pgm_ctx *c, size_t spread, size_t gather) {
     return r;
 }
 
+// Spread tiles: per axis the largest T8 in 8..P-1 with at least three tiles and a last tile at least
+// P-1 wide; gather tiles of T16 = 2 T8 (the union of two spread tiles per axis).  No such T8: the
+// atomic spread and the per-atom gather are used.
 static int setup_tiles(pgm_ctx *c) {
     const int P = c->s.order;
-    PMETiles t{};
+    PMETiles t8{}, t16{};
     bool ok = true;
     for (int a = 0; a < 3; ++a) {
         const int K = c->s.grid[a];
         int best = 0;
-        std::vector<int> cand;
-        for (int T = 16; T >= 8; --T) cand.push_back(T);
-        for (int T = 7; T >= P - 1; --T) cand.push_back(T);
-        for (int T : cand) {
+        for (int T = 8; T >= P - 1 && !best; --T) {
             const int nt = (K + T - 1) / T;
             const int last = K - (nt - 1) * T;
-            if (nt >= 2 && last >= P - 1) {
-                best = T;
-                break;
-            }
+            if (nt >= 3 && last >= P - 1) best = T;
         }
         if (!best) {
             ok = false;
             break;
         }
-        t.T[a] = best;
-        t.nt[a] = (K + best - 1) / best;
-        t.B[a] = best + P - 1;
+        t8.T[a] = best;
+        t8.nt[a] = (K + best - 1) / best;
+        t8.B[a] = best + P - 1;
+        t16.T[a] = 2 * best;
+        t16.nt[a] = (t8.nt[a] + 1) / 2;
+        t16.B[a] = 2 * best + P - 1;
     }
     c->tiled = ok;
     if (!ok) return PGM_OK;
-    t.ntiles = t.nt[0] * t.nt[1] * t.nt[2];
-    t.bsize = t.B[0] * t.B[1] * t.B[2];
-    c->tiles = t;
+    t16.ntiles = t16.nt[0] * t16.nt[1] * t16.nt[2];
+    t16.bsize = t16.B[0] * t16.B[1] * t16.B[2];
+    t8.ntiles = 8 * t16.ntiles;  // by fine key (octants past the grid stay empty)
+    t8.bsize = t8.B[0] * t8.B[1] * t8.B[2];
+    c->tiles = t16;
+    c->tiles8 = t8;
     const int n = c->n;
     PGM_TRY(c->pkey.ensure(sizeof(int) * n));
     PGM_TRY(c->pidx.ensure(sizeof(int) * n));
     PGM_TRY(c->pskey.ensure(sizeof(int) * n));
     PGM_TRY(c->psidx.ensure(sizeof(int) * n));
     PGM_TRY(c->loc.ensure(sizeof(int) * n));
-    PGM_TRY(c->tstart.ensure(sizeof(int) * t.ntiles));
-    PGM_TRY(c->tend.ensure(sizeof(int) * t.ntiles));
+    PGM_TRY(c->loc16.ensure(sizeof(int) * n));
+    PGM_TRY(c->tstart.ensure(sizeof(int) * t8.ntiles));
+    PGM_TRY(c->tend.ensure(sizeof(int) * t8.ntiles));
+    PGM_TRY(c->tstart16.ensure(sizeof(int) * t16.ntiles));
+    PGM_TRY(c->tend16.ensure(sizeof(int) * t16.ntiles));
     PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
-    PGM_TRY(c->blocks.ensure(c->rsize * (size_t)t.ntiles * t.bsize));
-    // gather partials come per tile: the partial buffers must hold ntiles entries
-    for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * (size_t)t.ntiles)));
+    PGM_TRY(c->blocks.ensure(c->rsize * (size_t)t8.ntiles * t8.bsize));
+    // gather partials come per gather tile: the partial buffers must hold ntiles entries
+    for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * (size_t)t16.ntiles)));
     // dynamic LDS of the tiled spread and gather beyond the default limit
-    const size_t lds_spread = spread_lds_bytes(c), lds_gather = c->rsize * (size_t)t.bsize;
+    const size_t lds_spread = spread_lds_bytes(c), lds_gather = c->rsize * (size_t)t16.bsize;
     PGM_TRY(allow_lds(c, lds_spread, lds_gather));
     return PGM_OK;
 }
 
-// Sort the atoms by grid tile and store their spline weights (once per geometry).
+// Sort the atoms by fine key and store their spline weights (once per geometry).
 template <typename real> static int pme_bin(pgm_ctx *c, const double *pos) {
     if (!c->tiled) return PGM_OK;
     const int n = c->n;
     hipStream_t st = c->stream;
-    const PMETiles &t = c->tiles;
-    k_pme_bin<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t, c->pkey.as<int>(), c->pidx.as<int>());
+    const PMETiles &t8 = c->tiles8, &t16 = c->tiles;
+    k_pme_bin<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t8, t16, c->pkey.as<int>(), c->pidx.as<int>());
     LAUNCH_CHECK();
     int bits = 1;
-    while ((1 << bits) < t.ntiles) ++bits;
+    while ((1 << bits) < t8.ntiles) ++bits;
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
                                                c->psidx.as<int>(), n, 0, bits, st));
     PGM_TRY(c->pme_sort_tmp.ensure(tmp));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->pme_sort_tmp.p, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
                                                c->psidx.as<int>(), n, 0, bits, st));
-    k_fill_int<<<nblk(t.ntiles, 256), 256, 0, st>>>(t.ntiles, c->tstart.as<int>(), 0);
-    k_fill_int<<<nblk(t.ntiles, 256), 256, 0, st>>>(t.ntiles, c->tend.as<int>(), 0);
+    k_fill_int<<<nblk(t8.ntiles, 256), 256, 0, st>>>(t8.ntiles, c->tstart.as<int>(), 0);
+    k_fill_int<<<nblk(t8.ntiles, 256), 256, 0, st>>>(t8.ntiles, c->tend.as<int>(), 0);
+    k_fill_int<<<nblk(t16.ntiles, 256), 256, 0, st>>>(t16.ntiles, c->tstart16.as<int>(), 0);
+    k_fill_int<<<nblk(t16.ntiles, 256), 256, 0, st>>>(t16.ntiles, c->tend16.as<int>(), 0);
     k_cell_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->pskey.as<int>(), c->tstart.as<int>(), c->tend.as<int>());
+    k_key_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->pskey.as<int>(), 3, c->tstart16.as<int>(), c->tend16.as<int>());
     LAUNCH_CHECK();
-    ORDER_SWITCH(c->s.order, (k_pme_splines<real, PO><<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t, c->psidx.as<int>(),
-                                                                                    c->spl.as<real>(), c->loc.as<int>())));
+    ORDER_SWITCH(c->s.order, (k_pme_splines<real, PO><<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t8, c->psidx.as<int>(),
+                                                                                    c->spl.as<real>(), c->loc.as<int>(),
+                                                                                    c->loc16.as<int>())));
     LAUNCH_CHECK();
     return PGM_OK;
 }
@@ -615,20 +627,20 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
     const int P = c->s.order;
     const int t_sp = energy ? -1 : timing_begin(c, 1, st);
     if (c->tiled) {
-        const PMETiles &t = c->tiles;
+        const PMETiles &t = c->tiles8;
         const size_t lds = spread_lds_bytes(c);
         if (withq) {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, 256, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, WAVE, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         } else {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, 256, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
         LAUNCH_CHECK();
         ORDER_SWITCH(P, (k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
-                            t, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
+                            t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
     } else {
     k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
     if (withq) {
@@ -685,8 +697,8 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
     if (c->tiled) {
         const PMETiles &t = c->tiles;
         ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<t.ntiles, 256, c->rsize * t.bsize, st>>>(
-                                     t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
-                                     c->loc.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                                     t, c->pg, c->tstart16.as<int>(), c->tend16.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                     c->loc16.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
                                      extra, cself, mode, out, part, conv, iter)));
     } else {
         ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
@@ -1434,7 +1446,8 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
                      &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
-                     &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->spl, &c->loc, &c->blocks,
+                     &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
+                     &c->loc16, &c->blocks,
                      &c->pme_sort_tmp, &c->stats, &c->mol_ptr, &c->mol_atoms, &c->com, &c->off, &c->rec_dd,
                      &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk,
                      &c->guess.params, &c->guess.grads, &c->guess.m, &c->guess.v, &c->guess.packed, &c->guess.bias,
@@ -2058,17 +2071,17 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     };
     int rc = PGM_OK;
     if (c->tiled) {
-        const PMETiles &t = c->tiles;
+        const PMETiles &t = c->tiles8;
         const size_t lds = spread_lds_bytes(c);
         ORDER_SWITCH(c->s.order, rc = timed(1, [&] {
-                                     k_spread_tiles<real, PO, false><<<t.ntiles, 256, lds, st>>>(
+                                     k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
                                          t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                          c->loc.as<int>(), dq, c->blocks.as<real>(), nullptr, -1);
                                  }));
         if (rc < 0) return rc;
         ORDER_SWITCH(c->s.order, rc = timed(0, [&] {
                                      k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
-                                         t, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
+                                         t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
                                  }));
         if (rc < 0) return rc;
     } else {

@@ -221,10 +221,32 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
     const double4 a = xr[i];
     const int c = count[i];
     real ex = 0, ey = 0, ez = 0;
-    for (int e = lane; e < c; e += RL) {
-        const size_t q = ell(e, i, n);
-        const int j = jell[q];
-        row_field_entry<real, WITHQ>(a, xr[j], dq[j], box, G1[q], G2[q], ex, ey, ez);
+    // Entries in batches of RF_UNROLL per lane: the partner indices and kernels of the batch are
+    // loaded first, then the partners' positions and dipoles, so the dependent loads of the batch
+    // are in flight together instead of one chain per entry.  Slots past the row's end are masked
+    // (own index, zero kernels); the summation order per lane is unchanged.
+    constexpr int RF_UNROLL = 4;
+    for (int e0 = lane; e0 < c; e0 += RF_UNROLL * RL) {
+        int j[RF_UNROLL];
+        real g1[RF_UNROLL], g2[RF_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RF_UNROLL; ++u) {
+            const int e = e0 + u * RL;
+            const bool ok = e < c;
+            const size_t q = ell(ok ? e : 0, i, n);
+            j[u] = ok ? jell[q] : i;
+            g1[u] = ok ? G1[q] : real(0);
+            g2[u] = ok ? G2[q] : real(0);
+        }
+        double4 b[RF_UNROLL];
+        real4<real> d[RF_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RF_UNROLL; ++u) {
+            b[u] = xr[j[u]];
+            d[u] = dq[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < RF_UNROLL; ++u) row_field_entry<real, WITHQ>(a, b[u], d[u], box, g1[u], g2[u], ex, ey, ez);
     }
     ex = group_sum<RL>(ex);
     ey = group_sum<RL>(ey);

@@ -106,43 +106,54 @@ PGM_DEV int tile_size(const PMETiles &t, int a, int u, const PMEGeom &g) {
     return u < t.nt[a] - 1 ? t.T[a] : g.K[a] - (t.nt[a] - 1) * t.T[a];
 }
 
-// key = linear tile index of each atom's stencil base; idx = atom index.
-__global__ void k_pme_bin(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t, int *__restrict__ key,
+// The spread works on tiles of T8 points per axis, the gather on tiles of T16 = 2 T8 (each the
+// union of 2 x 2 x 2 spread tiles).  Atoms are sorted once per geometry by the fine key
+// key = t16 index * 8 + octant, so both the spread tiles (one key each) and the gather tiles (eight
+// consecutive keys) are contiguous ranges of the sorted order.
+PGM_DEV int fine_key(const int (&u8)[3], const PMETiles &t16) {
+    const int oct = ((u8[0] & 1) << 2) | ((u8[1] & 1) << 1) | (u8[2] & 1);
+    return (((u8[0] >> 1) * t16.nt[1] + (u8[1] >> 1)) * t16.nt[2] + (u8[2] >> 1)) * 8 + oct;
+}
+
+// key = fine key of each atom's stencil base; idx = atom index.
+__global__ void k_pme_bin(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t8, PMETiles t16, int *__restrict__ key,
                           int *__restrict__ idx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int base[3];
     double f[3];
     stencil<double, 4>(pos, i, g, base, f);
-    int tt[3];
+    int u8[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
-        int u = b / t.T[a];
-        tt[a] = u < t.nt[a] ? u : t.nt[a] - 1;
+        const int bb = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
+        const int u = bb / t8.T[a];
+        u8[a] = u < t8.nt[a] ? u : t8.nt[a] - 1;
     }
-    key[i] = (tt[0] * t.nt[1] + tt[1]) * t.nt[2] + tt[2];
+    key[i] = fine_key(u8, t16);
     idx[i] = i;
 }
 
-// Spline weights in tile-sorted order: spl[s] = (t0[P], d0[P], t1[P], d1[P], t2[P], d2[P]) of atom
-// sidx[s]; loc[s] = the stencil base relative to its tile start, packed (lx | ly << 10 | lz << 20).
+// Spline weights in sorted order: spl[s] = (t0[P], d0[P], t1[P], d1[P], t2[P], d2[P]) of atom
+// sidx[s]; loc8[s] / loc16[s] = the stencil base relative to its spread / gather tile start,
+// packed (lx | ly << 10 | lz << 20).
 template <typename real, int P>
-__global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t, const int *__restrict__ sidx,
-                              real *__restrict__ spl, int *__restrict__ loc) {
+__global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t8, const int *__restrict__ sidx,
+                              real *__restrict__ spl, int *__restrict__ loc8, int *__restrict__ loc16) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int i = sidx[s];
     int base[3];
     real f[3];
     stencil<real, P>(pos, i, g, base, f);
-    int l[3];
+    int l8[3], l16[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
-        int u = b / t.T[a];
-        u = u < t.nt[a] ? u : t.nt[a] - 1;
-        l[a] = b - u * t.T[a];
+        const int bb = base[a] >= g.K[a] ? base[a] - g.K[a] : base[a];
+        int u = bb / t8.T[a];
+        u = u < t8.nt[a] ? u : t8.nt[a] - 1;
+        l8[a] = bb - u * t8.T[a];
+        l16[a] = l8[a] + (u & 1) * t8.T[a];
         real th[P], dth[P];
         bspline<real, P, false>(f[a], th, dth, nullptr);
 #pragma unroll
@@ -151,15 +162,25 @@ __global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, 
             spl[(size_t)s * 6 * P + (2 * a + 1) * P + k] = dth[k];
         }
     }
-    loc[s] = l[0] | (l[1] << 10) | (l[2] << 20);
+    loc8[s] = l8[0] | (l8[1] << 10) | (l8[2] << 20);
+    loc16[s] = l16[0] | (l16[1] << 10) | (l16[2] << 20);
 }
 
-// Tiled spread: one 256-thread workgroup per grid tile accumulates the stencils of the atoms
-// binned to the tile into an LDS block of (T + P - 1)^3 points, written to `blocks` (merged by
-// k_merge_blocks).  The atoms' spline weights, scaled dipoles, charges and local offsets are staged
-// through LDS in chunks of SPREAD_CHUNK (one parallel load instead of a dependent chain per atom).
-// Ownership keeps it deterministic and free of atomics: wave w writes only the block x-planes
-// x = w (mod 4), so two waves never touch one point, and each wave adds the atoms in index order.
+// start / end of every key >> shift in the sorted keys (gather tiles: shift 3)
+__global__ void k_key_bounds(int n, const int *__restrict__ skey, int shift, int *__restrict__ start, int *__restrict__ end) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = skey[s] >> shift;
+    if (s == 0 || (skey[s - 1] >> shift) != k) start[k] = s;
+    if (s == n - 1 || (skey[s + 1] >> shift) != k) end[k] = s + 1;
+}
+
+// Tiled spread: one wave per spread tile (fine key) accumulates the stencils of its atoms into an
+// LDS block of (T8 + P - 1)^3 points, written to `blocks` (merged by k_merge_blocks).  The atoms'
+// spline weights, scaled dipoles, charges and local offsets are staged through LDS in chunks of
+// SPREAD_CHUNK (one parallel load instead of a chain of dependent loads per atom); the 64 lanes
+// cover the P^3 stencil points of one atom at a time, so no two lanes touch one point and the sum
+// is free of atomics and bitwise reproducible.
 constexpr int SPREAD_CHUNK = 64;
 
 template <typename real, int P>
@@ -168,50 +189,48 @@ PGM_DEV constexpr int spread_lds_atoms() {
 }
 
 template <typename real, int P, bool WITHQ>
-__global__ __launch_bounds__(256) void k_spread_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
-                                                      const int *__restrict__ tend, const int *__restrict__ sidx,
-                                                      const real *__restrict__ spl, const int *__restrict__ loc,
-                                                      const real4<real> *__restrict__ dq, real *__restrict__ blocks,
-                                                      const int *__restrict__ conv_it, int iter) {
+__global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
+                                                     const int *__restrict__ tend, const int *__restrict__ sidx,
+                                                     const real *__restrict__ spl, const int *__restrict__ loc,
+                                                     const real4<real> *__restrict__ dq, real *__restrict__ blocks,
+                                                     const int *__restrict__ conv_it, int iter) {
     if (conv_it && iter >= *conv_it) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     constexpr int SW = 6 * P + 4;  // staged values per atom: splines, e0 e1 e2 q
     real *sv = reinterpret_cast<real *>(lds_raw);
     int *sl = reinterpret_cast<int *>(sv + SPREAD_CHUNK * SW);
     real *blk = reinterpret_cast<real *>(lds_raw + spread_lds_atoms<real, P>());
-    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int key = blockIdx.x;
     const int BY = t.B[1], BZ = t.B[2];
-    for (int k = tid; k < t.bsize; k += blockDim.x) blk[k] = real(0);
-    const int s0 = tstart[tile], s1 = tend[tile];
+    for (int k = lane; k < t.bsize; k += WAVE) blk[k] = real(0);
+    const int s0 = tstart[key], s1 = tend[key];
+    constexpr int NP = P * P * P;
     for (int c0 = s0; c0 < s1; c0 += SPREAD_CHUNK) {
         const int m = min(SPREAD_CHUNK, s1 - c0);
-        __syncthreads();  // the previous chunk is consumed (and the block is zeroed)
-        for (int k = tid; k < m * 6 * P; k += blockDim.x) {
+        __syncthreads();
+        for (int k = lane; k < m * 6 * P; k += WAVE) {
             const int a = k / (6 * P), v = k - a * 6 * P;
             sv[a * SW + v] = spl[(size_t)(c0 + a) * 6 * P + v];
         }
-        if (tid < m) {
-            const real4<real> d = dq[sidx[c0 + tid]];
+        if (lane < m) {
+            const real4<real> d = dq[sidx[c0 + lane]];
             real e[3];
             scaled_dipole<real>(d, g, e);
-            sv[tid * SW + 6 * P] = e[0];
-            sv[tid * SW + 6 * P + 1] = e[1];
-            sv[tid * SW + 6 * P + 2] = e[2];
-            sv[tid * SW + 6 * P + 3] = WITHQ ? d.w : real(0);
-            sl[tid] = loc[c0 + tid];
+            sv[lane * SW + 6 * P] = e[0];
+            sv[lane * SW + 6 * P + 1] = e[1];
+            sv[lane * SW + 6 * P + 2] = e[2];
+            sv[lane * SW + 6 * P + 3] = WITHQ ? d.w : real(0);
+            sl[lane] = loc[c0 + lane];
         }
         __syncthreads();
         for (int a = 0; a < m; ++a) {
             const real *v = sv + a * SW;
             const int lc = sl[a];
             const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
-            const int ix0 = (wave - lx) & 3;         // first stencil x-plane this wave owns
-            const int np = ix0 + 4 < P ? 2 : (ix0 < P ? 1 : 0);
             const real e0 = v[6 * P], e1 = v[6 * P + 1], e2 = v[6 * P + 2], q = v[6 * P + 3];
-            for (int p = lane; p < np * P * P; p += WAVE) {
-                const int ix = ix0 + 4 * (p / (P * P));
-                const int iy = (p / P) % P, iz = p % P;
+            for (int p = lane; p < NP; p += WAVE) {
+                const int ix = p / (P * P), iy = (p / P) % P, iz = p % P;
                 const real tx = v[ix], dx = v[P + ix];
                 const real ty = v[2 * P + iy], dy = v[3 * P + iy];
                 const real tz = v[4 * P + iz], dz = v[5 * P + iz];
@@ -221,14 +240,15 @@ __global__ __launch_bounds__(256) void k_spread_tiles(PMETiles t, PMEGeom g, con
         }
     }
     __syncthreads();
-    real *out = blocks + (size_t)tile * t.bsize;
-    for (int k = tid; k < t.bsize; k += blockDim.x) out[k] = blk[k];
+    real *out = blocks + (size_t)key * t.bsize;
+    for (int k = lane; k < t.bsize; k += WAVE) out[k] = blk[k];
 }
 
-// Grid point value = sum of the blocks covering it: along each axis its own tile (local l) and,
-// when l <= P-2, the previous tile (local l + width of that tile); fixed order (deterministic).
+// Grid point value = sum of the spread blocks covering it: along each axis its own spread tile
+// (local l) and, when l <= P-2, the previous one (local l + width of that tile); fixed order
+// (deterministic).  Blocks are stored by fine key.
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMEGeom g, const real *__restrict__ blocks,
+__global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMETiles t16, PMEGeom g, const real *__restrict__ blocks,
                                                       real *__restrict__ grid, const int *__restrict__ conv_it, int iter) {
     if (conv_it && iter >= *conv_it) return;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,8 +276,9 @@ __global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMEGeom g, con
     for (int i0 = 0; i0 < nu[0]; ++i0)
         for (int i1 = 0; i1 < nu[1]; ++i1)
             for (int i2 = 0; i2 < nu[2]; ++i2) {
-                const int tile = (u[0][i0] * t.nt[1] + u[1][i1]) * t.nt[2] + u[2][i2];
-                acc += blocks[(size_t)tile * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
+                const int uu[3] = {u[0][i0], u[1][i1], u[2][i2]};
+                const int key = fine_key(uu, t16);
+                acc += blocks[(size_t)key * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
             }
     grid[idx] = acc;
 }
