@@ -63,6 +63,7 @@ typedef struct {
     int precision;      /* 0 double, 1 mixed (MDSettings.precision) */
     int recip;          /* reciprocal space: 0 smooth PME, 1 exact Ewald sum (PeriodicPGM, ewald.py:186-470) */
     double k_tol;       /* Ewald: k-vectors with exp(-k^2/(4 beta^2)) >= k_tol, fixed at the creation box */
+    int nbead;          /* beads (0 or 1: one system): copies of the topology advanced together (PGMBeads) */
 } pgm_settings;
 
 /* Induction state (pgm_jax/md/forcefield.py:679-713 InductionState), caller-owned:
@@ -116,6 +117,16 @@ int pgm_pme_energy(pgm_ctx *ctx, const double *pos, int with_charges, const doub
  * iterations and residual as Result.iterations / Result.residual. */
 int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces,
                 int *iterations, double *residual);
+
+/* Beads (settings.nbead > 1; PGMBeads.compute over the force beads, pgm_jax/md/pimd.py:1104-1172,
+ * with every bead's force field in the same launches instead of vmap / lax.map chunks,
+ * pimd.py:1068-1102): positions, forces, mu and hist are (nbead, N, 3) with N = top->n_atoms; every
+ * bead has its own rows, PME grid and CG (bead-wise FP64 reductions).  pgm_compute then returns the
+ * energies summed over the beads and the largest iteration count and residual; pgm_bead_results
+ * gives each bead's energies, CG iterations and final residual of the last evaluation (host arrays
+ * of nbead, each may be NULL).  Strain, the learned guess and the Ewald sum take one system. */
+int pgm_bead_count(const pgm_ctx *ctx);
+int pgm_bead_results(pgm_ctx *ctx, pgm_energy *energies, int *iterations, double *residuals);
 
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy
  * and forces at fixed induced dipoles mu (N,3) f64 device. */

@@ -16,6 +16,8 @@ from ._lib import PGMError
 __all__ = ["MDSettings", "grid_size", "Molecule", "System", "PGMForceField", "InductionState", "Result", "PGMError"]
 from .md import Langevin, RigidWaterMD  # noqa: E402
 from .periodic import PeriodicModel, PeriodicPGM, pressure_bar  # noqa: E402
+from .beads import PGMBeads, PGMBeadState  # noqa: E402
 from . import models, solver  # noqa: E402
 
-__all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "models", "solver"]
+__all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "PGMBeads", "PGMBeadState",
+            "models", "solver"]

@@ -30,6 +30,8 @@ EXPORTS = (
     "pgm_field",
     "pgm_pme_energy",
     "pgm_compute",
+    "pgm_bead_count",
+    "pgm_bead_results",
     "pgm_energy_forces",
     "pgm_strain_derivative",
     "pgm_solve_linear_induction",
@@ -87,6 +89,7 @@ class Settings(C.Structure):
         ("precision", C.c_int),
         ("recip", C.c_int),
         ("k_tol", C.c_double),
+        ("nbead", C.c_int),
     ]
 
 
@@ -163,6 +166,8 @@ def lib() -> C.CDLL:
     L.pgm_field.argtypes = [P, P, I, P, P]
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
     L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
+    L.pgm_bead_count.argtypes = [P]
+    L.pgm_bead_results.argtypes = [P, P, P, P]
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
     L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]

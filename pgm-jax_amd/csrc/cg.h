@@ -8,6 +8,10 @@
 // by every block of the next kernel (deterministic, no atomics).  Iteration k's kernels run only
 // while k < *conv_it; the update kernel of iteration k sets *conv_it = k + 1 when it converges, so
 // the host can queue iterations ahead of the convergence test.
+//
+// Beads: every vector kernel runs on a (blocks per bead, beads) grid, and each bead has its own
+// CGState, rz and err rows (stride rzs) and partials, so the beads' solves are independent CGs
+// (bead-wise FP64 reductions) advanced by the same launches; a converged bead's blocks return.
 #pragma once
 
 #include "common.h"
@@ -19,6 +23,7 @@ struct CGState {
     int pad;
     double norm;  // mean |alpha b| (convergence normaliser)
 };
+static_assert(sizeof(CGState) == CG_STRIDE * sizeof(int), "CGState is CG_STRIDE ints");
 
 // Sum of np partials (stride `stride`, component c) by the whole block, fixed order; result in all threads.
 PGM_DEV double block_reduce_partials(const double *__restrict__ part, int np, int stride, int c, double *lds, bool is_max) {
@@ -110,11 +115,11 @@ __global__ void k_to_double3(int n, const real4<real> *__restrict__ v, double *_
     out[3 * i + 2] = a.z;
 }
 
-// norm = sum(partials) / (3 N) + 1e-300 (mean |alpha b|)
+// norm = sum(partials) / (3 N) + 1e-300 (mean |alpha b|); one block per bead (np partials each).
 __global__ void k_set_norm(const double *__restrict__ part, int np, double inv3n, CGState *st) {
     __shared__ double lds[32];
-    const double s = block_reduce_partials(part, np, 1, 0, lds, false);
-    if (threadIdx.x == 0) st->norm = s * inv3n + 1e-300;
+    const double s = block_reduce_partials(part + (size_t)blockIdx.x * np, np, 1, 0, lds, false);
+    if (threadIdx.x == 0) st[blockIdx.x].norm = s * inv3n + 1e-300;
 }
 
 __global__ void k_set_norm_value(CGState *st, double norm) {
@@ -124,14 +129,16 @@ __global__ void k_set_norm_value(CGState *st, double norm) {
 // ---------------------------------------------------------------------------- CG
 // Start: z = alpha r, p = z, partials (r . z, max |alpha r|).
 template <typename real>
-__global__ __launch_bounds__(256) void k_cg_start(int n, const real4<real> *__restrict__ r, const real *__restrict__ alpha,
+__global__ __launch_bounds__(256) void k_cg_start(int nper, const real4<real> *__restrict__ r, const real *__restrict__ alpha,
                                                   real4<real> *__restrict__ z, real4<real> *__restrict__ p,
                                                   double *__restrict__ part) {
     __shared__ double lds[32];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
     double s[1] = {0.0};
     double m = 0.0;
-    if (i < n) {
+    if (li < nper) {
         const real a = alpha[i];
         const real4<real> rr = r[i];
         real4<real> zz;
@@ -147,15 +154,20 @@ __global__ __launch_bounds__(256) void k_cg_start(int n, const real4<real> *__re
     block_sum<1>(s, lds);
     m = block_max(m, lds + 16);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s[0];
-        part[2 * blockIdx.x + 1] = m;
+        part[2 * pb] = s[0];
+        part[2 * pb + 1] = m;
     }
 }
 
-// rz[0], err[0], conv_it (one block).
+// rz[0], err[0], conv_it (one block per bead; np partials per bead, rows of rzs).
 __global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGState *st, double *__restrict__ rz,
-                                  double *__restrict__ err, double tol, int max_iter) {
+                                  double *__restrict__ err, int rzs, double tol, int max_iter) {
     __shared__ double lds[32];
+    const int b = blockIdx.x;
+    part += (size_t)b * np * 2;
+    st += b;
+    rz += (size_t)b * rzs;
+    err += (size_t)b * rzs;
     const double s = block_reduce_partials(part, np, 2, 0, lds, false);
     const double m = block_reduce_partials(part, np, 2, 1, lds, true);
     if (threadIdx.x == 0) {
@@ -169,20 +181,23 @@ __global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGSta
 // Iteration k, first half: al = rz_k / max(p . Ap, 1e-300) (rounded to real); x += al p;
 // r' = r - al Ap; z' = alpha r'; partials (z' . (r' - r), r' . z', max |alpha r'|).
 template <typename real>
-__global__ __launch_bounds__(256) void k_cg_update1(int n, const double *__restrict__ pap_part, int np,
-                                                    const double *__restrict__ rz, int k, real4<real> *__restrict__ x,
+__global__ __launch_bounds__(256) void k_cg_update1(int nper, const double *__restrict__ pap_part, int np,
+                                                    const double *__restrict__ rz, int rzs, int k, real4<real> *__restrict__ x,
                                                     real4<real> *__restrict__ r, real4<real> *__restrict__ z,
                                                     const real4<real> *__restrict__ p, const real4<real> *__restrict__ Ap,
                                                     const real *__restrict__ alpha, double *__restrict__ part,
                                                     const CGState *__restrict__ st) {
-    if (k >= st->conv_it) return;
+    const int b = blockIdx.y;
+    if (k >= st[b].conv_it) return;
     __shared__ double lds[32];
-    const double pap = block_reduce_partials(pap_part, np, 1, 0, lds, false);
-    const real al = (real)(rz[k] / fmax(pap, 1e-300));
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double pap = block_reduce_partials(pap_part + (size_t)b * np, np, 1, 0, lds, false);
+    const real al = (real)(rz[(size_t)b * rzs + k] / fmax(pap, 1e-300));
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = b * nper + li;
+    const int pb = b * gridDim.x + blockIdx.x;
     double s[2] = {0.0, 0.0};
     double m = 0.0;
-    if (i < n) {
+    if (li < nper) {
         const real4<real> pp = p[i], ap = Ap[i], r0 = r[i];
         real4<real> xx = x[i];
         xx.x += al * pp.x;
@@ -209,26 +224,32 @@ __global__ __launch_bounds__(256) void k_cg_update1(int n, const double *__restr
     block_sum<2>(s, lds);
     m = block_max(m, lds + 16);
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = s[0];
-        part[3 * blockIdx.x + 1] = s[1];
-        part[3 * blockIdx.x + 2] = m;
+        part[3 * pb] = s[0];
+        part[3 * pb + 1] = s[1];
+        part[3 * pb + 2] = m;
     }
 }
 
 // Iteration k, second half: beta = z'.(r' - r) / max(rz_k, 1e-300) (rounded to real); p = z' + beta p.
-// Block 0 records rz[k+1], err[k+1] and, when converged or out of iterations, conv_it = k + 1.
+// Block 0 of each bead records rz[k+1], err[k+1] and, when converged or out of iterations,
+// conv_it = k + 1.
 template <typename real>
-__global__ __launch_bounds__(256) void k_cg_update2(int n, const double *__restrict__ part, int np, double *__restrict__ rz,
-                                                    double *__restrict__ err, int k, const real4<real> *__restrict__ z,
+__global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__restrict__ part, int np, double *__restrict__ rz,
+                                                    double *__restrict__ err, int rzs, int k, const real4<real> *__restrict__ z,
                                                     real4<real> *__restrict__ p, CGState *st, double tol, int max_iter) {
-    if (k >= st->conv_it) return;
+    const int b = blockIdx.y;
+    if (k >= st[b].conv_it) return;
     __shared__ double lds[32];
+    part += (size_t)b * np * 3;
+    rz += (size_t)b * rzs;
+    err += (size_t)b * rzs;
     const double s0 = block_reduce_partials(part, np, 3, 0, lds, false);
     const double s1 = block_reduce_partials(part, np, 3, 1, lds, false);
     const double mx = block_reduce_partials(part, np, 3, 2, lds, true);
     const real beta = (real)(s0 / fmax(rz[k], 1e-300));
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = b * nper + li;
+    if (li < nper) {
         const real4<real> zz = z[i];
         real4<real> pp = p[i];
         pp.x = zz.x + beta * pp.x;
@@ -239,15 +260,28 @@ __global__ __launch_bounds__(256) void k_cg_update2(int n, const double *__restr
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         rz[k + 1] = s1;
-        const double e = mx / st->norm;
+        const double e = mx / st[b].norm;
         err[k + 1] = e;
-        if (!(e > tol && k + 1 < max_iter)) st->conv_it = k + 1;
+        if (!(e > tol && k + 1 < max_iter)) st[b].conv_it = k + 1;
     }
 }
 
-// stats slot = err[it] (the final residual of a solve, kept on the device).
-__global__ void k_record_residual(const double *__restrict__ err, int it, double *__restrict__ slot) {
-    if (threadIdx.x == 0) slot[0] = err[it];
+// Per bead b: iterations it_b = min(conv_it_b, launched) and the final residual err_b[it_b]
+// (bead_it, bead_res); stats slot = the largest of the residuals (kept on the device).  One block.
+__global__ void k_record_residual(int nbead, const double *__restrict__ err, int rzs, const CGState *__restrict__ st,
+                                  int launched, int *__restrict__ bead_it, double *__restrict__ bead_res,
+                                  double *__restrict__ slot) {
+    __shared__ double lds[32];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nbead; b += blockDim.x) {
+        const int it = min(st[b].conv_it, launched);
+        const double e = err[(size_t)b * rzs + it];
+        bead_it[b] = it;
+        bead_res[b] = e;
+        m = fmax(m, e);
+    }
+    m = block_max(m, lds);
+    if (threadIdx.x == 0) slot[0] = m;
 }
 
 // mu = x + peek alpha r (FP64 out).
@@ -304,14 +338,16 @@ __global__ void k_perm_dipoles(int n, const double *__restrict__ pos, Box box, c
 
 // d = p + mu (FP64) and dq = (d, q) in real; partials (sum |d|^2, sum |mu|^2 / (2 alpha)).
 template <typename real>
-__global__ __launch_bounds__(256) void k_energy_pack(int n, const double *__restrict__ p, const double *__restrict__ mu,
+__global__ __launch_bounds__(256) void k_energy_pack(int nper, const double *__restrict__ p, const double *__restrict__ mu,
                                                      const double *__restrict__ q, const double *__restrict__ alpha,
                                                      double *__restrict__ dd, real4<real> *__restrict__ dq,
                                                      double *__restrict__ part) {
     __shared__ double lds[32];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
     double s[2] = {0.0, 0.0};
-    if (i < n) {
+    if (li < nper) {
         double d[3];
         double m2 = 0.0;
 #pragma unroll
@@ -332,8 +368,8 @@ __global__ __launch_bounds__(256) void k_energy_pack(int n, const double *__rest
     }
     block_sum<2>(s, lds);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s[0];
-        part[2 * blockIdx.x + 1] = s[1];
+        part[2 * pb] = s[0];
+        part[2 * pb + 1] = s[1];
     }
 }
 
@@ -375,12 +411,18 @@ __global__ void k_forces_final(int n, const double *__restrict__ pos, Box box, c
     for (int c = 0; c < 3; ++c) forces[3 * a + c] = -v[c];
 }
 
-// Energy assembly (one block): parts = {pair rows (e_el, e_lj) x npair, (sum|d|^2, u_pol) x npack,
-// u_rec x nconv}; consts = {KE * (u_self_q + u_bg), e_tail, c_self}.
+// Energy assembly (one block per bead, out[4 b .. 4 b + 3] = elec, vdw, total, u_rec): parts =
+// {pair rows (e_el, e_lj) x npair, (sum|d|^2, u_pol) x npack, u_rec x nconv} per bead; consts =
+// {KE * (u_self_q + u_bg), e_tail, c_self}.
 __global__ void k_energy_final(const double *__restrict__ pair_part, int npair, const double *__restrict__ pack_part, int npack,
                                const double *__restrict__ conv_part, int nconv, double ke_const, double tail, double cself,
                                double *__restrict__ out) {
     __shared__ double lds[32];
+    const int b = blockIdx.x;
+    pair_part += (size_t)b * npair * 2;
+    pack_part += (size_t)b * npack * 2;
+    conv_part += (size_t)b * nconv;
+    out += 4 * b;
     const double e_el = block_reduce_partials(pair_part, npair, 2, 0, lds, false);
     const double e_lj = block_reduce_partials(pair_part, npair, 2, 1, lds, false);
     const double d2 = block_reduce_partials(pack_part, npack, 2, 0, lds, false);

@@ -23,6 +23,14 @@ template <> struct vec4<float> { using type = float4; };
 template <> struct vec4<double> { using type = double4; };
 template <typename real> using real4 = typename vec4<real>::type;
 
+// Beads: B independent copies of one system (ring-polymer beads, replicas) stacked along the atom
+// axis, atom i of bead b at b * nper + i.  Kernels that write per-block partial sums run on a
+// (blocks per bead, B) grid so that no block straddles two beads; the CG state is one record of
+// CG_STRIDE ints per bead (conv_it first), and iteration `iter` of a bead runs while
+// iter < conv_it of that bead.
+constexpr int CG_STRIDE = 4;
+PGM_DEV bool cg_done(const int *conv, int bead, int iter) { return conv && iter >= conv[bead * CG_STRIDE]; }
+
 // Reduced lower-triangular box: lattice vectors as rows, h[3a+b].
 struct Box {
     double h[9];

@@ -99,6 +99,9 @@ struct DevBuf {
 
 struct pgm_ctx {
     int n = 0;
+    // beads: nbead copies of one system of nper atoms stacked along the atom axis (n = nbead nper);
+    // each has its own rows, PME grid, CG state and energies, all advanced by the same launches
+    int nbead = 1, nper = 0;
     pgm_settings s{};
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;           // second stream: the pair rows run beside the PME chain
@@ -149,10 +152,12 @@ struct pgm_ctx {
     DevBuf vx, vr, vz, vp, vAp, vb, vx0, vrow, vdq, vtmp;
     DevBuf pperm, dd, g_el, g_lj, rowf, rec_dr, gp;
     DevBuf partA, partB, partC, pair_part, pack_part, conv_part;
-    DevBuf st, rz, err, energy;
-    int *h_pin_int = nullptr;
-    int *h_pin_ring = nullptr;   // conv_it read-backs of the pipelined CG (4 slots)
-    double *h_pin_dbl = nullptr;
+    DevBuf st, rz, err, energy;    // CGState, rz and err rows (stride rzs), energy: 4 per bead then aux slots
+    int rzs = 0;
+    DevBuf bead_it, bead_res;      // iterations and final residual of every bead's last solve
+    int *h_pin_int = nullptr;      // 16 + nbead ints (bead iterations from 16)
+    CGState *h_pin_conv = nullptr; // CG state read-backs of the pipelined CG (4 slots of nbead states)
+    double *h_pin_dbl = nullptr;   // 32 + 5 nbead doubles (bead energies from 32, residuals after them)
     int last_iters = 4;
     bool norm_on_device = false;  // CGState.norm holds the normaliser of the last unfused solve
     DevBuf stats;                 // per-solve final residuals (ring of stats_cap)
@@ -196,6 +201,10 @@ struct pgm_ctx {
     bool md_ready = false;
     int rebuilds = 0;
 };
+
+// Device slots after the per-bead energies: [0] sum m v^2 of the MD step, [1] largest displacement
+// since the row build.
+static double *aux_slots(const pgm_ctx *c) { return c->energy.as<double>() + 4 * c->nbead; }
 
 extern "C" const char *pgm_last_error(void) { return g_err.c_str(); }
 extern "C" int pgm_version(void) { return 1; }
@@ -250,7 +259,7 @@ static void inv3(const double *H, double *Hi) {
 // topology, computed once at creation.
 static double lj_sum_b(const pgm_ctx *c) {
     double M[7] = {0};
-    for (int i = 0; i < c->n; ++i) {
+    for (int i = 0; i < c->nper; ++i) {
         double p = c->h_se[i];
         for (int k = 0; k < 7; ++k) {
             M[k] += p;
@@ -419,20 +428,21 @@ static int setup_tiles(pgm_ctx *c) {
     c->tiles = t16;
     c->tiles8 = t8;
     const int n = c->n;
+    const size_t B = (size_t)c->nbead;
     PGM_TRY(c->pkey.ensure(sizeof(int) * n));
     PGM_TRY(c->pidx.ensure(sizeof(int) * n));
     PGM_TRY(c->pskey.ensure(sizeof(int) * n));
     PGM_TRY(c->psidx.ensure(sizeof(int) * n));
     PGM_TRY(c->loc.ensure(sizeof(int) * n));
     PGM_TRY(c->loc16.ensure(sizeof(int) * n));
-    PGM_TRY(c->tstart.ensure(sizeof(int) * t8.ntiles));
-    PGM_TRY(c->tend.ensure(sizeof(int) * t8.ntiles));
-    PGM_TRY(c->tstart16.ensure(sizeof(int) * t16.ntiles));
-    PGM_TRY(c->tend16.ensure(sizeof(int) * t16.ntiles));
+    PGM_TRY(c->tstart.ensure(sizeof(int) * B * t8.ntiles));
+    PGM_TRY(c->tend.ensure(sizeof(int) * B * t8.ntiles));
+    PGM_TRY(c->tstart16.ensure(sizeof(int) * B * t16.ntiles));
+    PGM_TRY(c->tend16.ensure(sizeof(int) * B * t16.ntiles));
     PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
-    PGM_TRY(c->blocks.ensure(c->rsize * (size_t)t8.ntiles * t8.bsize));
-    // gather partials come per gather tile: the partial buffers must hold ntiles entries
-    for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * (size_t)t16.ntiles)));
+    PGM_TRY(c->blocks.ensure(c->rsize * B * t8.ntiles * t8.bsize));
+    // gather partials come per gather tile: the partial buffers must hold B ntiles entries
+    for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * B * t16.ntiles)));
     // dynamic LDS of the tiled spread and gather beyond the default limit
     const size_t lds_spread = spread_lds_bytes(c), lds_gather = c->rsize * (size_t)t16.bsize;
     PGM_TRY(allow_lds(c, lds_spread, lds_gather));
@@ -445,20 +455,21 @@ template <typename real> static int pme_bin(pgm_ctx *c, const double *pos) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const PMETiles &t8 = c->tiles8, &t16 = c->tiles;
-    k_pme_bin<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t8, t16, c->pkey.as<int>(), c->pidx.as<int>());
+    const int nk8 = c->nbead * t8.ntiles, nk16 = c->nbead * t16.ntiles;
+    k_pme_bin<<<nblk(n, 256), 256, 0, st>>>(n, c->nper, pos, c->pg, t8, t16, c->pkey.as<int>(), c->pidx.as<int>());
     LAUNCH_CHECK();
     int bits = 1;
-    while ((1 << bits) < t8.ntiles) ++bits;
+    while ((1 << bits) < nk8) ++bits;
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
                                                c->psidx.as<int>(), n, 0, bits, st));
     PGM_TRY(c->pme_sort_tmp.ensure(tmp));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->pme_sort_tmp.p, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
                                                c->psidx.as<int>(), n, 0, bits, st));
-    k_fill_int<<<nblk(t8.ntiles, 256), 256, 0, st>>>(t8.ntiles, c->tstart.as<int>(), 0);
-    k_fill_int<<<nblk(t8.ntiles, 256), 256, 0, st>>>(t8.ntiles, c->tend.as<int>(), 0);
-    k_fill_int<<<nblk(t16.ntiles, 256), 256, 0, st>>>(t16.ntiles, c->tstart16.as<int>(), 0);
-    k_fill_int<<<nblk(t16.ntiles, 256), 256, 0, st>>>(t16.ntiles, c->tend16.as<int>(), 0);
+    k_fill_int<<<nblk(nk8, 256), 256, 0, st>>>(nk8, c->tstart.as<int>(), 0);
+    k_fill_int<<<nblk(nk8, 256), 256, 0, st>>>(nk8, c->tend.as<int>(), 0);
+    k_fill_int<<<nblk(nk16, 256), 256, 0, st>>>(nk16, c->tstart16.as<int>(), 0);
+    k_fill_int<<<nblk(nk16, 256), 256, 0, st>>>(nk16, c->tend16.as<int>(), 0);
     k_cell_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->pskey.as<int>(), c->tstart.as<int>(), c->tend.as<int>());
     k_key_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->pskey.as<int>(), 3, c->tstart16.as<int>(), c->tend16.as<int>());
     LAUNCH_CHECK();
@@ -475,10 +486,31 @@ static int make_plans(pgm_ctx *c) {
         FFT_TRY(rocfft_setup());
         setup_done = true;
     }
-    const size_t len[3] = {(size_t)c->s.grid[2], (size_t)c->s.grid[1], (size_t)c->s.grid[0]};
+    // one batched plan per direction over the beads' grids: real grids of K1 K2 K3 points and half
+    // spectra of K1 K2 (K3/2 + 1) complex points, each bead's after the previous one
+    const size_t K1 = c->s.grid[0], K2 = c->s.grid[1], K3 = c->s.grid[2], K3h = K3 / 2 + 1;
+    const size_t len[3] = {K3, K2, K1};
+    const size_t rstr[3] = {1, K3, K3 * K2}, cstr[3] = {1, K3h, K3h * K2};
     const rocfft_precision prec = c->mixed ? rocfft_precision_single : rocfft_precision_double;
-    FFT_TRY(rocfft_plan_create(&c->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3, len, 1, nullptr));
-    FFT_TRY(rocfft_plan_create(&c->bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3, len, 1, nullptr));
+    const size_t nb = (size_t)c->nbead;
+    rocfft_plan_description df = nullptr, db = nullptr;
+    FFT_TRY(rocfft_plan_description_create(&df));
+    FFT_TRY(rocfft_plan_description_create(&db));
+    int rc = PGM_OK;
+    if (rocfft_plan_description_set_data_layout(df, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved, nullptr,
+                                                nullptr, 3, rstr, c->ngrid, 3, cstr, c->ncplx) != rocfft_status_success ||
+        rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, nullptr,
+                                                nullptr, 3, cstr, c->ncplx, 3, rstr, c->ngrid) != rocfft_status_success ||
+        rocfft_plan_create(&c->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3, len, nb, df) !=
+            rocfft_status_success ||
+        rocfft_plan_create(&c->bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3, len, nb, db) !=
+            rocfft_status_success) {
+        g_err = "rocFFT plan creation failed";
+        rc = PGM_ERR_FFT;
+    }
+    rocfft_plan_description_destroy(df);
+    rocfft_plan_description_destroy(db);
+    if (rc < 0) return rc;
     size_t w1 = 0, w2 = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(c->fwd, &w1));
     FFT_TRY(rocfft_plan_get_work_buffer_size(c->bwd, &w2));
@@ -511,10 +543,10 @@ static int alloc_rows(pgm_ctx *c, int cap) {
 static int build_rows(pgm_ctx *c, const double *pos) {
     const int n = c->n;
     hipStream_t st = c->stream;
-    const int ncell = c->nc.x * c->nc.y * c->nc.z;
+    const int ncell = c->nbead * c->nc.x * c->nc.y * c->nc.z;
     PGM_TRY(c->cell_start.ensure(sizeof(int) * ncell));
     PGM_TRY(c->cell_end.ensure(sizeof(int) * ncell));
-    k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->box, c->nc, c->key.as<int>(), c->atom.as<int>());
+    k_cell_keys<<<nblk(n, 256), 256, 0, st>>>(n, c->nper, pos, c->box, c->nc, c->key.as<int>(), c->atom.as<int>());
     LAUNCH_CHECK();
     int bits = 1;
     while ((1 << bits) < ncell) ++bits;
@@ -531,7 +563,7 @@ static int build_rows(pgm_ctx *c, const double *pos) {
     const double rl = c->s.cutoff + c->s.skin;
     for (int attempt = 0; attempt < 3; ++attempt) {
         k_fill_int<<<1, 64, 0, st>>>(1, c->maxcount.as<int>(), 0);
-        k_build_rows<<<nblk(n, 128), 128, 0, st>>>(n, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
+        k_build_rows<<<nblk(n, 128), 128, 0, st>>>(n, c->nper, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
                                                    c->cell_end.as<int>(), rl * rl, c->cap_full, c->rows_full.as<int>(),
                                                    c->count_full.as<int>(), c->maxcount.as<int>());
         LAUNCH_CHECK();
@@ -626,21 +658,22 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
     hipStream_t st = c->stream;
     const int P = c->s.order;
     const int t_sp = energy ? -1 : timing_begin(c, 1, st);
+    const int B = c->nbead;
     if (c->tiled) {
         const PMETiles &t = c->tiles8;
         const size_t lds = spread_lds_bytes(c);
         if (withq) {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, WAVE, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<B * t.ntiles, WAVE, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         } else {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<B * t.ntiles, WAVE, lds, st>>>(
                                 t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
         LAUNCH_CHECK();
-        ORDER_SWITCH(P, (k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
-                            t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
+        ORDER_SWITCH(P, (k_merge_blocks<real, PO><<<nblk((long long)B * c->ngrid, 256), 256, 0, st>>>(
+                            B, t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
     } else {
     k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
     if (withq) {
@@ -665,12 +698,13 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
                                                                                   c->Gc.as<real>(), vir_part);
         LAUNCH_CHECK();
     }
+    const dim3 cgridB(nblk((long long)c->ncplx, 256), B);
     if (energy)
-        k_convolve<real, true><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
-            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), c->conv_part.as<double>(), conv, iter);
+        k_convolve<real, true><<<cgridB, 256, 0, st>>>(c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(),
+                                                       c->conv_part.as<double>(), conv, iter);
     else
-        k_convolve<real, false><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
-            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, conv, iter);
+        k_convolve<real, false><<<cgridB, 256, 0, st>>>(c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(),
+                                                        c->Gc.as<real>(), nullptr, conv, iter);
     LAUNCH_CHECK();
     void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
     FFT_TRY(rocfft_execute(c->bwd, in2, out2, c->info));
@@ -681,11 +715,14 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
 // field, then the gather combined with the row field according to `mode` (pme.h).
 static double c_self(const pgm_ctx *c) { return 4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI); }
 
-// Number of per-block partials a field evaluation writes (MODE_FIELD / MODE_AV).
+// Number of per-block partials per bead a field evaluation writes (MODE_FIELD / MODE_AV).
 static int gather_parts(const pgm_ctx *c) {
     if (c->ewald) return nblk(c->n, EW_TILE);
     return c->tiled ? c->tiles.ntiles : nblk(c->n, 256);
 }
+
+// Grid of a per-atom kernel with per-block partials: (blocks per bead, beads).
+static dim3 bead_grid(const pgm_ctx *c, long long per_bead_threads) { return dim3(nblk(per_bead_threads, 256), c->nbead); }
 
 // PME gather + combine (tiled through LDS when the grid is tiled).
 template <typename real>
@@ -696,7 +733,7 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
     const real cself = (real)c_self(c);
     if (c->tiled) {
         const PMETiles &t = c->tiles;
-        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<t.ntiles, 256, c->rsize * t.bsize, st>>>(
+        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<c->nbead * t.ntiles, 256, c->rsize * t.bsize, st>>>(
                                      t, c->pg, c->tstart16.as<int>(), c->tend16.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                      c->loc16.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
                                      extra, cself, mode, out, part, conv, iter)));
@@ -721,11 +758,11 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     const int t_row = timing_begin(c, 0, c->side);
     if (withq)
         k_row_field<real, true><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
-            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
             c->vrow.as<real4<real>>(), conv, iter);
     else
         k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
-            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
             c->vrow.as<real4<real>>(), conv, iter);
     LAUNCH_CHECK();
     timing_end(c, 0, t_row, c->side);
@@ -989,13 +1026,17 @@ static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
 // not look at the device at all; from there on, after queueing iteration k it waits for the
 // convergence flag written by iteration k-1 (iteration k keeps the GPU busy meanwhile, and turns
 // into no-ops if k-1 converged).  With `quiet` (the MD loop) nothing else is read back: the
-// residual and the normaliser stay on the device (stats slot `slot`).
+// residual and the normaliser stay on the device (stats slot `slot`).  With beads every bead runs
+// its own CG in the same launches; the host waits for the slowest, and iterations / residual are
+// the largest over the beads (per bead: bead_it, bead_res).  Bead normalisers stay on the device.
 template <typename real>
 static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual, bool quiet = false) {
-    const int n = c->n;
+    const int n = c->n, B = c->nbead, nper = c->nper;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
     const int nb = nblk(n, 256);
+    const dim3 gB = bead_grid(c, nper);
+    const int npb = nblk(nper, 256);
     R4 *x = c->vx.as<R4>(), *r = c->vr.as<R4>(), *z = c->vz.as<R4>(), *p = c->vp.as<R4>(), *Ap = c->vAp.as<R4>();
     R4 *b = c->vb.as<R4>(), *x0 = c->vx0.as<R4>(), *dq = c->vdq.as<R4>();
     const double *q = c->q.as<double>();
@@ -1030,56 +1071,62 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         k_predict<real><<<nb, 256, 0, st>>>(n, ind->hist, cf[0], cf[1], cf[2], cf[3], x0);
         LAUNCH_CHECK();
     }
-    const bool fused = have && c->s.fused && (c->s.norm_refresh <= 0 || ind->count % c->s.norm_refresh != 0);
+    const bool fused = have && c->s.fused && (c->s.norm_refresh <= 0 || ind->count % c->s.norm_refresh != 0) &&
+                       (B == 1 || c->norm_on_device);
     if (fused) {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), x0, q, dq);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, dq, true, MODE_FUSED, x0, r, nullptr, -1));
-        if (!quiet || !c->norm_on_device) k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
+        if (B == 1 && (!quiet || !c->norm_on_device)) k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
     } else {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, q, dq);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, dq, true, MODE_FIELD, nullptr, b, c->partA.as<double>(), -1));
-        k_set_norm<<<1, 256, 0, st>>>(c->partA.as<double>(), gather_parts(c), 1.0 / (3.0 * n), S);
+        k_set_norm<<<B, 256, 0, st>>>(c->partA.as<double>(), gather_parts(c), 1.0 / (3.0 * nper), S);
         if (!have) k_scale_alpha<real><<<nb, 256, 0, st>>>(n, b, c->alpha_r.as<real>(), x0);
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, x0, false, MODE_PLAIN, b, r, nullptr, -1));
     }
     c->norm_on_device = true;
     HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
-    k_cg_start<real><<<nb, 256, 0, st>>>(n, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
-    k_cg_start_finish<<<1, 256, 0, st>>>(c->partB.as<double>(), nb, S, rz, err, c->s.dipole_tol, c->s.max_iter);
+    k_cg_start<real><<<gB, 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
+    k_cg_start_finish<<<B, 256, 0, st>>>(c->partB.as<double>(), npb, S, rz, err, c->rzs, c->s.dipole_tol, c->s.max_iter);
     LAUNCH_CHECK();
     const int max_iter = c->s.max_iter;
     const int expect = std::max(1, std::min(c->last_iters, max_iter));
     int launched = 0;
-    int conv = 0x7fffffff;  // last conv_it seen on the host
-    auto read_conv = [&](int k) -> int {  // queue the read-back of conv_it after iteration k-1 (slot k)
-        int *slot = c->h_pin_ring + (k & 3);
-        HIP_TRY(hipMemcpyAsync(slot, &S->conv_it, sizeof(int), hipMemcpyDeviceToHost, st));
+    int conv = 0x7fffffff;  // last conv_it seen on the host (largest over the beads)
+    auto read_conv = [&](int k) -> int {  // queue the read-back of the CG states after iteration k-1 (slot k)
+        HIP_TRY(hipMemcpyAsync(c->h_pin_conv + (size_t)(k & 3) * B, S, sizeof(CGState) * B, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipEventRecord(c->ev_read[k & 3], st));
         return PGM_OK;
+    };
+    auto seen_conv = [&](int k) -> int {  // the slowest bead's conv_it in read slot k (after it completed)
+        int m = 0;
+        for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
+        return m;
     };
     PGM_TRY(read_conv(0));  // converged before any iteration?
     while (launched < max_iter) {
         const int k = launched;
         PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
-        k_cg_update1<real><<<nb, 256, 0, st>>>(n, c->partA.as<double>(), gather_parts(c), rz, k, x, r, z, p, Ap, c->alpha_r.as<real>(),
-                                               c->partB.as<double>(), S);
-        k_cg_update2<real><<<nb, 256, 0, st>>>(n, c->partB.as<double>(), nb, rz, err, k, z, p, S, c->s.dipole_tol, max_iter);
+        k_cg_update1<real><<<gB, 256, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
+                                               c->alpha_r.as<real>(), c->partB.as<double>(), S);
+        k_cg_update2<real><<<gB, 256, 0, st>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, c->s.dipole_tol,
+                                               max_iter);
         LAUNCH_CHECK();
         launched = k + 1;
         PGM_TRY(read_conv(launched));
         if (launched >= expect - 1 || launched == max_iter) {
             // wait for the state after iteration k-1 (read slot k) while iteration k runs
             HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
-            conv = c->h_pin_ring[k & 3];
-            if (conv <= k) break;  // converged before iteration k: k was a no-op
+            conv = seen_conv(k);
+            if (conv <= k) break;  // every bead converged before iteration k: k was a no-op
         }
     }
     if (conv > launched) {  // not yet seen converged: the state after the last launched iteration decides
         HIP_TRY(hipEventSynchronize(c->ev_read[launched & 3]));
-        conv = c->h_pin_ring[launched & 3];
+        conv = seen_conv(launched);
     }
     const int it = std::min(conv, launched);
     c->last_iters = std::max(1, it);
@@ -1092,13 +1139,14 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         G.nring = std::min(G.nring + 1, GUESS_K);
     }
     k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
-    k_record_residual<<<1, 64, 0, st>>>(err, it, c->stats.as<double>() + (c->stats_slot % c->stats_cap));
+    double *slot = c->stats.as<double>() + (c->stats_slot % c->stats_cap);
+    k_record_residual<<<1, 256, 0, st>>>(B, err, c->rzs, S, launched, c->bead_it.as<int>(), c->bead_res.as<double>(), slot);
     LAUNCH_CHECK();
     c->stats_slot += 1;
     ind->count += 1;
     if (iterations) *iterations = it;
     if (!quiet) {
-        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, err + it, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, slot, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &S->norm, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (residual) *residual = c->h_pin_dbl[0];
@@ -1109,13 +1157,18 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
 }
 
 // ---------------------------------------------------------------------------- energy and forces
+// Energies of the last evaluation, summed over the beads (per bead in h_pin_dbl[32 + 4 b ..]).
 static int read_energy(pgm_ctx *c, pgm_energy *energy) {
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double *h = c->h_pin_dbl + 32;
+    HIP_TRY(hipMemcpyAsync(h, c->energy.p, 4 * sizeof(double) * c->nbead, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (energy) {
-        energy->elec = c->h_pin_dbl[0];
-        energy->vdw = c->h_pin_dbl[1];
-        energy->total = c->h_pin_dbl[2];
+        energy->elec = energy->vdw = energy->total = 0.0;
+        for (int b = 0; b < c->nbead; ++b) {
+            energy->elec += h[4 * b];
+            energy->vdw += h[4 * b + 1];
+            energy->total += h[4 * b + 2];
+        }
     }
     return PGM_OK;
 }
@@ -1123,15 +1176,18 @@ static int read_energy(pgm_ctx *c, pgm_energy *energy) {
 template <typename real>
 static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, double *forces, bool quiet = false,
                               const double *masses = nullptr, double *strain = nullptr, bool molecular = true) {
-    const int n = c->n;
+    const int n = c->n, B = c->nbead, nper = c->nper;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
     const int nb = nblk(n, 256);
+    const int npb = nblk(nper, 256);  // pack partials per bead
     R4 *dq = c->vdq.as<R4>();
-    k_energy_pack<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), mu, c->q.as<double>(), c->alpha64.as<double>(),
-                                            c->dd.as<double>(), dq, c->pack_part.as<double>());
+    k_energy_pack<real><<<bead_grid(c, nper), 256, 0, st>>>(nper, c->pperm.as<double>(), mu, c->q.as<double>(),
+                                                            c->alpha64.as<double>(), c->dd.as<double>(), dq,
+                                                            c->pack_part.as<double>());
     LAUNCH_CHECK();
-    const int npair = nblk((long long)n * ROWL, 256);
+    const int npair = nblk((long long)nper * ROWL, 256);  // pair partials per bead
+    const dim3 gpair(npair, B);
     const int nconv = c->ewald ? nblk(c->nk, EW_TILE) : nblk((long long)c->ncplx, 256);
     const bool vir = strain != nullptr;
     if (vir) {
@@ -1152,14 +1208,14 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
             k_fill<double><<<nblk(3LL * n, 256), 256, 0, st>>>(3 * (size_t)n, c->off.as<double>(), 0.0);
         }
         LAUNCH_CHECK();
-        k_pair_forces<real, true><<<npair, 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+        k_pair_forces<real, true><<<gpair, 256, 0, st>>>(
+            nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
             (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), c->off.as<double>(),
             c->vir_pair.as<double>());
     } else {
-        k_pair_forces<real, false><<<npair, 256, 0, st>>>(
-            n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+        k_pair_forces<real, false><<<gpair, 256, 0, st>>>(
+            nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
             (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
     }
@@ -1179,7 +1235,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     } else {
         PGM_TRY(spread_fft<real>(c, dq, true, true, nullptr, -1, vir ? c->vir_conv.as<double>() : nullptr));
         ORDER_SWITCH(c->s.order, (k_gather_forces<real, PO><<<nb, 256, 0, st>>>(
-                                     n, c->pos, c->pg, c->grid.as<real>(), dq, c->dd.as<double>(), c->rowf.as<double>(),
+                                     n, nper, c->pos, c->pg, c->grid.as<real>(), dq, c->dd.as<double>(), c->rowf.as<double>(),
                                      c_self(c), c->rec_dr.as<double>(), c->gp.as<double>(),
                                      vir ? c->rec_dd.as<double>() : nullptr)));
     }
@@ -1212,7 +1268,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     const double b0 = c->s.ewald_beta;
     const double ke_const = KE * (-(b0 / SQRT_PI) * c->sum_q2 - M_PI * c->sum_q * c->sum_q / (2.0 * c->volume * b0 * b0));
     const double tail = c->s.lj_lrc ? lj_tail(c) : 0.0;
-    k_energy_final<<<1, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), nb, c->conv_part.as<double>(),
+    k_energy_final<<<B, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), npb, c->conv_part.as<double>(),
                                       nconv, ke_const, tail, c_self(c), c->energy.as<double>());
     LAUNCH_CHECK();
     if (quiet) return PGM_OK;  // energies stay in c->energy
@@ -1244,8 +1300,19 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         g_err = "pgm_ctx_create: Ewald needs 0 < k_tol < 1";
         return PGM_ERR_ARG;
     }
+    const int nbead = s->nbead > 1 ? s->nbead : 1;
+    if (nbead > 1 && (s->recip != 0 || s->predictor == 4)) {
+        g_err = "pgm_ctx_create: beads run with smooth PME and a polynomial or previous-step predictor";
+        return PGM_ERR_ARG;
+    }
+    if ((long long)nbead * top->n_atoms > (1LL << 26)) {
+        g_err = "pgm_ctx_create: more than 2^26 atoms over all beads";
+        return PGM_ERR_ARG;
+    }
     pgm_ctx *c = new pgm_ctx();
-    c->n = top->n_atoms;
+    c->nbead = nbead;
+    c->nper = top->n_atoms;
+    c->n = nbead * top->n_atoms;
     c->s = *s;
     c->ewald = s->recip == 1;
     if (c->ewald) {  // half-space index set with 0 < |k| < kcut at the creation box (pgm_jax/ewald.py:129-158)
@@ -1279,15 +1346,21 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     c->stream = (hipStream_t)stream;
     c->mixed = s->precision == 1;
     c->rsize = c->mixed ? sizeof(float) : sizeof(double);
-    const int n = c->n;
-    c->h_q.assign(top->q, top->q + n);
-    c->h_alpha.assign(top->alpha, top->alpha + n);
-    c->h_rh.assign(top->lj_rmin_half, top->lj_rmin_half + n);
-    c->h_se.assign(top->lj_sqrt_eps, top->lj_sqrt_eps + n);
-    c->h_mol.assign(top->mol, top->mol + n);
+    const int n = c->n, nper = c->nper;
+    // per-atom parameters of one bead, repeated for every bead (molecule indices shifted by the
+    // bead's molecule count)
+    int nmol_bead = 0;
+    for (int i = 0; i < nper; ++i) nmol_bead = std::max(nmol_bead, top->mol[i] + 1);
+    for (int b = 0; b < nbead; ++b) {
+        c->h_q.insert(c->h_q.end(), top->q, top->q + nper);
+        c->h_alpha.insert(c->h_alpha.end(), top->alpha, top->alpha + nper);
+        c->h_rh.insert(c->h_rh.end(), top->lj_rmin_half, top->lj_rmin_half + nper);
+        c->h_se.insert(c->h_se.end(), top->lj_sqrt_eps, top->lj_sqrt_eps + nper);
+        for (int i = 0; i < nper; ++i) c->h_mol.push_back(top->mol[i] + b * nmol_bead);
+    }
     c->h_R2.resize(n);
-    for (int i = 0; i < n; ++i) {
-        c->h_R2[i] = top->radius[i] * top->radius[i];
+    for (int i = 0; i < n; ++i) c->h_R2[i] = top->radius[i % nper] * top->radius[i % nper];
+    for (int i = 0; i < nper; ++i) {  // per bead
         c->sum_q += c->h_q[i];
         c->sum_q2 += c->h_q[i] * c->h_q[i];
         if (!(c->h_alpha[i] > 0)) {
@@ -1312,7 +1385,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     {  // molecules as CSR over atom indices (for centres of mass)
         int mmax = -1;
         for (int i = 0; i < n; ++i) {
-            if (c->h_mol[i] < 0 || c->h_mol[i] >= n) {
+            if (top->mol[i % nper] < 0 || top->mol[i % nper] >= nper) {
                 g_err = "pgm_ctx_create: molecule index out of range [0, n_atoms)";
                 return fail(PGM_ERR_ARG);
             }
@@ -1333,13 +1406,14 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     } else {
         if ((r = upload(c->alpha_r, c->h_alpha.data(), n, st)) < 0) return fail(r);
     }
-    // covalent dipoles: CSR by owner atom (cstart over m sorted by cov_i) and by target (jstart/jlist)
-    const int nc = top->n_cov;
+    // covalent dipoles: CSR by owner atom (cstart over m sorted by cov_i) and by target (jstart/jlist);
+    // one bead's list, repeated with atom offsets for every bead
+    const int nc1 = top->n_cov, nc = nbead * top->n_cov;
     c->ncov = nc;
-    std::vector<int> order(nc), ci(nc), cj(nc), cs(n + 1, 0), js(n + 1, 0), jl(nc);
+    std::vector<int> order(nc1), ci(nc), cj(nc), cs(n + 1, 0), js(n + 1, 0), jl(nc);
     std::vector<double> cc(nc);
-    for (int m = 0; m < nc; ++m) {
-        if (top->cov_i[m] < 0 || top->cov_i[m] >= n || top->cov_j[m] < 0 || top->cov_j[m] >= n) {
+    for (int m = 0; m < nc1; ++m) {
+        if (top->cov_i[m] < 0 || top->cov_i[m] >= nper || top->cov_j[m] < 0 || top->cov_j[m] >= nper) {
             g_err = "pgm_ctx_create: covalent dipole atom out of range";
             return fail(PGM_ERR_ARG);
         }
@@ -1347,9 +1421,9 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return top->cov_i[a] < top->cov_i[b]; });
     for (int k = 0; k < nc; ++k) {
-        const int m = order[k];
-        ci[k] = top->cov_i[m];
-        cj[k] = top->cov_j[m];
+        const int m = order[k % nc1], off = (k / nc1) * nper;
+        ci[k] = top->cov_i[m] + off;
+        cj[k] = top->cov_j[m] + off;
         cc[k] = top->cov_c[m];
         cs[ci[k] + 1]++;
         js[cj[k] + 1]++;
@@ -1375,14 +1449,17 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     for (DevBuf *b : v3)
         if ((r = b->ensure(d3)) < 0) return fail(r);
     if ((r = c->xr.ensure(sizeof(double4) * n)) < 0) return fail(r);
-    const int nbmax = nblk((long long)n * ROWL, 64) + 16;
+    const long long nbmax = (long long)nbead * (nblk((long long)nper * ROWL, 64) + 16);
     DevBuf *parts[] = {&c->partA, &c->partB, &c->partC, &c->pair_part, &c->pack_part};
     for (DevBuf *b : parts)
         if ((r = b->ensure(sizeof(double) * 4 * nbmax)) < 0) return fail(r);
-    if ((r = c->st.ensure(sizeof(CGState))) < 0) return fail(r);
-    if ((r = c->rz.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
-    if ((r = c->err.ensure(sizeof(double) * (s->max_iter + 2))) < 0) return fail(r);
-    if ((r = c->energy.ensure(sizeof(double) * 4)) < 0) return fail(r);
+    c->rzs = s->max_iter + 2;
+    if ((r = c->st.ensure(sizeof(CGState) * nbead)) < 0) return fail(r);
+    if ((r = c->rz.ensure(sizeof(double) * c->rzs * nbead)) < 0) return fail(r);
+    if ((r = c->err.ensure(sizeof(double) * c->rzs * nbead)) < 0) return fail(r);
+    if ((r = c->energy.ensure(sizeof(double) * (4 * nbead + 4))) < 0) return fail(r);
+    if ((r = c->bead_it.ensure(sizeof(int) * nbead)) < 0) return fail(r);
+    if ((r = c->bead_res.ensure(sizeof(double) * nbead)) < 0) return fail(r);
     if ((r = c->stats.ensure(sizeof(double) * c->stats_cap)) < 0) return fail(r);
     if ((r = c->maxcount.ensure(sizeof(int))) < 0) return fail(r);
     DevBuf *ib[] = {&c->key, &c->atom, &c->skey, &c->satom, &c->count_full, &c->count, &c->count_near};
@@ -1399,25 +1476,30 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
             g_err = "event creation failed";
             return fail(PGM_ERR_HIP);
         }
-    if (hipHostMalloc((void **)&c->h_pin_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_pin_dbl, 32 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_pin_ring, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->h_pin_int, (16 + nbead) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pin_dbl, (32 + 5 * (size_t)nbead) * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pin_conv, 4 * (size_t)nbead * sizeof(CGState), hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         return fail(PGM_ERR_HIP);
     }
     c->ngrid = (size_t)s->grid[0] * s->grid[1] * s->grid[2];
     c->ncplx = (size_t)s->grid[0] * s->grid[1] * (s->grid[2] / 2 + 1);
-    if ((r = c->grid.ensure(c->rsize * c->ngrid)) < 0) return fail(r);
-    if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx)) < 0) return fail(r);
-    if ((r = c->conv_part.ensure(sizeof(double) * (std::max(nblk((long long)c->ncplx, 256), nblk(c->nk, EW_TILE)) + 16))) < 0)
+    if ((r = c->grid.ensure(c->rsize * c->ngrid * nbead)) < 0) return fail(r);
+    if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx * nbead)) < 0) return fail(r);
+    if ((r = c->conv_part.ensure(sizeof(double) * nbead *
+                                 (std::max(nblk((long long)c->ncplx, 256), nblk(c->nk, EW_TILE)) + 16))) < 0)
         return fail(r);
     if (c->ewald && (r = c->sk.ensure(sizeof(double2) * std::max(c->nk, 1))) < 0) return fail(r);
     if ((r = setup_tiles(c)) < 0) return fail(r);
+    if (nbead > 1 && !c->tiled) {
+        g_err = "pgm_ctx_create: beads need a tiled PME grid (per axis at least three tiles of P-1..8 points)";
+        return fail(PGM_ERR_ARG);
+    }
     if ((r = set_box(c, H)) < 0) return fail(r);
     // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
     const double rl = s->cutoff + s->skin;
-    const int cap0 = (int)(1.25 * 4.0 / 3.0 * M_PI * rl * rl * rl * n / c->volume) + 32;
-    if ((r = alloc_rows(c, std::min(cap0, n))) < 0) return fail(r);
+    const int cap0 = (int)(1.25 * 4.0 / 3.0 * M_PI * rl * rl * rl * c->nper / c->volume) + 32;
+    if ((r = alloc_rows(c, std::min(cap0, c->nper))) < 0) return fail(r);
     if ((r = make_plans(c)) < 0) return fail(r);
     *out = c;
     return PGM_OK;
@@ -1444,7 +1526,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
-                     &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy,
+                     &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy, &c->bead_it, &c->bead_res,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
                      &c->loc16, &c->blocks,
@@ -1456,7 +1538,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->guess.scal, &c->guess.ring};
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
-    if (c->h_pin_ring) (void)hipHostFree(c->h_pin_ring);
+    if (c->h_pin_conv) (void)hipHostFree(c->h_pin_conv);
     if (c->h_pin_dbl) (void)hipHostFree(c->h_pin_dbl);
     delete c;
     return PGM_OK;
@@ -1541,8 +1623,15 @@ static int pme_api(pgm_ctx *c, const double *pos, int withq, const double *d, do
     return PGM_OK;
 }
 
+static int single_system(const pgm_ctx *c, const char *what) {
+    if (c->nbead == 1) return PGM_OK;
+    g_err = std::string(what) + ": one system per context (this context holds beads)";
+    return PGM_ERR_STATE;
+}
+
 extern "C" int pgm_pme_energy(pgm_ctx *c, const double *pos, int with_charges, const double *d, double *e_rec, double *grad_d) {
     if (!c || !pos || !d) return PGM_ERR_ARG;
+    PGM_TRY(single_system(c, "pgm_pme_energy"));
     return c->mixed ? pme_api<float>(c, pos, with_charges, d, e_rec, grad_d) : pme_api<double>(c, pos, with_charges, d, e_rec, grad_d);
 }
 
@@ -1566,6 +1655,29 @@ extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pg
                     : compute_api<double>(c, pos, ind, energy, forces, iterations, residual);
 }
 
+extern "C" int pgm_bead_count(const pgm_ctx *c) { return c ? c->nbead : PGM_ERR_ARG; }
+
+extern "C" int pgm_bead_results(pgm_ctx *c, pgm_energy *energies, int *iterations, double *residuals) {
+    if (!c) return PGM_ERR_ARG;
+    const int B = c->nbead;
+    hipStream_t st = c->stream;
+    double *h = c->h_pin_dbl + 32;
+    HIP_TRY(hipMemcpyAsync(h, c->energy.p, 4 * sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(h + 4 * B, c->bead_res.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_int + 16, c->bead_it.p, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int b = 0; b < B; ++b) {
+        if (energies) {
+            energies[b].elec = h[4 * b];
+            energies[b].vdw = h[4 * b + 1];
+            energies[b].total = h[4 * b + 2];
+        }
+        if (iterations) iterations[b] = c->h_pin_int[16 + b];
+        if (residuals) residuals[b] = h[4 * B + b];
+    }
+    return PGM_OK;
+}
+
 template <typename real>
 static int ef_api(pgm_ctx *c, const double *pos, const double *mu, pgm_energy *energy, double *forces,
                   const double *masses = nullptr, double *strain = nullptr, bool molecular = true) {
@@ -1587,6 +1699,7 @@ extern "C" int pgm_strain_derivative(pgm_ctx *c, const double *pos, const double
         g_err = "pgm_strain_derivative: null argument";
         return PGM_ERR_ARG;
     }
+    PGM_TRY(single_system(c, "pgm_strain_derivative"));
     return c->mixed ? ef_api<float>(c, pos, mu, energy, forces, masses, strain, molecular != 0)
                     : ef_api<double>(c, pos, mu, energy, forces, masses, strain, molecular != 0);
 }
@@ -1598,6 +1711,7 @@ extern "C" int pgm_guess_config(pgm_ctx *c, int hidden_layers, double dipole_sca
         g_err = "pgm_guess_config: 1..4 hidden layers, dipole_scale > 0, harvest_every >= 1, capacity >= n_atoms";
         return PGM_ERR_ARG;
     }
+    PGM_TRY(single_system(c, "pgm_guess_config"));
     if (c->n % 3 != 0) {
         g_err = "pgm_guess_config: the learned guess needs consecutive 3-site molecules";
         return PGM_ERR_ARG;
@@ -1894,9 +2008,9 @@ extern "C" int pgm_md_setup(pgm_ctx *c, const double *masses, const double *pos)
 
 static int md_kinetic(pgm_ctx *c, double *ke) {
     const int nb = nblk(c->nmol, 256);
-    k_reduce_partials<<<1, 256, 0, c->stream>>>(c->md_part.as<double>(), nb, 0, c->energy.as<double>());
+    k_reduce_partials<<<1, 256, 0, c->stream>>>(c->md_part.as<double>(), nb, 0, aux_slots(c));
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 8, c->energy.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 8, aux_slots(c), sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     *ke = 0.5 * c->h_pin_dbl[8];
     return PGM_OK;
@@ -1972,9 +2086,9 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         PGM_TRY(md_first(c, s, dt));
     forces:
         k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
-        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, c->energy.as<double>() + 3);
+        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, aux_slots(c) + 1);
         LAUNCH_CHECK();
-        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, c->energy.as<double>() + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, aux_slots(c) + 1, sizeof(double), hipMemcpyDeviceToHost, st));
         disp_pending = true;
         {
             const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true)
@@ -2015,7 +2129,7 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         rep->vdw = e.vdw;
         rep->ekin = ke;
         rep->etot = e.total + ke;
-        const double dof = 3.0 * n - 3.0 * c->nmol - 3.0;
+        const double dof = 3.0 * n - 3.0 * c->nmol - 3.0 * c->nbead;  // rigid molecules, each bead's centre of mass
         rep->temperature = 2.0 * ke / (dof * 0.0083144626181532);
         rep->cg_iterations_total = it_total;
         rep->cg_iterations_max = it_max;
@@ -2074,14 +2188,14 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
         const PMETiles &t = c->tiles8;
         const size_t lds = spread_lds_bytes(c);
         ORDER_SWITCH(c->s.order, rc = timed(1, [&] {
-                                     k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+                                     k_spread_tiles<real, PO, false><<<c->nbead * t.ntiles, WAVE, lds, st>>>(
                                          t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                          c->loc.as<int>(), dq, c->blocks.as<real>(), nullptr, -1);
                                  }));
         if (rc < 0) return rc;
         ORDER_SWITCH(c->s.order, rc = timed(0, [&] {
-                                     k_merge_blocks<real, PO><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(
-                                         t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
+                                     k_merge_blocks<real, PO><<<nblk((long long)c->nbead * c->ngrid, 256), 256, 0, st>>>(
+                                         c->nbead, t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
                                  }));
         if (rc < 0) return rc;
     } else {
@@ -2097,7 +2211,7 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     rc = timed(2, [&] { rocfft_execute(c->fwd, in, out, c->info); });
     if (rc < 0) return rc;
     rc = timed(3, [&] {
-        k_convolve<real, false><<<nblk((long long)c->ncplx, 256), 256, 0, st>>>(
+        k_convolve<real, false><<<dim3(nblk((long long)c->ncplx, 256), c->nbead), 256, 0, st>>>(
             c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, nullptr, -1);
     });
     if (rc < 0) return rc;
@@ -2106,8 +2220,8 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     if (rc < 0) return rc;
     rc = timed(5, [&] {
         k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
-            n, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
-            c->vrow.as<R4>(), nullptr, -1);
+            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq,
+            c->box, c->vrow.as<R4>(), nullptr, -1);
     });
     if (rc < 0) return rc;
     rc = timed(6, [&] { (void)gather_combine<real>(c, dq, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1); });

@@ -19,8 +19,9 @@ constexpr int ROWL = 8;  // lanes per atom in the row kernels
 
 // Packed per-atom geometry xr: x, y, z [nm] and R^2 (Gaussian radius squared) [nm^2].
 // ---------------------------------------------------------------------------- cells
-// Cell of every atom in fractional coordinates of the box; key = (c0 * n1 + c1) * n2 + c2.
-__global__ void k_cell_keys(int n, const double *__restrict__ pos, Box box, int3 nc, int *__restrict__ key,
+// Cell of every atom in fractional coordinates of the box; key = bead * ncell + (c0 * n1 + c1) * n2 + c2
+// (nper atoms per bead).
+__global__ void k_cell_keys(int n, int nper, const double *__restrict__ pos, Box box, int3 nc, int *__restrict__ key,
                             int *__restrict__ atom) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -35,7 +36,7 @@ __global__ void k_cell_keys(int n, const double *__restrict__ pos, Box box, int3
         int k = (int)(f * ncs[a]);
         c[a] = k < 0 ? 0 : (k >= ncs[a] ? ncs[a] - 1 : k);
     }
-    key[i] = (c[0] * nc.y + c[1]) * nc.z + c[2];
+    key[i] = (i / nper) * (nc.x * nc.y * nc.z) + (c[0] * nc.y + c[1]) * nc.z + c[2];
     atom[i] = i;
 }
 
@@ -54,9 +55,9 @@ __global__ void k_cell_bounds(int n, const int *__restrict__ skey, int *__restri
 }
 
 // Full rows within rc_list (cutoff + skin) by a search of the neighbouring cells (27, or every
-// distinct cell along an axis with fewer than 3 cells).  One thread per atom; the partners come in
-// cell order and, within a cell, in atom order (deterministic).
-__global__ void k_build_rows(int n, const double *__restrict__ pos, Box box, int3 nc, const int *__restrict__ satom,
+// distinct cell along an axis with fewer than 3 cells) of the atom's own bead.  One thread per atom;
+// the partners come in cell order and, within a cell, in atom order (deterministic).
+__global__ void k_build_rows(int n, int nper, const double *__restrict__ pos, Box box, int3 nc, const int *__restrict__ satom,
                              const int *__restrict__ cstart, const int *__restrict__ cend, double rc2, int cap,
                              int *__restrict__ rows, int *__restrict__ count, int *__restrict__ maxcount) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,13 +78,14 @@ __global__ void k_build_rows(int n, const double *__restrict__ pos, Box box, int
     }
     int cnt = 0;
     int *row = rows + (size_t)i * cap;
+    const int cell0 = (i / nper) * (nc.x * nc.y * nc.z);
     for (int o0 = lo[0]; o0 <= hiO[0]; ++o0) {
         const int c0 = (c[0] + o0 + ncs[0]) % ncs[0];
         for (int o1 = lo[1]; o1 <= hiO[1]; ++o1) {
             const int c1 = (c[1] + o1 + ncs[1]) % ncs[1];
             for (int o2 = lo[2]; o2 <= hiO[2]; ++o2) {
                 const int c2 = (c[2] + o2 + ncs[2]) % ncs[2];
-                const int cell = (c0 * nc.y + c1) * nc.z + c2;
+                const int cell = cell0 + (c0 * nc.y + c1) * nc.z + c2;
                 const int s0 = cstart[cell], s1 = cend[cell];
                 for (int s = s0; s < s1; ++s) {
                     const int j = satom[s];
@@ -210,14 +212,13 @@ PGM_DEV void row_field_entry(const double4 &a, const double4 &b, const real4<rea
 // indices and G1, G2 precomputed by k_pair_tensors (ELL layout, RL lanes per atom).  dq holds
 // (d_x, d_y, d_z, q) per atom; WITHQ adds the charge term.
 template <typename real, bool WITHQ>
-__global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict__ jell, const int *__restrict__ count,
+__global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *__restrict__ jell, const int *__restrict__ count,
                                                    const real *__restrict__ G1, const real *__restrict__ G2,
                                                    const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
                                                    real4<real> *__restrict__ out, const int *__restrict__ conv_it, int iter) {
-    if (conv_it && iter >= *conv_it) return;
     const int lane = threadIdx.x % RL;
     const int i = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
-    if (i >= n) return;
+    if (i >= n || cg_done(conv_it, i / nper, iter)) return;
     const double4 a = xr[i];
     const int c = count[i];
     real ex = 0, ey = 0, ez = 0;
@@ -323,8 +324,9 @@ __global__ __launch_bounds__(256) void k_row_field(int n, const int *__restrict_
                 _Pragma("unroll") for (int q_ = 0; q_ < 3; ++q_) w[3 * p_ + q_] += v[p_] * X[q_]; \
             }
 
+// Grid (blocks per bead, beads): partials per block of one bead, at blockIdx.y * gridDim.x + blockIdx.x.
 template <typename real, bool VIRIAL>
-__global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restrict__ rows, const int *__restrict__ count,
+__global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__restrict__ rows, const int *__restrict__ count,
                                                      const int *__restrict__ count_near, int cap,
                                                      const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
                                                      real beta, const double *__restrict__ rh, const double *__restrict__ se,
@@ -334,10 +336,12 @@ __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restric
                                                      double *__restrict__ wpart) {
     __shared__ double lds[9 * 16];
     const int lane = threadIdx.x % ROWL;
-    const int i = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    const int li = blockIdx.x * (blockDim.x / ROWL) + threadIdx.x / ROWL;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
     double esum[2] = {0.0, 0.0};
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < n) {
+    if (li < nper) {
         const double4 a = xr[i];
         const real4<real> di = dq[i];
         const real qi = di.w;
@@ -386,13 +390,13 @@ __global__ __launch_bounds__(256) void k_pair_forces(int n, const int *__restric
     }
     block_sum<2>(esum, lds);
     if (threadIdx.x == 0) {
-        epart[2 * blockIdx.x] = esum[0];
-        epart[2 * blockIdx.x + 1] = esum[1];
+        epart[2 * pb] = esum[0];
+        epart[2 * pb + 1] = esum[1];
     }
     if (VIRIAL) {
         block_sum<9>(w, lds);
         if (threadIdx.x == 0)
-            for (int k = 0; k < 9; ++k) wpart[9 * blockIdx.x + k] = w[k];
+            for (int k = 0; k < 9; ++k) wpart[9 * pb + k] = w[k];
     }
 }
 

@@ -115,9 +115,9 @@ PGM_DEV int fine_key(const int (&u8)[3], const PMETiles &t16) {
     return (((u8[0] >> 1) * t16.nt[1] + (u8[1] >> 1)) * t16.nt[2] + (u8[2] >> 1)) * 8 + oct;
 }
 
-// key = fine key of each atom's stencil base; idx = atom index.
-__global__ void k_pme_bin(int n, const double *__restrict__ pos, PMEGeom g, PMETiles t8, PMETiles t16, int *__restrict__ key,
-                          int *__restrict__ idx) {
+// key = bead * t8.ntiles + fine key of each atom's stencil base (nper atoms per bead); idx = atom index.
+__global__ void k_pme_bin(int n, int nper, const double *__restrict__ pos, PMEGeom g, PMETiles t8, PMETiles t16,
+                          int *__restrict__ key, int *__restrict__ idx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int base[3];
@@ -130,7 +130,7 @@ __global__ void k_pme_bin(int n, const double *__restrict__ pos, PMEGeom g, PMET
         const int u = bb / t8.T[a];
         u8[a] = u < t8.nt[a] ? u : t8.nt[a] - 1;
     }
-    key[i] = fine_key(u8, t16);
+    key[i] = (i / nper) * t8.ntiles + fine_key(u8, t16);
     idx[i] = i;
 }
 
@@ -175,8 +175,9 @@ __global__ void k_key_bounds(int n, const int *__restrict__ skey, int shift, int
     if (s == n - 1 || (skey[s + 1] >> shift) != k) end[k] = s + 1;
 }
 
-// Tiled spread: one wave per spread tile (fine key) accumulates the stencils of its atoms into an
-// LDS block of (T8 + P - 1)^3 points, written to `blocks` (merged by k_merge_blocks).  The atoms'
+// Tiled spread: one wave per spread tile (fine key; block b * t.ntiles + key for bead b) accumulates
+// the stencils of its atoms into an LDS block of (T8 + P - 1)^3 points, written to `blocks` (merged
+// by k_merge_blocks).  The atoms'
 // spline weights, scaled dipoles, charges and local offsets are staged through LDS in chunks of
 // SPREAD_CHUNK (one parallel load instead of a chain of dependent loads per atom); the 64 lanes
 // cover the P^3 stencil points of one atom at a time, so no two lanes touch one point and the sum
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, cons
                                                      const real *__restrict__ spl, const int *__restrict__ loc,
                                                      const real4<real> *__restrict__ dq, real *__restrict__ blocks,
                                                      const int *__restrict__ conv_it, int iter) {
-    if (conv_it && iter >= *conv_it) return;
+    if (cg_done(conv_it, blockIdx.x / t.ntiles, iter)) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     constexpr int SW = 6 * P + 4;  // staged values per atom: splines, e0 e1 e2 q
     real *sv = reinterpret_cast<real *>(lds_raw);
@@ -246,14 +247,19 @@ __global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, cons
 
 // Grid point value = sum of the spread blocks covering it: along each axis its own spread tile
 // (local l) and, when l <= P-2, the previous one (local l + width of that tile); fixed order
-// (deterministic).  Blocks are stored by fine key.
+// (deterministic).  Blocks are stored by fine key, bead after bead; grid holds nbead grids.
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMETiles t16, PMEGeom g, const real *__restrict__ blocks,
-                                                      real *__restrict__ grid, const int *__restrict__ conv_it, int iter) {
-    if (conv_it && iter >= *conv_it) return;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_merge_blocks(int nbead, PMETiles t, PMETiles t16, PMEGeom g,
+                                                      const real *__restrict__ blocks, real *__restrict__ grid,
+                                                      const int *__restrict__ conv_it, int iter) {
+    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
-    if (idx >= (size_t)K1 * K2 * K3) return;
+    const size_t ng = (size_t)K1 * K2 * K3;
+    if (gidx >= ng * nbead) return;
+    const int bead = (int)(gidx / ng);
+    if (cg_done(conv_it, bead, iter)) return;
+    const size_t idx = gidx - (size_t)bead * ng;
+    const size_t kb = (size_t)bead * t.ntiles;
     const int gz = (int)(idx % K3), gy = (int)((idx / K3) % K2), gx = (int)(idx / ((size_t)K3 * K2));
     const int gg[3] = {gx, gy, gz};
     int u[3][2], l[3][2], nu[3];
@@ -277,21 +283,23 @@ __global__ __launch_bounds__(256) void k_merge_blocks(PMETiles t, PMETiles t16, 
         for (int i1 = 0; i1 < nu[1]; ++i1)
             for (int i2 = 0; i2 < nu[2]; ++i2) {
                 const int uu[3] = {u[0][i0], u[1][i1], u[2][i2]};
-                const int key = fine_key(uu, t16);
-                acc += blocks[(size_t)key * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
+                const size_t key = kb + fine_key(uu, t16);
+                acc += blocks[key * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
             }
-    grid[idx] = acc;
+    grid[gidx] = acc;
 }
 
 // Convolution on the half spectrum: F[Q] *= Gc; with ENERGY also sum (w3/2) Gc |F[Q]|^2 per block
-// (= sum G |F[Q]|^2 with G of pme.py:243-267), FP64.
+// (= sum G |F[Q]|^2 with G of pme.py:243-267), FP64.  Grid (blocks per bead, beads): bead b's
+// transform at cgrid + 2 b nc, its partials at epart[b * gridDim.x + blockIdx.x].
 template <typename real, bool ENERGY>
 __global__ __launch_bounds__(256) void k_convolve(size_t nc, int K3h, int K3even, real *__restrict__ cgrid,
                                                   const real *__restrict__ Gc, double *__restrict__ epart,
                                                   const int *__restrict__ conv_it, int iter) {
-    if (conv_it && iter >= *conv_it) return;
+    if (cg_done(conv_it, blockIdx.y, iter)) return;
     __shared__ double lds[16];
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    cgrid += 2 * nc * blockIdx.y;
     double e[1] = {0.0};
     if (s < nc) {
         const real gc = Gc[s];
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(256) void k_convolve(size_t nc, int K3h, int K3even
     }
     if (ENERGY) {
         block_sum<1>(e, lds);
-        if (threadIdx.x == 0) epart[blockIdx.x] = e[0];
+        if (threadIdx.x == 0) epart[blockIdx.y * gridDim.x + blockIdx.x] = e[0];
     }
 }
 
@@ -416,7 +424,8 @@ PGM_DEV void combine_out(int mode, int i, real rx, real ry, real rz, const real4
 // tile's (T + P - 1)^3 block (periodic wrap) into LDS once, then every atom binned to the tile
 // (thread per atom, sorted order s, atom sidx[s]) sums its 216 stencil values from LDS with the
 // spline weights stored by k_pme_splines, and combine_out writes its result.  Per-block partials
-// of MODE_FIELD / MODE_AV go to part[tile].
+// of MODE_FIELD / MODE_AV go to part[tile].  Block b * t.ntiles + tile serves tile `tile` of bead
+// b (potential at phi + b K1 K2 K3).
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
                                                       const int *__restrict__ tend, const int *__restrict__ sidx,
@@ -426,15 +435,17 @@ __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, con
                                                       const real4<real> *__restrict__ extra, real cself, int mode,
                                                       real4<real> *__restrict__ out, double *__restrict__ part,
                                                       const int *__restrict__ conv_it, int iter) {
-    if (conv_it && iter >= *conv_it) return;
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int bead = tile / t.ntiles, lt = tile - bead * t.ntiles;
+    if (cg_done(conv_it, bead, iter)) return;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     real *blk = reinterpret_cast<real *>(lds_raw);
     __shared__ double lds[16];
-    const int tile = blockIdx.x, tid = threadIdx.x;
     const int BX = t.B[0], BY = t.B[1], BZ = t.B[2];
-    const int u2 = tile % t.nt[2], u1 = (tile / t.nt[2]) % t.nt[1], u0 = tile / (t.nt[2] * t.nt[1]);
+    const int u2 = lt % t.nt[2], u1 = (lt / t.nt[2]) % t.nt[1], u0 = lt / (t.nt[2] * t.nt[1]);
     const int o0 = u0 * t.T[0], o1 = u1 * t.T[1], o2 = u2 * t.T[2];
     const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    phi += (size_t)bead * K1 * K2 * K3;
     for (int k = tid; k < t.bsize; k += blockDim.x) {
         const int bz = k % BZ, by = (k / BZ) % BY, bx = k / (BZ * BY);
         int gx = o0 + bx, gy = o1 + by, gz = o2 + bz;
@@ -562,7 +573,7 @@ __global__ __launch_bounds__(256) void k_gather_combine(int n, const double *__r
 // the total dE/dd of the atom for the covalent-frame pull-back.  rec_dd (optional) receives
 // dU_rec/dd itself (for the strain derivative).
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_gather_forces(int n, const double *__restrict__ pos, PMEGeom g,
+__global__ __launch_bounds__(256) void k_gather_forces(int n, int nper, const double *__restrict__ pos, PMEGeom g,
                                                        const real *__restrict__ phi, const real4<real> *__restrict__ dq,
                                                        const double *__restrict__ dd, const double *__restrict__ rowf,
                                                        double cself, double *__restrict__ rec_dr, double *__restrict__ gp,
@@ -580,6 +591,7 @@ __global__ __launch_bounds__(256) void k_gather_forces(int n, const double *__re
     scaled_dipole<real>(d, g, e);
     const real q = d.w;
     const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
+    phi += (size_t)(i / nper) * K1 * K2 * K3;
     // phi-weighted sums S[wx][wy][wz] with w = 0 (theta), 1 (dtheta), 2 (d2theta), needed:
     // gd_a = S(d along a), gw_a = q S(d along a) + sum_b e_b S(d along a, d along b / d2 if a == b)
     double s100 = 0, s010 = 0, s001 = 0, s200 = 0, s020 = 0, s002 = 0, s110 = 0, s101 = 0, s011 = 0;

@@ -68,38 +68,44 @@ class Result:
     overflow: bool = False
 
 
+def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nbead: int = 1):
+    """Create the device context of sys (nbead copies) on torch's current stream; return (ctx, stream)."""
+    torch = _torch()
+    L = _lib.lib()
+    top = _lib.Topology(
+        sys.n, _ptr(sys.q, C.c_double), _ptr(sys.radius, C.c_double), _ptr(sys.alpha, C.c_double),
+        _ptr(sys.lj_rmin_half, C.c_double), _ptr(sys.lj_sqrt_eps, C.c_double), _ptr(sys.mol, C.c_int),
+        len(sys.cov_i), _ptr(sys.cov_i, C.c_int), _ptr(sys.cov_j, C.c_int), _ptr(sys.cov_c, C.c_double),
+    )
+    cs = _lib.Settings()
+    cs.cutoff, cs.skin, cs.ewald_beta = settings.cutoff, settings.skin, settings.ewald_beta
+    cs.grid[:] = list(K)
+    cs.order = settings.pme_order
+    cs.lj_lrc = int(settings.lj_lrc)
+    cs.dipole_tol, cs.max_iter = settings.dipole_tol, settings.max_iter
+    cs.predictor = PREDICTORS[settings.predictor]
+    cs.fused, cs.norm_refresh, cs.peek = int(settings.fused), settings.norm_refresh, settings.peek
+    cs.precision = 1 if settings.precision == "mixed" else 0
+    cs.recip = 1 if settings.recip == "ewald" else 0
+    cs.k_tol = settings.k_tol
+    cs.nbead = int(nbead)
+    stream = torch.cuda.current_stream()
+    ctx = C.c_void_p()
+    _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),
+                                C.c_void_p(stream.cuda_stream)), "pgm_ctx_create")
+    return ctx, stream
+
+
 class PGMForceField:
     """The pGM + LJ force field of one system on the GPU (forcefield.py:810-1000)."""
 
     def __init__(self, sys: System, H, settings: MDSettings = MDSettings()) -> None:
         """Build the device context: topology, PME grid (settings or grid_size(H, spacing)), plans."""
-        torch = _torch()
         self.sys, self.s = sys, settings
         self.n = sys.n
         H = np.ascontiguousarray(H, float)
         self.K = tuple(int(k) for k in (settings.pme_grid or grid_size(H, settings.pme_spacing)))
-        L = _lib.lib()
-        top = _lib.Topology(
-            sys.n, _ptr(sys.q, C.c_double), _ptr(sys.radius, C.c_double), _ptr(sys.alpha, C.c_double),
-            _ptr(sys.lj_rmin_half, C.c_double), _ptr(sys.lj_sqrt_eps, C.c_double), _ptr(sys.mol, C.c_int),
-            len(sys.cov_i), _ptr(sys.cov_i, C.c_int), _ptr(sys.cov_j, C.c_int), _ptr(sys.cov_c, C.c_double),
-        )
-        cs = _lib.Settings()
-        cs.cutoff, cs.skin, cs.ewald_beta = settings.cutoff, settings.skin, settings.ewald_beta
-        cs.grid[:] = list(self.K)
-        cs.order = settings.pme_order
-        cs.lj_lrc = int(settings.lj_lrc)
-        cs.dipole_tol, cs.max_iter = settings.dipole_tol, settings.max_iter
-        cs.predictor = PREDICTORS[settings.predictor]
-        cs.fused, cs.norm_refresh, cs.peek = int(settings.fused), settings.norm_refresh, settings.peek
-        cs.precision = 1 if settings.precision == "mixed" else 0
-        cs.recip = 1 if settings.recip == "ewald" else 0
-        cs.k_tol = settings.k_tol
-        self._stream = torch.cuda.current_stream()
-        ctx = C.c_void_p()
-        _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),
-                                    C.c_void_p(self._stream.cuda_stream)), "pgm_ctx_create")
-        self._ctx = ctx
+        self._ctx, self._stream = _make_context(sys, H, settings, self.K)
         self._H = H.copy()
         self._built_at = None
 
