@@ -17,6 +17,11 @@ force / dipole errors of mixed vs FP64 at the final configuration, the HBM roofl
 dominant kernel (the CG matvec's row field; HIP events around every launch of the timed steps)
 with the PME spread / gather beside it, and the float64 numpy oracle on the host (cpu_baseline).
 
+Beads (--bead-evals > 0): 32 beads of 1,024 waters (3.94 nm box, PME 48^3, mixed, previous-step
+guess), force evaluations of all beads in one context (every stage one launch for all beads,
+bead-wise CG) against 32 single-bead contexts called one after the other, on the same sequence of
+positions (every evaluation moves every atom by 2e-4 nm Gaussian noise, on the GPU).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--guess learned|mu4|previous]
 """
 
@@ -138,6 +143,65 @@ def roofline_of(live: dict, kern: dict, entries: int, n: int, K3: int) -> dict:
     }
 
 
+def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
+    """Force evaluations of `beads` beads of 1,024 waters: one bead context vs one context per bead."""
+    import numpy as np
+    import torch
+
+    nw, edge, K = 1024, 3.94, 48
+    qs, sys_, H = [], None, None
+    for b in range(beads):
+        sys_, pos, H = P.models.water_box(nw, edge, seed=100 + b)
+        qs.append(pos)
+    q0 = torch.as_tensor(np.stack(qs), device="cuda")
+    s = P.MDSettings(precision="mixed", predictor="previous", cutoff=0.9, skin=0.1, ewald_beta=4.0, pme_grid=(K,) * 3,
+                     pme_order=6, dipole_tol=1e-7, max_iter=200)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    moves = [torch.randn(q0.shape, generator=gen, device="cuda", dtype=torch.float64) * 2e-4 for _ in range(warmup + evals)]
+
+    def batched():
+        pb = P.PGMBeads(sys_, H, s, beads=beads)
+        e, q, its = pb.init_state(), q0.clone(), 0
+        e = pb.compute(q, H, e)[2]  # first solve from alpha * b (untimed)
+        for k in range(warmup + evals):
+            if k == warmup:
+                torch.cuda.synchronize()
+                t0, its = time.perf_counter(), 0
+            q += moves[k]
+            _, U, e = pb.compute(q, H, e)
+            its += e.iters
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / evals, its / evals
+
+    def sequential():
+        ffs = [P.PGMForceField(sys_, H, s) for _ in range(beads)]
+        q = q0.clone()
+        inds = [ff.compute(q[b], H).induction for b, ff in enumerate(ffs)]
+        its = 0
+        for k in range(warmup + evals):
+            if k == warmup:
+                torch.cuda.synchronize()
+                t0, its = time.perf_counter(), 0
+            q += moves[k]
+            m = 0
+            for b, ff in enumerate(ffs):
+                r = ff.compute(q[b], H, None, inds[b])
+                inds[b] = r.induction
+                m = max(m, r.iterations)
+            its += m
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / evals, its / evals
+
+    ms_b, it_b = batched()
+    log(f"beads: {beads} x {nw} waters, one context: {ms_b:.3f} ms per evaluation of all beads, {it_b:.2f} CG it (max over beads)")
+    ms_s, it_s = sequential()
+    log(f"beads: {beads} single-bead contexts in turn: {ms_s:.3f} ms per evaluation of all beads, {it_s:.2f} CG it (max)")
+    return {"workload": f"{beads} beads x {nw} rigid pGM3P-25 waters ({3 * nw} atoms per bead), {edge} nm box, PME {K}^3 "
+                        f"order 6, mixed, previous-step guess, CG tol 1e-7; force evaluation of every bead",
+            "evals": evals, "ms_per_eval_batched": round(ms_b, 4), "ms_per_eval_sequential": round(ms_s, 4),
+            "speedup": round(ms_s / ms_b, 3), "cg_iters_batched": round(it_b, 3), "cg_iters_sequential": round(it_s, 3)}
+
+
 def main() -> int:  # noqa: C901
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +218,7 @@ def main() -> int:  # noqa: C901
                     help="CG initial guess of the measured mixed path")
     ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
     ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
+    ap.add_argument("--bead-evals", type=int, default=10, help="timed force evaluations of the 32-bead leg (0: off)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build has no multi-GPU path", file=sys.stderr)
@@ -250,6 +315,7 @@ def main() -> int:  # noqa: C901
                 "cg_iters_per_step": round(repd["cg_iterations"] / a.fp64_steps, 3),
                 "speedup_mixed_vs_fp64": round(ms_d / ms_m, 3)}
 
+    beads = bead_leg(P, a.bead_evals, 3) if a.bead_evals > 0 else None
     n = 3 * a.waters
     K3 = a.grid**3
     roofline = roofline_of(live, kern, entries, n, K3)
@@ -281,6 +347,7 @@ def main() -> int:  # noqa: C901
         "errors_vs_fp64": None if errors is None else {k: float(f"{v:.3e}") for k, v in errors.items()},
         "energy_drift_check": {"etot_end_kJmol": repm["etot"], "temperature_K": round(repm["temperature"], 2)},
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+        "beads": beads,
         "roofline": roofline,
     }
     if not a.no_cpu_baseline:

@@ -128,6 +128,14 @@ int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy 
 int pgm_bead_count(const pgm_ctx *ctx);
 int pgm_bead_results(pgm_ctx *ctx, pgm_energy *energies, int *iterations, double *residuals);
 
+/* Layout of the CG row field (the data of PGMForceField._row_field, forcefield.py:1395-1402): 0 atom
+ * rows (partner index, G1, G2 per pair in sliced ELL), 1 molecule blocks (contexts of consecutive
+ * 3-atom molecules: per molecule pair the 9 atom pairs' G1, G2, `lanes` = 4, 8 or 16 lanes per
+ * molecule; the default where it applies).  Same sums in another order; rows are rebuilt at the
+ * next evaluation.  pgm_row_layout reports the layout and its lanes. */
+int pgm_set_row_layout(pgm_ctx *ctx, int layout, int lanes);
+int pgm_row_layout(const pgm_ctx *ctx, int *layout, int *lanes);
+
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy
  * and forces at fixed induced dipoles mu (N,3) f64 device. */
 int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_energy *energy, double *forces);

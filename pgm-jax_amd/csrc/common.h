@@ -51,6 +51,32 @@ PGM_DEV void min_image(double &x, double &y, double &z, const Box &b) {
     x -= n * b.h[0];
 }
 
+// min_image that also returns the lattice shifts it applied (n[a] times lattice row a subtracted).
+PGM_DEV void min_image_n(double &x, double &y, double &z, const Box &b, int (&n)[3]) {
+    double m = rint(z * b.inv_diag[2]);
+    x -= m * b.h[6];
+    y -= m * b.h[7];
+    z -= m * b.h[8];
+    n[2] = (int)m;
+    m = rint(y * b.inv_diag[1]);
+    x -= m * b.h[3];
+    y -= m * b.h[4];
+    n[1] = (int)m;
+    m = rint(x * b.inv_diag[0]);
+    x -= m * b.h[0];
+    n[0] = (int)m;
+}
+
+// The same shifts applied in min_image's order (bitwise the min_image result when n is its own).
+PGM_DEV void apply_shift(double &x, double &y, double &z, const Box &b, const int (&n)[3]) {
+    x -= n[2] * b.h[6];
+    y -= n[2] * b.h[7];
+    z -= n[2] * b.h[8];
+    x -= n[1] * b.h[3];
+    y -= n[1] * b.h[4];
+    x -= n[0] * b.h[0];
+}
+
 // ---------------------------------------------------------------------------- reductions
 template <typename T> PGM_DEV T wave_sum(T v) {
 #pragma unroll

@@ -129,6 +129,13 @@ struct pgm_ctx {
     int cap_full = 0, cap = 0;
     bool rows_valid = false;
     DevBuf rows_full, count_full, rows, count, count_near, maxcount, jell, G1, G2;
+    // CG row field layout: atom rows in sliced ELL (jell, G1, G2; RL lanes per atom) or, for 3-site
+    // molecules, molecule blocks (mj, mg; rlm lanes per molecule) from the candidate molecules mrows
+    bool mol3 = false;            // every bead is consecutive 3-atom molecules
+    int layout = 0;               // 0 atom rows, 1 molecule blocks
+    int rlm = 4;
+    int mcap_full = 0, mcap = 0;
+    DevBuf mrows, mcount_full, mcount, mj, mg;
     DevBuf key, atom, skey, satom, cell_start, cell_end, sort_tmp;
     long long entries = 0;
     // reciprocal space: PME, or (ewald) the exact Ewald sum over the half-space index set m
@@ -540,6 +547,39 @@ static int alloc_rows(pgm_ctx *c, int cap) {
     return PGM_OK;
 }
 
+// Molecule-block buffers for candidate capacity mcap_full (records rounded up to rlm lanes).
+static int alloc_mol(pgm_ctx *c, int mcap_full) {
+    const int nmol = c->n / 3;
+    c->mcap_full = mcap_full;
+    c->mcap = (mcap_full + c->rlm - 1) / c->rlm * c->rlm;
+    PGM_TRY(c->mrows.ensure(sizeof(int) * (size_t)nmol * mcap_full));
+    PGM_TRY(c->mcount_full.ensure(sizeof(int) * nmol));
+    PGM_TRY(c->mcount.ensure(sizeof(int) * nmol));
+    PGM_TRY(c->mj.ensure(sizeof(int) * (size_t)nmol * c->mcap));
+    PGM_TRY(c->mg.ensure(c->rsize * MOL_REC * (size_t)nmol * c->mcap));
+    return PGM_OK;
+}
+
+// Candidate partner molecules from the full atom rows (molecule-block layout only).
+static int build_mol_rows(pgm_ctx *c) {
+    const int nmol = c->n / 3, nmol_bead = c->nper / 3;
+    hipStream_t st = c->stream;
+    const size_t lds = sizeof(unsigned int) * 4 * (size_t)((nmol_bead + 31) / 32);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        k_fill_int<<<1, 64, 0, st>>>(1, c->maxcount.as<int>(), 0);
+        k_mol_rows<<<nblk(nmol, 4), 256, lds, st>>>(nmol, nmol_bead, c->rows_full.as<int>(), c->count_full.as<int>(), c->cap_full,
+                                                    c->mcap_full, c->mrows.as<int>(), c->mcount_full.as<int>(), c->maxcount.as<int>());
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_int, c->maxcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const int mc = c->h_pin_int[0];
+        if (mc <= c->mcap_full) return PGM_OK;
+        PGM_TRY(alloc_mol(c, (int)(mc * 1.2) + 8));
+    }
+    g_err = "molecule row capacity could not be sized";
+    return PGM_ERR_STATE;
+}
+
 static int build_rows(pgm_ctx *c, const double *pos) {
     const int n = c->n;
     hipStream_t st = c->stream;
@@ -571,6 +611,7 @@ static int build_rows(pgm_ctx *c, const double *pos) {
         HIP_TRY(hipStreamSynchronize(st));
         const int mc = c->h_pin_int[0];
         if (mc <= c->cap_full) {
+            if (c->layout == 1) PGM_TRY(build_mol_rows(c));
             c->rows_valid = true;
             PGM_TRY(c->pos_build.ensure(sizeof(double) * 3 * (size_t)n));
             HIP_TRY(hipMemcpyAsync(c->pos_build.p, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
@@ -594,7 +635,19 @@ static int prepare(pgm_ctx *c, const double *pos) {
                                                c->cap_full, c->s.cutoff * c->s.cutoff, c->s.ewald_beta * c->s.ewald_beta, c->cap,
                                                c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>());
     LAUNCH_CHECK();
-    if (c->mixed)
+    if (c->layout == 1) {
+        const int nmol = n / 3;
+        const double rc2 = c->s.cutoff * c->s.cutoff;
+        if (c->mixed)
+            k_mol_tensors<float><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
+                                                               c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2,
+                                                               (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(),
+                                                               c->mcount.as<int>());
+        else
+            k_mol_tensors<double><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
+                                                                c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2, c->s.ewald_beta,
+                                                                c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>());
+    } else if (c->mixed)
         k_pair_tensors<float><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
             (float)c->s.ewald_beta, c->jell.as<int>(), c->G1.as<float>(), c->G2.as<float>());
@@ -746,6 +799,29 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
     return PGM_OK;
 }
 
+// The CG row field of dq into vrow on stream st, in the context's layout.
+template <typename real, bool WITHQ>
+static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, int iter, hipStream_t st) {
+    using R4 = real4<real>;
+    if (c->layout == 1) {
+        const int nmol = c->n / 3, nb = c->nper / 3;
+        const int *mj = c->mj.as<int>(), *mc = c->mcount.as<int>();
+        const real *mg = c->mg.as<real>();
+        R4 *out = c->vrow.as<R4>();
+        switch (c->rlm) {
+            case 4: k_row_field_mol<real, WITHQ, 4><<<nblk((long long)nmol * 4, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
+            case 8: k_row_field_mol<real, WITHQ, 8><<<nblk((long long)nmol * 8, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
+            default: k_row_field_mol<real, WITHQ, 16><<<nblk((long long)nmol * 16, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
+        }
+    } else {
+        k_row_field<real, WITHQ><<<nblk((long long)c->n * RL, 256), 256, 0, st>>>(
+            c->n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq,
+            c->box, c->vrow.as<R4>(), conv, iter);
+    }
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
 template <typename real>
 static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, const real4<real> *extra, real4<real> *out,
                       double *part, int iter) {
@@ -757,14 +833,9 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     const int t_row = timing_begin(c, 0, c->side);
     if (withq)
-        k_row_field<real, true><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
-            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
-            c->vrow.as<real4<real>>(), conv, iter);
+        PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side)));
     else
-        k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, c->side>>>(
-            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
-            c->vrow.as<real4<real>>(), conv, iter);
-    LAUNCH_CHECK();
+        PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side)));
     timing_end(c, 0, t_row, c->side);
     HIP_TRY(hipEventRecord(c->ev_join, c->side));
     const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
@@ -926,15 +997,19 @@ static const GuessCoef MU4_COEF{{4.0, -6.0, 4.0, -1.0, 0.0, 0.0, 0.0, 0.0}};
 
 // Before the solve (predictor 4, ring full): features of every atom, the network's output when
 // trained, and x0 by the judge's mode: the fitted linear predictor (+ the network), or mu4.
-template <typename real> static int guess_x0(pgm_ctx *c, const double *pos, real4<real> *x0) {
+// x0 of the learned guess.  The features and the network run when the judge uses the network
+// (mode 2) or when this solve is harvested (the harvest scores the network and stores features).
+template <typename real> static int guess_x0(pgm_ctx *c, const double *pos, real4<real> *x0, bool harvest) {
     auto &G = c->guess;
     const int n = c->n;
     hipStream_t st = c->stream;
-    k_guess_features<<<nblk(n, 256), 256, 0, st>>>(n, pos, G.ring.as<double>(), G.head, 1.0 / G.scale, G.feat.as<float>(),
-                                                   G.frames.as<double>());
-    LAUNCH_CHECK();
     G.have_y = false;
-    if (G.trained) {
+    if (harvest || G.mode == 2) {
+        k_guess_features<<<nblk(n, 256), 256, 0, st>>>(n, pos, G.ring.as<double>(), G.head, 1.0 / G.scale, G.feat.as<float>(),
+                                                       G.frames.as<double>());
+        LAUNCH_CHECK();
+    }
+    if (G.trained && (harvest || G.mode == 2)) {
         PGM_TRY(guess_infer(c, G.feat.as<float>(), n, G.y.as<float>()));
         G.have_y = true;
     }
@@ -1063,7 +1138,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     const bool have = K > 0 && ind->count >= K;
     const bool use_net = learned && c->guess.nring >= GUESS_K;
     if (use_net) {
-        PGM_TRY(guess_x0<real>(c, c->pos, x0));
+        PGM_TRY(guess_x0<real>(c, c->pos, x0, ind->count % c->guess.every == 0));
     } else if (have) {
         if (c->s.predictor == 4 && K == 4) {
             cf[0] = 4, cf[1] = -6, cf[2] = 4, cf[3] = -1;
@@ -1495,6 +1570,14 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         g_err = "pgm_ctx_create: beads need a tiled PME grid (per axis at least three tiles of P-1..8 points)";
         return fail(PGM_ERR_ARG);
     }
+    {  // molecule blocks of the CG row field when every molecule is 3 consecutive atoms (and the
+       // candidate bitmap of a bead fits 64 KB of LDS per block)
+        bool m3 = nper % 3 == 0;
+        for (int i = 0; i < nper && m3; ++i) m3 = top->mol[i] == i / 3;
+        c->mol3 = m3 && (nper / 3 + 31) / 32 <= 4096;
+        c->layout = c->mol3 ? 1 : 0;
+        if (c->layout == 1 && (r = alloc_mol(c, 64)) < 0) return fail(r);
+    }
     if ((r = set_box(c, H)) < 0) return fail(r);
     // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
     const double rl = s->cutoff + s->skin;
@@ -1527,6 +1610,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
                      &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy, &c->bead_it, &c->bead_res,
+                     &c->mrows, &c->mcount_full, &c->mcount, &c->mj, &c->mg,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
                      &c->loc16, &c->blocks,
@@ -1656,6 +1740,32 @@ extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pg
 }
 
 extern "C" int pgm_bead_count(const pgm_ctx *c) { return c ? c->nbead : PGM_ERR_ARG; }
+
+extern "C" int pgm_set_row_layout(pgm_ctx *c, int layout, int lanes) {
+    if (!c || (layout != 0 && layout != 1) || (layout == 1 && lanes != 4 && lanes != 8 && lanes != 16)) {
+        g_err = "pgm_set_row_layout: layout 0 (atom rows) or 1 (molecule blocks, 4, 8 or 16 lanes)";
+        return PGM_ERR_ARG;
+    }
+    if (layout == 1 && !c->mol3) {
+        g_err = "pgm_set_row_layout: molecule blocks need consecutive 3-atom molecules";
+        return PGM_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->layout = layout;
+    if (layout == 1) {
+        c->rlm = lanes;
+        PGM_TRY(alloc_mol(c, std::max(c->mcap_full, 64)));
+    }
+    c->rows_valid = false;
+    return PGM_OK;
+}
+
+extern "C" int pgm_row_layout(const pgm_ctx *c, int *layout, int *lanes) {
+    if (!c) return PGM_ERR_ARG;
+    if (layout) *layout = c->layout;
+    if (lanes) *lanes = c->layout == 1 ? c->rlm : RL;
+    return PGM_OK;
+}
 
 extern "C" int pgm_bead_results(pgm_ctx *c, pgm_energy *energies, int *iterations, double *residuals) {
     if (!c) return PGM_ERR_ARG;
@@ -2218,11 +2328,7 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
     rc = timed(4, [&] { rocfft_execute(c->bwd, in2, out2, c->info); });
     if (rc < 0) return rc;
-    rc = timed(5, [&] {
-        k_row_field<real, false><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
-            n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq,
-            c->box, c->vrow.as<R4>(), nullptr, -1);
-    });
+    rc = timed(5, [&] { (void)row_field_launch<real, false>(c, dq, nullptr, -1, st); });
     if (rc < 0) return rc;
     rc = timed(6, [&] { (void)gather_combine<real>(c, dq, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1); });
     if (rc < 0) return rc;

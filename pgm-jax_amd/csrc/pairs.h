@@ -400,4 +400,225 @@ __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__rest
     }
 }
 
+// ---------------------------------------------------------------------------- molecule-blocked field rows
+// For systems of 3-site molecules (atoms 3I, 3I+1, 3I+2) the CG row field runs on molecule pairs:
+// record (I, J) holds the 9 atom pairs (p of I, k of J) as G1[3p+k], G2[3p+k] (zero beyond the
+// cutoff and for p == k when J == I), so one lane fetches J's three positions and dipoles once for
+// nine pair terms instead of one gather per pair.  Records of molecule I sit at
+// mell(e, I) = (e / RLM) nmol RLM + I RLM + e % RLM for its RLM lanes.  The record's word is
+// J | code << 26: code = the lattice shifts (n_a + 1 in two bits per axis) shared by every pair
+// within the cutoff, or MOL_OWN_IMAGES when they differ (the pairs then take min_image each).
+constexpr int MOL_OWN_IMAGES = 63;
+constexpr int MOL_REC = 18;  // reals per record: G1 of the 9 pairs, then G2
+PGM_DEV size_t mell(int e, int I, int nmol, int rlm) {
+    return (size_t)(e / rlm) * ((size_t)nmol * rlm) + (size_t)I * rlm + (e % rlm);
+}
+
+// Candidate partner molecules of molecule I: the molecules of the atoms in the full (cutoff + skin)
+// rows of I's three atoms, deduplicated with a bitmap over the bead's molecules in LDS (`words`
+// words per wave) and written in increasing order.  One wave per molecule, no block barriers.
+__global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const int *__restrict__ rows_full,
+                                                  const int *__restrict__ count_full, int cap_full, int mcap,
+                                                  int *__restrict__ mrows, int *__restrict__ mcount, int *__restrict__ maxcount) {
+    extern __shared__ unsigned int mol_bits[];
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int I = blockIdx.x * (blockDim.x / WAVE) + w;
+    if (I >= nmol) return;
+    const int words = (nmol_bead + 31) / 32;
+    unsigned int *bits = mol_bits + (size_t)w * words;
+    for (int k = lane; k < words; k += WAVE) bits[k] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int m0 = (I / nmol_bead) * nmol_bead;
+    for (int p = 0; p < 3; ++p) {
+        const int i = 3 * I + p;
+        const int c = min(count_full[i], cap_full);
+        const int *row = rows_full + (size_t)i * cap_full;
+        for (int e = lane; e < c; e += WAVE) {
+            const int J = row[e] / 3 - m0;
+            atomicOr(&bits[J >> 5], 1u << (J & 31));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int per = (words + WAVE - 1) / WAVE;
+    const int w0 = min(words, lane * per), w1 = min(words, w0 + per);
+    int cnt = 0;
+    for (int k = w0; k < w1; ++k) cnt += __popc(bits[k]);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+    }
+    int pos = incl - cnt;
+    const int total = __shfl(incl, WAVE - 1, WAVE);
+    int *mrow = mrows + (size_t)I * mcap;
+    for (int k = w0; k < w1; ++k) {
+        unsigned int b = bits[k];
+        while (b) {
+            const int t = __ffs(b) - 1;
+            b &= b - 1u;
+            if (pos < mcap) mrow[pos] = m0 + 32 * k + t;
+            ++pos;
+        }
+    }
+    if (lane == 0) {
+        mcount[I] = total;
+        atomicMax(maxcount, total);
+    }
+}
+
+// Records of the molecule blocks, once per geometry (the molecule form of k_compact_rows +
+// k_pair_tensors): every candidate J of I with at least one atom pair inside rc is kept, in candidate
+// order, its G1 / G2 by pair_G (both branches) from the FP64 minimum-image displacement rounded to
+// real.  One wave per molecule, lanes over candidates, ballot compaction.
+template <typename real>
+__global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
+                                                     int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
+                                                     double rc2, real beta, int *__restrict__ mj, real *__restrict__ mg,
+                                                     int *__restrict__ mcount) {
+    const int lane = threadIdx.x % WAVE;
+    const int I = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (I >= nmol) return;
+    const double4 a[3] = {xr[3 * I], xr[3 * I + 1], xr[3 * I + 2]};
+    const int cf = min(mcount_full[I], mcap_full);
+    const int *cand = mrows + (size_t)I * mcap_full;
+    int kept = 0;
+    for (int e0 = 0; e0 < cf; e0 += WAVE) {
+        const int e = e0 + lane;
+        bool keep = false;
+        int J = 0, code = 0;
+        real G1[9], G2[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) G1[t] = G2[t] = real(0);
+        if (e < cf) {
+            J = cand[e];
+            int n0[3] = {0, 0, 0};
+            bool first = true, same = true;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (J == I && p == k) continue;
+                    const double4 b = xr[3 * J + k];
+                    double dx = a[p].x - b.x, dy = a[p].y - b.y, dz = a[p].z - b.z;
+                    int n[3];
+                    min_image_n(dx, dy, dz, box, n);
+                    const double r2d = dx * dx + dy * dy + dz * dz;
+                    if (r2d >= rc2) continue;
+                    if (first) {
+                        n0[0] = n[0], n0[1] = n[1], n0[2] = n[2];
+                        first = false;
+                    } else if (n[0] != n0[0] || n[1] != n0[1] || n[2] != n0[2]) {
+                        same = false;
+                    }
+                    const real x = (real)dx, y = (real)dy, z = (real)dz;
+                    const real aij = rsqrt_(real(2) * (real)(a[p].w + b.w));
+                    real G[3];
+                    pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
+                    G1[3 * p + k] = G[1];
+                    G2[3 * p + k] = G[2];
+                    keep = true;
+                }
+            const bool small = n0[0] >= -1 && n0[0] <= 1 && n0[1] >= -1 && n0[1] <= 1 && n0[2] >= -1 && n0[2] <= 1;
+            code = (same && small) ? ((n0[0] + 1) | ((n0[1] + 1) << 2) | ((n0[2] + 1) << 4)) : MOL_OWN_IMAGES;
+        }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const unsigned long long m = __ballot(keep);
+        const int slot_e = kept + __popcll(m & below);
+        if (keep && slot_e < mcap) {
+            const size_t q = mell(slot_e, I, nmol, rlm);
+            mj[q] = (int)((unsigned)J | ((unsigned)code << 26));
+            real *g = mg + q * MOL_REC;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                g[t] = G1[t];
+                g[9 + t] = G2[t];
+            }
+        }
+        kept += __popcll(m);
+    }
+    if (lane == 0) mcount[I] = kept < mcap ? kept : mcap;
+}
+
+// Row field of the molecule blocks: out_i for the three atoms of I (the same sum as k_row_field over
+// the pairs within rc, in molecule-pair order).  RLM lanes per molecule.
+template <typename real, bool WITHQ, int RLM>
+__global__ __launch_bounds__(256) void k_row_field_mol(int nmol, int nmol_bead, const int *__restrict__ mj,
+                                                       const real *__restrict__ mg, const int *__restrict__ mcount,
+                                                       const double4 *__restrict__ xr, const real4<real> *__restrict__ dq,
+                                                       Box box, real4<real> *__restrict__ out, const int *__restrict__ conv_it,
+                                                       int iter) {
+    const int lane = threadIdx.x % RLM;
+    const int I = blockIdx.x * (blockDim.x / RLM) + threadIdx.x / RLM;
+    if (I >= nmol || cg_done(conv_it, I / nmol_bead, iter)) return;
+    const double4 a[3] = {xr[3 * I], xr[3 * I + 1], xr[3 * I + 2]};
+    real acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = real(0);
+    const int c = mcount[I];
+    for (int e = lane; e < c; e += RLM) {
+        const size_t q = mell(e, I, nmol, RLM);
+        const unsigned word = (unsigned)mj[q];
+        const int J = (int)(word & ((1u << 26) - 1u)), code = (int)(word >> 26);
+        real G[MOL_REC];
+        if (sizeof(real) == 4) {
+            const float2 *g2 = reinterpret_cast<const float2 *>(mg + q * MOL_REC);
+#pragma unroll
+            for (int t = 0; t < MOL_REC / 2; ++t) {
+                const float2 v = g2[t];
+                G[2 * t] = (real)v.x;
+                G[2 * t + 1] = (real)v.y;
+            }
+        } else {
+            const double2 *g2 = reinterpret_cast<const double2 *>(mg + q * MOL_REC);
+#pragma unroll
+            for (int t = 0; t < MOL_REC / 2; ++t) {
+                const double2 v = g2[t];
+                G[2 * t] = (real)v.x;
+                G[2 * t + 1] = (real)v.y;
+            }
+        }
+        double4 b[3];
+        real4<real> d[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            b[k] = xr[3 * J + k];
+            d[k] = dq[3 * J + k];
+        }
+        const int n[3] = {(code & 3) - 1, ((code >> 2) & 3) - 1, ((code >> 4) & 3) - 1};
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double dx = a[p].x - b[k].x, dy = a[p].y - b[k].y, dz = a[p].z - b[k].z;
+                if (code == MOL_OWN_IMAGES)
+                    min_image(dx, dy, dz, box);
+                else
+                    apply_shift(dx, dy, dz, box, n);
+                const real x = (real)dx, y = (real)dy, z = (real)dz;
+                const real g1 = G[3 * p + k], g2 = G[9 + 3 * p + k];
+                const real dkx = d[k].x * x + d[k].y * y + d[k].z * z;
+                real cf = -g2 * dkx;
+                if (WITHQ) cf -= d[k].w * g1;
+                acc[3 * p] += cf * x + g1 * d[k].x;
+                acc[3 * p + 1] += cf * y + g1 * d[k].y;
+                acc[3 * p + 2] += cf * z + g1 * d[k].z;
+            }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = group_sum<RLM>(acc[t]);
+    if (lane == 0)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            real4<real> v;
+            v.x = acc[3 * p];
+            v.y = acc[3 * p + 1];
+            v.z = acc[3 * p + 2];
+            v.w = 0;
+            out[3 * I + p] = v;
+        }
+}
+
 }  // namespace pgm

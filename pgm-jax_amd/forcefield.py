@@ -315,3 +315,15 @@ class PGMForceField:
     def row_entries(self) -> int:
         """Number of stored row entries within the cutoff at the last evaluation (pairs counted twice)."""
         return int(_lib.lib().pgm_row_entries(self._ctx))
+
+    def set_row_layout(self, layout: str = "molecule", lanes: int = 4) -> None:
+        """Choose the CG row-field layout: "atom" rows or "molecule" blocks (3-site molecules; 4, 8 or 16 lanes)."""
+        code = {"atom": 0, "molecule": 1}[layout]
+        _lib.check(_lib.lib().pgm_set_row_layout(self._ctx, code, int(lanes)), "pgm_set_row_layout")
+        self._built_at = None
+
+    def row_layout(self) -> tuple[str, int]:
+        """Return the CG row-field layout and its lanes per row."""
+        lay, lanes = C.c_int(), C.c_int()
+        _lib.check(_lib.lib().pgm_row_layout(self._ctx, C.byref(lay), C.byref(lanes)), "pgm_row_layout")
+        return ("atom", "molecule")[lay.value], lanes.value
