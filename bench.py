@@ -100,20 +100,29 @@ def pmc_traffic(kernel: str):
         return None
 
 
-def roofline_of(live: dict, kern: dict, entries: int, n: int, K3: int) -> dict:
+def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int) -> dict:
     """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
 
     Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
     timed steps (row field on its side stream; spread + merge and gather on the main stream).
 
     Algorithmic bytes per launch (each logical read or write once):
-      row field: per row entry j (4 B) + G1, G2 (FP32, 8 B); per atom the dipole read (16 B) and the
-                 field written (16 B);
+      row field, molecule blocks (3-site molecules, the default for water): per record (molecule
+                 pair with a pair inside the cutoff) the partner word (4 B) + 9 G1 + 9 G2 (FP32,
+                 72 B); per atom its position (32 B) and dipole (16 B) read and the field written
+                 (16 B);
+      row field, atom rows: per row entry j (4 B) + G1, G2 (FP32, 8 B); per atom the dipole read
+                 (16 B) and the field written (16 B);
       spread:    per atom positions 12 + q 4 + d 12 B, the FP32 grid written once (K^3 x 4 B) -- the
                  tiled spread plus its block merge;
       gather:    the FP32 grid read once (K^3 x 4 B), per atom d in 12 B and the field out 12 B.
     """
-    row_bytes = entries * 12 + n * 32
+    if layout == "molecule":
+        row_bytes = blocks * 76 + n * 64
+        kname, kdesc = "k_row_field_mol", "k_row_field_mol (CG matvec real-space rows, FP32, molecule blocks of 9 pairs)"
+    else:
+        row_bytes = entries * 12 + n * 32
+        kname, kdesc = "k_row_field", "k_row_field (CG matvec real-space rows, FP32, sliced-ELL j/G1/G2 stream)"
     spread_bytes = n * 28 + K3 * 4
     gather_bytes = K3 * 4 + n * 24
     row_ms, spread_ms, gather_ms = live["row_field"]["ms"], live["spread"]["ms"], live["gather"]["ms"]
@@ -122,17 +131,18 @@ def roofline_of(live: dict, kern: dict, entries: int, n: int, K3: int) -> dict:
         return b / (ms * 1e-3) / 1e9
 
     return {
-        "kernel": "k_row_field (CG matvec real-space rows, FP32, sliced-ELL j/G1/G2 stream)",
+        "kernel": kdesc,
         "bound": "hbm",
         "achieved": round(gbs(row_bytes, row_ms), 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs(row_bytes, row_ms) / HBM_PEAK_GBS, 5),
-        "traffic": pmc_traffic("k_row_field"),
+        "traffic": pmc_traffic(kname),
         "algorithmic_bytes": row_bytes,
         "ms_per_launch": round(row_ms, 5),
         "launches": live["row_field"]["launches"],
         "row_entries": entries,
+        "row_blocks": blocks,
         "spread": {"kernels": "k_spread_tiles + k_merge_blocks", "achieved": round(gbs(spread_bytes, spread_ms), 2),
                    "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
                    "ms_per_launch": round(spread_ms, 5)},
@@ -219,6 +229,8 @@ def main() -> int:  # noqa: C901
     ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
     ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
     ap.add_argument("--bead-evals", type=int, default=10, help="timed force evaluations of the 32-bead leg (0: off)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the library's copy kernel 3 times over MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build has no multi-GPU path", file=sys.stderr)
@@ -230,6 +242,19 @@ def main() -> int:  # noqa: C901
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     P = _package()
+    if a.copy_probe > 0:
+        import ctypes
+
+        nbytes = a.copy_probe << 20
+        src = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        L = P._lib.lib()
+        for _ in range(3):
+            P._lib.check(L.pgm_copy_probe(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "pgm_copy_probe")
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {a.copy_probe} MiB read + written")
+        return 0
     log("building system", a.waters, "waters")
     sys_, pos, H = P.models.water_box(a.waters, a.edge, seed=3)
     vel = P.models.maxwell_boltzmann_rigid(pos, sys_.masses, 298.0, seed=4)
@@ -288,6 +313,8 @@ def main() -> int:  # noqa: C901
     log("live kernel timing over the timed steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
     entries = ffm.row_entries()
+    layout = ffm.row_layout()[0]
+    blocks = ffm.row_blocks()
     log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}), "row entries", entries)
     prev = None
     if a.guess != "previous" and a.prev_steps > 0:  # the same mixed path from the previous-step guess
@@ -318,7 +345,7 @@ def main() -> int:  # noqa: C901
     beads = bead_leg(P, a.bead_evals, 3) if a.bead_evals > 0 else None
     n = 3 * a.waters
     K3 = a.grid**3
-    roofline = roofline_of(live, kern, entries, n, K3)
+    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3)
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,

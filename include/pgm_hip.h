@@ -20,6 +20,8 @@
 #ifndef PGM_HIP_H
 #define PGM_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -99,6 +101,10 @@ int pgm_build_rows(pgm_ctx *ctx, const double *pos);
 
 /* Number of stored row entries (pairs counted twice) within cutoff at the last compute. */
 long long pgm_row_entries(pgm_ctx *ctx);
+
+/* Number of molecule-block records (molecule pairs with a pair inside the cutoff, each direction) at
+ * the last evaluation; 0 with atom rows, -1 before the first. */
+long long pgm_row_blocks(pgm_ctx *ctx);
 
 /* PGMForceField.perm_dipoles (forcefield.py:1032-1042): p (N,3) f64 device. */
 int pgm_perm_dipoles(pgm_ctx *ctx, const double *pos, double *p);
@@ -235,6 +241,10 @@ int pgm_guess_status(pgm_ctx *ctx, double *out);
  * [1] spread, [2] gather. */
 int pgm_set_timing(pgm_ctx *ctx, int on);
 int pgm_kernel_timing(pgm_ctx *ctx, double *avg_ms, long long *launches);
+
+/* Measurement hook (no counterpart in the reference): a plain copy of `bytes` (a multiple of 16)
+ * between device buffers on `stream`, the known-size kernel that calibrates HBM traffic counters. */
+int pgm_copy_probe(void *dst, const void *src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

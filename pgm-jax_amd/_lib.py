@@ -26,6 +26,7 @@ EXPORTS = (
     "pgm_set_box",
     "pgm_build_rows",
     "pgm_row_entries",
+    "pgm_row_blocks",
     "pgm_perm_dipoles",
     "pgm_field",
     "pgm_pme_energy",
@@ -51,6 +52,7 @@ EXPORTS = (
     "pgm_guess_grad",
     "pgm_guess_train",
     "pgm_guess_status",
+    "pgm_copy_probe",
 )
 
 
@@ -164,6 +166,9 @@ def lib() -> C.CDLL:
     L.pgm_build_rows.argtypes = [P, P]
     L.pgm_row_entries.argtypes = [P]
     L.pgm_row_entries.restype = C.c_longlong
+    L.pgm_row_blocks.argtypes = [P]
+    L.pgm_row_blocks.restype = C.c_longlong
+    L.pgm_copy_probe.argtypes = [P, P, C.c_size_t, P]
     L.pgm_perm_dipoles.argtypes = [P, P, P]
     L.pgm_field.argtypes = [P, P, I, P, P]
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
@@ -191,7 +196,7 @@ def lib() -> C.CDLL:
     L.pgm_guess_status.argtypes = [P, C.POINTER(D)]
     L.pgm_kernel_timing.argtypes = [P, C.POINTER(D), C.POINTER(C.c_longlong)]
     for name in EXPORTS:
-        if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries", "pgm_guess_nparams"):
+        if name not in ("pgm_last_error", "pgm_version", "pgm_row_entries", "pgm_row_blocks", "pgm_guess_nparams"):
             getattr(L, name).restype = I
     _lib = L
     return L

@@ -1655,6 +1655,18 @@ extern "C" long long pgm_row_entries(pgm_ctx *c) {
     return s;
 }
 
+extern "C" long long pgm_row_blocks(pgm_ctx *c) {
+    if (!c || !c->rows_valid) return -1;
+    if (c->layout != 1) return 0;
+    const int nmol = c->n / 3;
+    std::vector<int> cnt(nmol);
+    if (hipMemcpyAsync(cnt.data(), c->mcount.p, sizeof(int) * nmol, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    long long s = 0;
+    for (int v : cnt) s += v;
+    return s;
+}
+
 extern "C" int pgm_perm_dipoles(pgm_ctx *c, const double *pos, double *p) {
     if (!c || !pos || !p) return PGM_ERR_ARG;
     k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
@@ -2254,6 +2266,23 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
 extern "C" int pgm_md_rescale(pgm_ctx *c, pgm_md_state *s, double factor) {
     if (!c || !s || !s->vel || !(factor > 0)) return PGM_ERR_ARG;
     k_scale3<<<nblk(3LL * c->n, 256), 256, 0, c->stream>>>(3 * c->n, s->vel, factor);
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- copy probe
+__global__ void k_copy_probe(size_t n4, const float4 *__restrict__ a, float4 *__restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) b[i] = a[i];
+}
+
+extern "C" int pgm_copy_probe(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!dst || !src || bytes % 16 != 0) {
+        g_err = "pgm_copy_probe: device buffers and a multiple of 16 bytes";
+        return PGM_ERR_ARG;
+    }
+    const size_t n4 = bytes / 16;
+    k_copy_probe<<<(unsigned)((n4 + 255) / 256), 256, 0, (hipStream_t)stream>>>(n4, (const float4 *)src, (float4 *)dst);
     LAUNCH_CHECK();
     return PGM_OK;
 }

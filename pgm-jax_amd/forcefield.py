@@ -316,6 +316,10 @@ class PGMForceField:
         """Number of stored row entries within the cutoff at the last evaluation (pairs counted twice)."""
         return int(_lib.lib().pgm_row_entries(self._ctx))
 
+    def row_blocks(self) -> int:
+        """Number of molecule-block records (molecule pairs within the cutoff, both directions) at the last evaluation."""
+        return int(_lib.lib().pgm_row_blocks(self._ctx))
+
     def set_row_layout(self, layout: str = "molecule", lanes: int = 4) -> None:
         """Choose the CG row-field layout: "atom" rows or "molecule" blocks (3-site molecules; 4, 8 or 16 lanes)."""
         code = {"atom": 0, "molecule": 1}[layout]
