@@ -471,8 +471,11 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 
 // Records of the molecule blocks, once per geometry (the molecule form of k_compact_rows +
 // k_pair_tensors): every candidate J of I with at least one atom pair inside rc is kept, in candidate
-// order, its G1 / G2 by pair_G (both branches) from the FP64 minimum-image displacement rounded to
-// real.  One wave per molecule, lanes over candidates, ballot compaction.
+// order, its G1 / G2 by pair_G from the FP64 minimum-image displacement rounded to real.  One wave
+// per molecule, lanes over candidates, ballot compaction.  Divergence of pair_G's series branches is
+// kept out of the common path: the closed form (both screenings with (a r)^2, (beta r)^2 >= 1) is
+// evaluated for every pair at once, the other pairs are marked and then evaluated one per lane per
+// round, and the own molecule's record (J == I, six close pairs) is computed by six lanes together.
 template <typename real>
 __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
                                                      int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
@@ -484,29 +487,36 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
     const double4 a[3] = {xr[3 * I], xr[3 * I + 1], xr[3 * I + 2]};
     const int cf = min(mcount_full[I], mcap_full);
     const int *cand = mrows + (size_t)I * mcap_full;
-    int kept = 0;
+    int kept = 0, self_slot = -1;
     for (int e0 = 0; e0 < cf; e0 += WAVE) {
         const int e = e0 + lane;
-        bool keep = false;
-        int J = 0, code = 0;
+        bool keep = false, own = false;
+        int J = 0, code = MOL_OWN_IMAGES;
+        unsigned near = 0u;
         real G1[9], G2[9];
+        double4 b[3];
 #pragma unroll
         for (int t = 0; t < 9; ++t) G1[t] = G2[t] = real(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) b[k] = make_double4(0.0, 0.0, 0.0, 0.0);
         if (e < cf) {
             J = cand[e];
+            own = J == I;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) b[k] = xr[3 * J + k];
             int n0[3] = {0, 0, 0};
             bool first = true, same = true;
 #pragma unroll
             for (int p = 0; p < 3; ++p)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    if (J == I && p == k) continue;
-                    const double4 b = xr[3 * J + k];
-                    double dx = a[p].x - b.x, dy = a[p].y - b.y, dz = a[p].z - b.z;
+                    if (own && p == k) continue;
+                    double dx = a[p].x - b[k].x, dy = a[p].y - b[k].y, dz = a[p].z - b[k].z;
                     int n[3];
                     min_image_n(dx, dy, dz, box, n);
-                    const double r2d = dx * dx + dy * dy + dz * dz;
-                    if (r2d >= rc2) continue;
+                    if (dx * dx + dy * dy + dz * dz >= rc2) continue;
+                    keep = true;
+                    if (own) continue;  // the own molecule's pairs come after the compaction
                     if (first) {
                         n0[0] = n[0], n0[1] = n[1], n0[2] = n[2];
                         first = false;
@@ -514,15 +524,40 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                         same = false;
                     }
                     const real x = (real)dx, y = (real)dy, z = (real)dz;
-                    const real aij = rsqrt_(real(2) * (real)(a[p].w + b.w));
-                    real G[3];
-                    pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
-                    G1[3 * p + k] = G[1];
-                    G2[3 * p + k] = G[2];
-                    keep = true;
+                    const real r2 = x * x + y * y + z * z;
+                    const real aij = rsqrt_(real(2) * (real)(a[p].w + b[k].w));
+                    if (aij * aij * r2 >= real(1) && beta * beta * r2 >= real(1)) {
+                        real G[3];
+                        pair_G_far<real, 3>(aij, beta, r2, G);
+                        G1[3 * p + k] = G[1];
+                        G2[3 * p + k] = G[2];
+                    } else {
+                        near |= 1u << (3 * p + k);
+                    }
                 }
             const bool small = n0[0] >= -1 && n0[0] <= 1 && n0[1] >= -1 && n0[1] <= 1 && n0[2] >= -1 && n0[2] <= 1;
-            code = (same && small) ? ((n0[0] + 1) | ((n0[1] + 1) << 2) | ((n0[2] + 1) << 4)) : MOL_OWN_IMAGES;
+            if (!own && same && small) code = (n0[0] + 1) | ((n0[1] + 1) << 2) | ((n0[2] + 1) << 4);
+        }
+        // pairs off the closed form: one per lane per round (the series branches run on useful lanes)
+        while (__ballot(near != 0u)) {
+            if (near) {
+                const int t = __ffs(near) - 1;
+                near &= near - 1u;
+                const int p = t / 3, k = t - 3 * p;
+                const double4 ap = xr[3 * I + p], bk = xr[3 * J + k];  // reloaded (cached) rather than selected
+                double dx = ap.x - bk.x, dy = ap.y - bk.y, dz = ap.z - bk.z;
+                min_image(dx, dy, dz, box);
+                const real x = (real)dx, y = (real)dy, z = (real)dz;
+                const real aij = rsqrt_(real(2) * (real)(ap.w + bk.w));
+                real G[3];
+                pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
+#pragma unroll
+                for (int u = 0; u < 9; ++u)
+                    if (u == t) {
+                        G1[u] = G[1];
+                        G2[u] = G[2];
+                    }
+            }
         }
         const unsigned long long below = (1ull << lane) - 1ull;
         const unsigned long long m = __ballot(keep);
@@ -531,13 +566,39 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
             const size_t q = mell(slot_e, I, nmol, rlm);
             mj[q] = (int)((unsigned)J | ((unsigned)code << 26));
             real *g = mg + q * MOL_REC;
+            if (own) {  // the diagonal (p == k) only; the six pairs follow
+                g[0] = g[4] = g[8] = real(0);
+                g[9] = g[13] = g[17] = real(0);
+                self_slot = slot_e;
+            } else {
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                g[t] = G1[t];
-                g[9 + t] = G2[t];
+                for (int t = 0; t < 9; ++t) {
+                    g[t] = G1[t];
+                    g[9 + t] = G2[t];
+                }
             }
         }
         kept += __popcll(m);
+    }
+    // the own molecule's record: lanes 0..5 take the pairs (0,1) (0,2) (1,0) (1,2) (2,0) (2,1)
+    const unsigned long long hs = __ballot(self_slot >= 0);
+    if (hs) {
+        const int sslot = __shfl(self_slot, (int)__ffsll((long long)hs) - 1, WAVE);
+        if (lane < 6) {
+            const int p = lane / 2, k = (lane % 2) + (lane % 2 >= p ? 1 : 0);
+            const double4 ap = xr[3 * I + p], ak = xr[3 * I + k];
+            double dx = ap.x - ak.x, dy = ap.y - ak.y, dz = ap.z - ak.z;
+            min_image(dx, dy, dz, box);
+            real G[3] = {real(0), real(0), real(0)};
+            if (dx * dx + dy * dy + dz * dz < rc2) {
+                const real x = (real)dx, y = (real)dy, z = (real)dz;
+                const real aij = rsqrt_(real(2) * (real)(ap.w + ak.w));
+                pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
+            }
+            real *g = mg + mell(sslot, I, nmol, rlm) * MOL_REC;
+            g[3 * p + k] = G[1];
+            g[9 + 3 * p + k] = G[2];
+        }
     }
     if (lane == 0) mcount[I] = kept < mcap ? kept : mcap;
 }
