@@ -171,6 +171,7 @@ def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
 
     def batched():
         pb = P.PGMBeads(sys_, H, s, beads=beads)
+        log("beads autotune", json.dumps(pb.autotune(q0, reps=5)))
         e, q, its = pb.init_state(), q0.clone(), 0
         e = pb.compute(q, H, e)[2]  # first solve from alpha * b (untimed)
         for k in range(warmup + evals):
@@ -287,7 +288,8 @@ def main() -> int:  # noqa: C901
         if ff is None:
             ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, predictor=predictor, **base))
             md = P.RigidWaterMD(ff, pos, vel)
-        log(precision, predictor, "warmup", warmup)
+        tune = ff.autotune(md.pos, reps=10)  # row-field layout and spread tiles for this context (untimed)
+        log(precision, predictor, "autotune", json.dumps(tune))
         md.run(warmup, a.dt_fs)
         torch.cuda.synchronize()
         log(precision, predictor, "timed", steps)
@@ -302,6 +304,7 @@ def main() -> int:  # noqa: C901
             ff.set_timing(False)
         log(precision, f"{ms:.3f} ms/step, {rep['cg_iterations'] / steps:.2f} CG it/step, T {rep['temperature']:.1f} K, "
             f"Etot {rep['etot']:.3f}, rebuilds {rep['row_rebuilds']}")
+        rep["autotune"] = tune
         return ff, md, rep, ms
 
     # the measured path: mixed precision with the guess (the equilibrated context keeps the trained network)
@@ -368,6 +371,7 @@ def main() -> int:  # noqa: C901
             "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": a.guess,
         },
         "cg_iters_per_step": round(repm["cg_iterations"] / a.steps, 3),
+        "autotune": repm["autotune"],
         "learned_guess": guess_status,
         "mixed_previous_guess": prev,
         "fp64": fp64,

@@ -142,6 +142,13 @@ int pgm_bead_results(pgm_ctx *ctx, pgm_energy *energies, int *iterations, double
 int pgm_set_row_layout(pgm_ctx *ctx, int layout, int lanes);
 int pgm_row_layout(const pgm_ctx *ctx, int *layout, int *lanes);
 
+/* Autotuner (no counterpart in the reference): times, on this context's system at positions pos
+ * (device), every applicable row-field layout (molecule blocks with 4, 8, 16 lanes; atom rows) and
+ * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that),
+ * `reps` launches each, and keeps the fastest.  report (host, 8 doubles): layout, lanes, spread tile
+ * width per axis (3), row-field ms, spread + merge + FFT + gather ms, layouts tried.  Synchronises. */
+int pgm_autotune(pgm_ctx *ctx, const double *pos, int reps, double *report);
+
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy
  * and forces at fixed induced dipoles mu (N,3) f64 device. */
 int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_energy *energy, double *forces);

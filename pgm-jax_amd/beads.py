@@ -25,7 +25,7 @@ import dataclasses
 import numpy as np
 
 from . import _lib
-from .forcefield import InductionState, _dev, _make_context, _ptr, _torch
+from .forcefield import InductionState, _autotune, _dev, _make_context, _ptr, _torch
 from .settings import MDSettings, grid_size
 from .system import System
 
@@ -109,6 +109,12 @@ class PGMBeads:
                 return
         _lib.check(_lib.lib().pgm_build_rows(self._ctx, C.c_void_p(q.data_ptr())), "pgm_build_rows")
         self._built_at = q.clone()
+
+    def autotune(self, q, reps: int = 20) -> dict:
+        """Time every row-field layout and PME spread-tile width on the beads q (P, N, 3) and keep the fastest."""
+        q = _dev(q, (self.P, self.n, 3))
+        self._built_at = None
+        return _autotune(self._ctx, q, reps)
 
     def bead_results(self) -> tuple[list[dict], np.ndarray, np.ndarray]:
         """Per-bead energies {"elec","vdw","total"}, CG iterations and final residuals of the last compute."""

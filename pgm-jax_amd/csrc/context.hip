@@ -153,6 +153,7 @@ struct pgm_ctx {
     // spread tiles of T8 points per axis (tiles8, one fine key each) inside gather tiles of 2 T8
     // (tiles); tstart/tend by fine key, tstart16/tend16 by gather tile
     bool tiled = false;
+    int tile_cap = 8;  // largest spread-tile width tried per axis (pgm_autotune lowers it)
     PMETiles tiles{}, tiles8{};
     DevBuf pkey, pidx, pskey, psidx, tstart, tend, tstart16, tend16, spl, loc, loc16, blocks, pme_sort_tmp;
     // vectors (real4 N) and FP64 (N, 3) work arrays
@@ -400,8 +401,8 @@ static int allow_lds(pgm_ctx *c, size_t spread, size_t gather) {
     return r;
 }
 
-// Spread tiles: per axis the largest T8 in 8..P-1 with at least three tiles and a last tile at least
-// P-1 wide; gather tiles of T16 = 2 T8 (the union of two spread tiles per axis).  No such T8: the
+// Spread tiles: per axis the largest T8 in tile_cap (8 unless tuned)..P-1 with at least three tiles
+// and a last tile at least P-1 wide; gather tiles of T16 = 2 T8 (the union of two spread tiles per axis).  No such T8: the
 // atomic spread and the per-atom gather are used.
 static int setup_tiles(pgm_ctx *c) {
     const int P = c->s.order;
@@ -410,7 +411,7 @@ static int setup_tiles(pgm_ctx *c) {
     for (int a = 0; a < 3; ++a) {
         const int K = c->s.grid[a];
         int best = 0;
-        for (int T = 8; T >= P - 1 && !best; --T) {
+        for (int T = c->tile_cap; T >= P - 1 && !best; --T) {
             const int nt = (K + T - 1) / T;
             const int last = K - (nt - 1) * T;
             if (nt >= 3 && last >= P - 1) best = T;
@@ -762,6 +763,14 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
     void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
     FFT_TRY(rocfft_execute(c->bwd, in2, out2, c->info));
     return PGM_OK;
+}
+
+template <typename real> static int spread_fft_only(pgm_ctx *c, const real4<real> *dq) {
+    const bool timing = c->timing;
+    c->timing = false;
+    const int r = spread_fft<real>(c, dq, false, false, nullptr, -1);
+    c->timing = timing;
+    return r;
 }
 
 // One field evaluation of the dipoles (and charges) dq: spread -> FFT -> convolve -> FFT, the row
@@ -1191,7 +1200,9 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
                                                max_iter);
         LAUNCH_CHECK();
         launched = k + 1;
-        PGM_TRY(read_conv(launched));
+        // slot `launched` is waited on at the next pass (launched >= expect - 2) or after the loop;
+        // earlier reads would only add a copy and its gap to the queue
+        if (launched >= expect - 2 || launched == max_iter) PGM_TRY(read_conv(launched));
         if (launched >= expect - 1 || launched == max_iter) {
             // wait for the state after iteration k-1 (read slot k) while iteration k runs
             HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
@@ -2367,6 +2378,108 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
     HIP_TRY(hipEventDestroy(e0));
     HIP_TRY(hipEventDestroy(e1));
     return PGM_OK;
+}
+
+// ---------------------------------------------------------------------------- autotuner
+// Time the CG matvec's pieces for every applicable shape on this context's own system and keep the
+// fastest: the row-field layout (molecule blocks with 4, 8 or 16 lanes per molecule, atom rows) by
+// the row field alone, then the spread-tile width (the largest valid width up to 8, 7, 6, ... P-1
+// per axis; gather tiles are 2 x 2 x 2 spread tiles) by spread + merge + gather.
+template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos, int reps, double *out) {
+    using R4 = real4<real>;
+    hipStream_t st = c->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    auto time_ms = [&](auto &&launch, double &ms) -> int {
+        launch();
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) launch();
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        ms = (double)t / reps;
+        return PGM_OK;
+    };
+    auto dipoles = [&]() -> int {  // permanent dipoles as the spread / field input
+        k_perm_dipoles<<<nblk(c->n, 256), 256, 0, st>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
+                                                         c->cc.as<double>(), c->pperm.as<double>());
+        k_pack_dq<real><<<nblk(c->n, 256), 256, 0, st>>>(c->n, c->pperm.as<double>(), nullptr, nullptr, c->vp.as<R4>());
+        LAUNCH_CHECK();
+        return PGM_OK;
+    };
+    // row-field layouts
+    struct Lay {
+        int layout, lanes;
+    };
+    std::vector<Lay> lays;
+    if (c->mol3)
+        for (int l : {4, 8, 16}) lays.push_back({1, l});
+    lays.push_back({0, RL});
+    double best_row = 1e300;
+    Lay best_lay = lays.front();
+    for (const Lay &L : lays) {
+        c->layout = L.layout;
+        if (L.layout == 1) {
+            c->rlm = L.lanes;
+            PGM_TRY(alloc_mol(c, std::max(c->mcap_full, 64)));
+        }
+        c->rows_valid = false;
+        PGM_TRY(prepare(c, pos));
+        PGM_TRY(dipoles());
+        double ms = 0;
+        PGM_TRY(time_ms([&] { (void)row_field_launch<real, false>(c, c->vp.as<R4>(), nullptr, -1, st); }, ms));
+        if (ms < best_row) best_row = ms, best_lay = L;
+    }
+    c->layout = best_lay.layout;
+    if (best_lay.layout == 1) {
+        c->rlm = best_lay.lanes;
+        PGM_TRY(alloc_mol(c, std::max(c->mcap_full, 64)));
+    }
+    c->rows_valid = false;
+    // spread-tile widths
+    double best_pme = 1e300;
+    int best_cap = c->tile_cap;
+    if (c->tiled && !c->ewald) {
+        std::vector<int> tried;  // T8 triples already timed (a lower cap can give the same tiles)
+        for (int cap = 8; cap >= c->s.order - 1; --cap) {
+            c->tile_cap = cap;
+            PGM_TRY(setup_tiles(c));
+            if (!c->tiled) continue;
+            const int key = (c->tiles8.T[0] * 16 + c->tiles8.T[1]) * 16 + c->tiles8.T[2];
+            if (std::find(tried.begin(), tried.end(), key) != tried.end()) continue;
+            tried.push_back(key);
+            PGM_TRY(prepare(c, pos));
+            PGM_TRY(dipoles());
+            double ms = 0;
+            PGM_TRY(time_ms([&] {
+                (void)spread_fft_only<real>(c, c->vp.as<R4>());
+                (void)gather_combine<real>(c, c->vp.as<R4>(), MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1);
+            }, ms));
+            if (ms < best_pme) best_pme = ms, best_cap = cap;
+        }
+        c->tile_cap = best_cap;
+        PGM_TRY(setup_tiles(c));
+    }
+    c->rows_valid = false;
+    PGM_TRY(prepare(c, pos));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    out[0] = c->layout;
+    out[1] = c->layout == 1 ? c->rlm : RL;
+    for (int a = 0; a < 3; ++a) out[2 + a] = c->tiled ? c->tiles8.T[a] : 0;
+    out[5] = best_row;
+    out[6] = best_pme < 1e300 ? best_pme : 0.0;
+    out[7] = (double)lays.size();
+    return PGM_OK;
+}
+
+extern "C" int pgm_autotune(pgm_ctx *c, const double *pos, int reps, double *report) {
+    if (!c || !pos || !report || reps <= 0) return PGM_ERR_ARG;
+    for (int k = 0; k < 8; ++k) report[k] = 0.0;
+    return c->mixed ? autotune_impl<float>(c, pos, reps, report) : autotune_impl<double>(c, pos, reps, report);
 }
 
 extern "C" int pgm_time_kernels(pgm_ctx *c, const double *pos, int reps, double *ms) {

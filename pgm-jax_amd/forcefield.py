@@ -96,6 +96,15 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
     return ctx, stream
 
 
+def _autotune(ctx, pos, reps: int) -> dict:
+    """pgm_autotune on a context; return the chosen shapes and their timings."""
+    rep = (C.c_double * 8)()
+    _lib.check(_lib.lib().pgm_autotune(ctx, C.c_void_p(pos.data_ptr()), int(reps), rep), "pgm_autotune")
+    return {"row_layout": ("atom", "molecule")[int(rep[0])], "row_lanes": int(rep[1]),
+            "spread_tile": tuple(int(x) for x in rep[2:5]), "row_field_ms": rep[5], "pme_ms": rep[6],
+            "layouts_tried": int(rep[7])}
+
+
 class PGMForceField:
     """The pGM + LJ force field of one system on the GPU (forcefield.py:810-1000)."""
 
@@ -325,6 +334,12 @@ class PGMForceField:
         code = {"atom": 0, "molecule": 1}[layout]
         _lib.check(_lib.lib().pgm_set_row_layout(self._ctx, code, int(lanes)), "pgm_set_row_layout")
         self._built_at = None
+
+    def autotune(self, pos, reps: int = 20) -> dict:
+        """Time every row-field layout and PME spread-tile width on this system at pos and keep the fastest."""
+        pos = _dev(pos, (self.n, 3))
+        self._built_at = None
+        return _autotune(self._ctx, pos, reps)
 
     def row_layout(self) -> tuple[str, int]:
         """Return the CG row-field layout and its lanes per row."""

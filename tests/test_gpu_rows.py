@@ -69,3 +69,23 @@ def test_atom_layout_only_for_other_molecules():
     assert ff.row_layout()[0] == "atom"
     with pytest.raises(P.PGMError):
         ff.set_row_layout("molecule", 4)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_autotune_keeps_the_results(precision):
+    """The autotuner only picks among shapes of the same sums: energies and iterations stay (to rounding)."""
+    import pgm_jax_amd as P
+
+    top, pos, H = systems.water_box(512, 2.483, seed=6)
+    print(f"autotune precision={precision}", flush=True)
+    ff = P.PGMForceField(_system(top), H, P.MDSettings(precision=precision, **CFG1))
+    r0 = ff.compute(pos, H)
+    rep = ff.autotune(pos, reps=5)
+    print("autotune:", rep, flush=True)
+    assert rep["layouts_tried"] == 4 and rep["row_layout"] in ("atom", "molecule")
+    assert ff.row_layout() == (rep["row_layout"], rep["row_lanes"])
+    assert all(5 <= t <= 8 for t in rep["spread_tile"]) and rep["row_field_ms"] > 0 and rep["pme_ms"] > 0
+    r1 = ff.compute(pos, H)
+    tol = 1e-11 if precision == "double" else 1e-6
+    assert abs(r1.energy["total"] - r0.energy["total"]) <= tol * abs(r0.energy["total"])
+    assert abs(r1.iterations - r0.iterations) <= (0 if precision == "double" else 1)
