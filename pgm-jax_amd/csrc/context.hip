@@ -132,6 +132,7 @@ struct pgm_ctx {
     // CG row field layout: atom rows in sliced ELL (jell, G1, G2; RL lanes per atom) or, for 3-site
     // molecules, molecule blocks (mj, mg; rlm lanes per molecule) from the candidate molecules mrows
     bool mol3 = false;            // every bead is consecutive 3-atom molecules
+    bool atom_rows_fresh = false; // rows / count / count_near compacted for the current geometry
     int layout = 0;               // 0 atom rows, 1 molecule blocks
     int rlm = 4;
     int mcap_full = 0, mcap = 0;
@@ -604,9 +605,9 @@ static int build_rows(pgm_ctx *c, const double *pos) {
     const double rl = c->s.cutoff + c->s.skin;
     for (int attempt = 0; attempt < 3; ++attempt) {
         k_fill_int<<<1, 64, 0, st>>>(1, c->maxcount.as<int>(), 0);
-        k_build_rows<<<nblk(n, 128), 128, 0, st>>>(n, c->nper, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
-                                                   c->cell_end.as<int>(), rl * rl, c->cap_full, c->rows_full.as<int>(),
-                                                   c->count_full.as<int>(), c->maxcount.as<int>());
+        k_build_rows_cells<<<ncell, 256, 0, st>>>(ncell / c->nbead, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
+                                                  c->cell_end.as<int>(), rl * rl, c->cap_full, c->rows_full.as<int>(),
+                                                  c->count_full.as<int>(), c->maxcount.as<int>());
         LAUNCH_CHECK();
         HIP_TRY(hipMemcpyAsync(c->h_pin_int, c->maxcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -625,17 +626,29 @@ static int build_rows(pgm_ctx *c, const double *pos) {
     return PGM_ERR_STATE;
 }
 
-// Pack positions and compact the rows to the cutoff (every force evaluation).
+// Atom rows compacted to the cutoff (rows, count, count_near) for the current geometry (xr packed).
+static int compact_atom_rows(pgm_ctx *c) {
+    const int n = c->n;
+    k_compact_rows<<<nblk(n, 4), 256, 0, c->stream>>>(n, c->xr.as<double4>(), c->box, c->rows_full.as<int>(),
+                                                      c->count_full.as<int>(), c->cap_full, c->s.cutoff * c->s.cutoff,
+                                                      c->s.ewald_beta * c->s.ewald_beta, c->cap, c->rows.as<int>(),
+                                                      c->count.as<int>(), c->count_near.as<int>());
+    LAUNCH_CHECK();
+    c->atom_rows_fresh = true;
+    return PGM_OK;
+}
+
+// Pack positions and compact the rows to the cutoff (every force evaluation).  With molecule
+// blocks the records are the compacted form; the atom rows are compacted only when asked for.
 static int prepare(pgm_ctx *c, const double *pos) {
     if (!c->rows_valid) PGM_TRY(build_rows(c, pos));
     c->pos = pos;
     const int n = c->n;
     hipStream_t st = c->stream;
     k_pack_xr<<<nblk(n, 256), 256, 0, st>>>(n, pos, c->R2.as<double>(), c->xr.as<double4>());
-    k_compact_rows<<<nblk(n, 4), 256, 0, st>>>(n, c->xr.as<double4>(), c->box, c->rows_full.as<int>(), c->count_full.as<int>(),
-                                               c->cap_full, c->s.cutoff * c->s.cutoff, c->s.ewald_beta * c->s.ewald_beta, c->cap,
-                                               c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>());
     LAUNCH_CHECK();
+    c->atom_rows_fresh = false;
+    if (c->layout != 1) PGM_TRY(compact_atom_rows(c));
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->s.cutoff * c->s.cutoff;
@@ -1272,7 +1285,8 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                                             c->alpha64.as<double>(), c->dd.as<double>(), dq,
                                                             c->pack_part.as<double>());
     LAUNCH_CHECK();
-    const int npair = nblk((long long)nper * ROWL, 256);  // pair partials per bead
+    if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));  // the force pass runs on the atom rows
+    const int npair = nblk((long long)nper * ROWL, 256);       // pair partials per bead
     const dim3 gpair(npair, B);
     const int nconv = c->ewald ? nblk(c->nk, EW_TILE) : nblk((long long)c->ncplx, 256);
     const bool vir = strain != nullptr;
@@ -1658,6 +1672,7 @@ extern "C" int pgm_build_rows(pgm_ctx *c, const double *pos) {
 
 extern "C" long long pgm_row_entries(pgm_ctx *c) {
     if (!c || !c->rows_valid) return -1;
+    if (!c->atom_rows_fresh && compact_atom_rows(c) < 0) return -1;
     std::vector<int> cnt(c->n);
     if (hipMemcpyAsync(cnt.data(), c->count.p, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;

@@ -104,6 +104,83 @@ __global__ void k_build_rows(int n, int nper, const double *__restrict__ pos, Bo
     atomicMax(maxcount, cnt);
 }
 
+// The same rows built cell by cell: one workgroup per cell (bead * ncell + cell, atoms in sorted
+// order), its atoms the owners (one thread each, in chunks of blockDim), the neighbouring cells'
+// positions staged through LDS in chunks of blockDim and tested by every owner.  Cells and atoms are
+// visited in k_build_rows's order, so the rows are identical; each position is read once per
+// workgroup instead of once per owner.
+__global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const double *__restrict__ pos, Box box, int3 nc,
+                                                          const int *__restrict__ satom, const int *__restrict__ cstart,
+                                                          const int *__restrict__ cend, double rc2, int cap,
+                                                          int *__restrict__ rows, int *__restrict__ count,
+                                                          int *__restrict__ maxcount) {
+    __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int sj[256];
+    const int cell = blockIdx.x;
+    const int bead = cell / ncell_bead, lc = cell - bead * ncell_bead;
+    const int own0 = cstart[cell], own1 = cend[cell];
+    if (own1 <= own0) return;  // empty cell (uniform)
+    const int ncs[3] = {nc.x, nc.y, nc.z};
+    const int c[3] = {lc / (nc.y * nc.z), (lc / nc.z) % nc.y, lc % nc.z};
+    int lo[3], hiO[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = ncs[a] >= 3 ? -1 : 0;
+        hiO[a] = ncs[a] >= 3 ? 1 : ncs[a] - 1;
+    }
+    for (int o0s = own0; o0s < own1; o0s += blockDim.x) {
+        const int s = o0s + threadIdx.x;
+        const bool owner = s < own1;
+        const int i = owner ? satom[s] : 0;
+        double xi = 0.0, yi = 0.0, zi = 0.0;
+        if (owner) {
+            xi = pos[3 * i];
+            yi = pos[3 * i + 1];
+            zi = pos[3 * i + 2];
+        }
+        int cnt = 0;
+        int *row = rows + (size_t)i * cap;
+        for (int q0 = lo[0]; q0 <= hiO[0]; ++q0) {
+            const int c0 = (c[0] + q0 + ncs[0]) % ncs[0];
+            for (int q1 = lo[1]; q1 <= hiO[1]; ++q1) {
+                const int c1 = (c[1] + q1 + ncs[1]) % ncs[1];
+                for (int q2 = lo[2]; q2 <= hiO[2]; ++q2) {
+                    const int c2 = (c[2] + q2 + ncs[2]) % ncs[2];
+                    const int nb = bead * ncell_bead + (c0 * nc.y + c1) * nc.z + c2;
+                    const int s0 = cstart[nb], s1 = cend[nb];
+                    for (int t0 = s0; t0 < s1; t0 += blockDim.x) {
+                        const int m = min((int)blockDim.x, s1 - t0);
+                        __syncthreads();
+                        if ((int)threadIdx.x < m) {
+                            const int j = satom[t0 + threadIdx.x];
+                            sj[threadIdx.x] = j;
+                            sx[threadIdx.x] = pos[3 * j];
+                            sy[threadIdx.x] = pos[3 * j + 1];
+                            sz[threadIdx.x] = pos[3 * j + 2];
+                        }
+                        __syncthreads();
+                        if (owner)
+                            for (int u = 0; u < m; ++u) {
+                                const int j = sj[u];
+                                if (j == i) continue;
+                                double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
+                                min_image(dx, dy, dz, box);
+                                if (dx * dx + dy * dy + dz * dz < rc2) {
+                                    if (cnt < cap) row[cnt] = j;
+                                    ++cnt;
+                                }
+                            }
+                    }
+                }
+            }
+        }
+        if (owner) {
+            count[i] = cnt;
+            atomicMax(maxcount, cnt);
+        }
+    }
+}
+
 // Compacted rows within rc: one wave per atom, ballot + prefix count, one pass.  "Near" partners
 // (those for which a series branch of the radial kernels may be taken: (a r)^2 or (beta r)^2
 // below 1.05, FP64) fill the row from the front, the others from the back (row[cap - 1 - k]);
