@@ -143,7 +143,7 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
         "launches": live["row_field"]["launches"],
         "row_entries": entries,
         "row_blocks": blocks,
-        "spread": {"kernels": "k_spread_tiles + k_merge_blocks", "achieved": round(gbs(spread_bytes, spread_ms), 2),
+        "spread": {"kernels": "k_spread_tiles + k_merge_table", "achieved": round(gbs(spread_bytes, spread_ms), 2),
                    "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
                    "ms_per_launch": round(spread_ms, 5)},
         "gather": {"kernels": "k_gather_tiles", "achieved": round(gbs(gather_bytes, gather_ms), 2),

@@ -156,7 +156,7 @@ struct pgm_ctx {
     bool tiled = false;
     int tile_cap = 8;  // largest spread-tile width tried per axis (pgm_autotune lowers it)
     PMETiles tiles{}, tiles8{};
-    DevBuf pkey, pidx, pskey, psidx, tstart, tend, tstart16, tend16, spl, loc, loc16, blocks, pme_sort_tmp;
+    DevBuf pkey, pidx, pskey, psidx, tstart, tend, tstart16, tend16, spl, loc, loc16, blocks, pme_sort_tmp, merge_tab;
     // vectors (real4 N) and FP64 (N, 3) work arrays
     DevBuf vx, vr, vz, vp, vAp, vb, vx0, vrow, vdq, vtmp;
     DevBuf pperm, dd, g_el, g_lj, rowf, rec_dr, gp;
@@ -450,6 +450,31 @@ static int setup_tiles(pgm_ctx *c) {
     PGM_TRY(c->tend16.ensure(sizeof(int) * B * t16.ntiles));
     PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
     PGM_TRY(c->blocks.ensure(c->rsize * B * t8.ntiles * t8.bsize));
+    {  // per-axis offsets of the block merge (k_merge_table)
+        std::vector<int> tab;
+        for (int a = 0; a < 3; ++a) {
+            const int K = c->s.grid[a], T = t8.T[a], nt = t8.nt[a];
+            const int stride = a == 0 ? t8.B[1] * t8.B[2] : (a == 1 ? t8.B[2] : 1);
+            auto A = [&](int u) {
+                const int hi = u >> 1, lo = u & 1;
+                if (a == 0) return hi * t16.nt[1] * t16.nt[2] * 8 + lo * 4;
+                if (a == 1) return hi * t16.nt[2] * 8 + lo * 2;
+                return hi * 8 + lo;
+            };
+            for (int g = 0; g < K; ++g) {
+                const int u = std::min(g / T, nt - 1), l = g - u * T;
+                tab.push_back(A(u) * t8.bsize + l * stride);
+                if (l <= P - 2) {
+                    const int up = u > 0 ? u - 1 : nt - 1;
+                    const int width = up < nt - 1 ? T : K - (nt - 1) * T;
+                    tab.push_back(A(up) * t8.bsize + (l + width) * stride);
+                } else {
+                    tab.push_back(-1);
+                }
+            }
+        }
+        PGM_TRY(upload(c->merge_tab, tab.data(), tab.size(), c->stream));
+    }
     // gather partials come per gather tile: the partial buffers must hold B ntiles entries
     for (DevBuf *b : {&c->partA, &c->partB}) PGM_TRY(b->ensure(std::max(b->bytes, sizeof(double) * 4 * B * t16.ntiles)));
     // dynamic LDS of the tiled spread and gather beyond the default limit
@@ -739,8 +764,10 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
         LAUNCH_CHECK();
-        ORDER_SWITCH(P, (k_merge_blocks<real, PO><<<nblk((long long)B * c->ngrid, 256), 256, 0, st>>>(
-                            B, t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), conv, iter)));
+        const int *K = c->s.grid;
+        k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0] * B), 128, 0, st>>>(
+            K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(), conv,
+            iter);
     } else {
     k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
     if (withq) {
@@ -1638,7 +1665,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->mrows, &c->mcount_full, &c->mcount, &c->mj, &c->mg,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
-                     &c->loc16, &c->blocks,
+                     &c->loc16, &c->blocks, &c->merge_tab,
                      &c->pme_sort_tmp, &c->stats, &c->mol_ptr, &c->mol_atoms, &c->com, &c->off, &c->rec_dd,
                      &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk,
                      &c->guess.params, &c->guess.grads, &c->guess.m, &c->guess.v, &c->guess.packed, &c->guess.bias,
@@ -2358,10 +2385,12 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
                                          c->loc.as<int>(), dq, c->blocks.as<real>(), nullptr, -1);
                                  }));
         if (rc < 0) return rc;
-        ORDER_SWITCH(c->s.order, rc = timed(0, [&] {
-                                     k_merge_blocks<real, PO><<<nblk((long long)c->nbead * c->ngrid, 256), 256, 0, st>>>(
-                                         c->nbead, t, c->tiles, c->pg, c->blocks.as<real>(), c->grid.as<real>(), nullptr, -1);
-                                 }));
+        const int *K = c->s.grid;
+        rc = timed(0, [&] {
+            k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0] * c->nbead), 128, 0, st>>>(
+                K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(),
+                nullptr, -1);
+        });
         if (rc < 0) return rc;
     } else {
         rc = timed(0, [&] { k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0)); });

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_spread(int n, const double *__restrict_
 // also stores the spline weights in sorted order).  One wave per tile accumulates its atoms' full
 // P^3 stencils into an LDS block of (T+P-1)^3 points: one atom at a time, the 64 lanes take 64
 // distinct stencil points, so plain LDS read-add-write is race free and the order is fixed.  The
-// blocks are written out and k_merge_blocks sums, for every grid point, the (up to 8) blocks that
+// blocks are written out and k_merge_table sums, for every grid point, the (up to 8) blocks that
 // cover it in a fixed order.
 struct PMETiles {
     int T[3];    // tile width per axis (the last tile may be narrower, >= P-1)
@@ -101,10 +101,6 @@ struct PMETiles {
     int ntiles;  // nt0 nt1 nt2
     int bsize;   // B0 B1 B2
 };
-
-PGM_DEV int tile_size(const PMETiles &t, int a, int u, const PMEGeom &g) {
-    return u < t.nt[a] - 1 ? t.T[a] : g.K[a] - (t.nt[a] - 1) * t.T[a];
-}
 
 // The spread works on tiles of T8 points per axis, the gather on tiles of T16 = 2 T8 (each the
 // union of 2 x 2 x 2 spread tiles).  Atoms are sorted once per geometry by the fine key
@@ -177,12 +173,12 @@ __global__ void k_key_bounds(int n, const int *__restrict__ skey, int shift, int
 
 // Tiled spread: one wave per spread tile (fine key; block b * t.ntiles + key for bead b) accumulates
 // the stencils of its atoms into an LDS block of (T8 + P - 1)^3 points, written to `blocks` (merged
-// by k_merge_blocks).  The atoms'
+// by k_merge_table).  The atoms'
 // spline weights, scaled dipoles, charges and local offsets are staged through LDS in chunks of
 // SPREAD_CHUNK (one parallel load instead of a chain of dependent loads per atom); the 64 lanes
 // cover the P^3 stencil points of one atom at a time, so no two lanes touch one point and the sum
 // is free of atomics and bitwise reproducible.
-constexpr int SPREAD_CHUNK = 64;
+constexpr int SPREAD_CHUNK = 16;  // atoms staged at a time (small: the LDS per wave bounds the occupancy)
 
 template <typename real, int P>
 PGM_DEV constexpr int spread_lds_atoms() {
@@ -245,48 +241,36 @@ __global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, cons
     for (int k = lane; k < t.bsize; k += WAVE) out[k] = blk[k];
 }
 
-// Grid point value = sum of the spread blocks covering it: along each axis its own spread tile
-// (local l) and, when l <= P-2, the previous one (local l + width of that tile); fixed order
-// (deterministic).  Blocks are stored by fine key, bead after bead; grid holds nbead grids.
-template <typename real, int P>
-__global__ __launch_bounds__(256) void k_merge_blocks(int nbead, PMETiles t, PMETiles t16, PMEGeom g,
-                                                      const real *__restrict__ blocks, real *__restrict__ grid,
-                                                      const int *__restrict__ conv_it, int iter) {
-    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
-    const size_t ng = (size_t)K1 * K2 * K3;
-    if (gidx >= ng * nbead) return;
-    const int bead = (int)(gidx / ng);
-    if (cg_done(conv_it, bead, iter)) return;
-    const size_t idx = gidx - (size_t)bead * ng;
-    const size_t kb = (size_t)bead * t.ntiles;
-    const int gz = (int)(idx % K3), gy = (int)((idx / K3) % K2), gx = (int)(idx / ((size_t)K3 * K2));
-    const int gg[3] = {gx, gy, gz};
-    int u[3][2], l[3][2], nu[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int tt = gg[a] / t.T[a];
-        tt = tt < t.nt[a] ? tt : t.nt[a] - 1;
-        const int la = gg[a] - tt * t.T[a];
-        u[a][0] = tt;
-        l[a][0] = la;
-        nu[a] = 1;
-        if (la <= P - 2) {
-            const int tp = tt > 0 ? tt - 1 : t.nt[a] - 1;
-            u[a][1] = tp;
-            l[a][1] = la + tile_size(t, a, tp, g);
-            nu[a] = 2;
-        }
-    }
+// Block merge: grid point value = sum of the spread blocks covering it -- along each axis its own
+// spread tile (local l) and, when l <= P-2, the previous one (local l + width of that tile), in a
+// fixed order (deterministic).  Table driven: for grid coordinate g along axis a, tab_a[2 g] is the offset (in reals) of the
+// point in the block of its own spread tile's axis part -- A_a(u) bsize + l stride_a with
+// A_0(u) = ((u >> 1) nt16_1 nt16_2 8 + (u & 1) 4), A_1(u) = ((u >> 1) nt16_2 8 + (u & 1) 2),
+// A_2(u) = ((u >> 1) 8 + (u & 1)), strides B1 B2, B2, 1 -- and tab_a[2 g + 1] the same for the
+// previous tile (local l + its width) when l <= P-2, else -1.  Both the fine key and the in-block
+// offset are sums over the axes, so a grid point's up to 8 block values are
+// blocks[bead base + X + Y + Z] over the axes' candidates, summed in a fixed order.
+// Launch: x = z-line (blockDim.x points), y = K2 rows, z = K1 planes times beads.
+template <typename real>
+__global__ __launch_bounds__(128) void k_merge_table(int K1, int K2, int K3, size_t bead_blocks, const int *__restrict__ tab,
+                                                     const real *__restrict__ blocks, real *__restrict__ grid,
+                                                     const int *__restrict__ conv_it, int iter) {
+    const int gz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    const int gx = blockIdx.z % K1, bead = blockIdx.z / K1;
+    if (gz >= K3 || cg_done(conv_it, bead, iter)) return;
+    const int *tx = tab, *ty = tab + 2 * K1, *tz = tab + 2 * (K1 + K2);
+    const int X[2] = {tx[2 * gx], tx[2 * gx + 1]}, Y[2] = {ty[2 * gy], ty[2 * gy + 1]}, Z[2] = {tz[2 * gz], tz[2 * gz + 1]};
+    const real *b = blocks + bead_blocks * bead;
     real acc = real(0);
-    for (int i0 = 0; i0 < nu[0]; ++i0)
-        for (int i1 = 0; i1 < nu[1]; ++i1)
-            for (int i2 = 0; i2 < nu[2]; ++i2) {
-                const int uu[3] = {u[0][i0], u[1][i1], u[2][i2]};
-                const size_t key = kb + fine_key(uu, t16);
-                acc += blocks[key * t.bsize + ((size_t)l[0][i0] * t.B[1] + l[1][i1]) * t.B[2] + l[2][i2]];
-            }
-    grid[gidx] = acc;
+#pragma unroll
+    for (int i0 = 0; i0 < 2; ++i0)
+#pragma unroll
+        for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+            for (int i2 = 0; i2 < 2; ++i2)
+                if (X[i0] >= 0 && Y[i1] >= 0 && Z[i2] >= 0) acc += b[(size_t)X[i0] + Y[i1] + Z[i2]];
+    grid[(((size_t)bead * K1 + gx) * K2 + gy) * K3 + gz] = acc;
 }
 
 // Convolution on the half spectrum: F[Q] *= Gc; with ENERGY also sum (w3/2) Gc |F[Q]|^2 per block
