@@ -11,8 +11,9 @@ rescaled to 298 K every 10 steps); with the learned guess (--guess learned, the 
 equilibration also harvests converged dipoles and trains the network.
 
 `value` is the mixed-precision path (FP32 pair/PME kernels, FP64 accumulation, FP64 CG reductions)
-with the learned CG initial guess; beside it: the same mixed path from the previous-step guess,
-the plain FP64 path from the previous-step guess ("fp64", the baseline of the metric), the energy /
+with the learned CG initial guess and the molecule-block preconditioner (--precond; "jacobi" is the
+reference's); beside it: the same mixed path from the previous-step guess, the plain FP64 path
+(Jacobi, previous-step guess: the reference's algorithm; "fp64", the baseline of the metric), the energy /
 force / dipole errors of mixed vs FP64 at the final configuration, the HBM roofline of the
 dominant kernel (the CG matvec's row field; HIP events around every launch of the timed steps)
 with the PME spread / gather beside it, and the float64 numpy oracle on the host (cpu_baseline).
@@ -230,6 +231,8 @@ def main() -> int:  # noqa: C901
     ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
     ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
     ap.add_argument("--bead-evals", type=int, default=10, help="timed force evaluations of the 32-bead leg (0: off)")
+    ap.add_argument("--precond", default="molecule", choices=["jacobi", "molecule"],
+                    help="CG preconditioner of the mixed paths (the FP64 baseline keeps the reference's Jacobi)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the library's copy kernel 3 times over MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -265,7 +268,7 @@ def main() -> int:  # noqa: C901
     # Equilibrate once (mixed precision, velocity rescaling to 298 K) from the lattice start.  With the
     # learned guess, this run harvests (features, converged dipoles) every 10 solves and trains the
     # network between blocks of steps (untimed), so the timed run starts with a trained guess.
-    ffe = P.PGMForceField(sys_, H, P.MDSettings(precision="mixed", predictor=a.guess, **base))
+    ffe = P.PGMForceField(sys_, H, P.MDSettings(precision="mixed", predictor=a.guess, precond=a.precond, **base))
     if a.guess == "learned":
         ffe.guess_config(hidden_layers=3, harvest_every=10, seed=11)
     mde = P.RigidWaterMD(ffe, pos, vel)
@@ -284,9 +287,9 @@ def main() -> int:  # noqa: C901
     vel = mde.vel.cpu().numpy()
 
     def run(precision: str, steps: int, warmup: int, live_timing: bool = False, predictor: str = "previous",
-            ff=None, md=None):
+            ff=None, md=None, precond: str = "jacobi"):
         if ff is None:
-            ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, predictor=predictor, **base))
+            ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, predictor=predictor, precond=precond, **base))
             md = P.RigidWaterMD(ff, pos, vel)
         tune = ff.autotune(md.pos, reps=10)  # row-field layout and spread tiles for this context (untimed)
         log(precision, predictor, "autotune", json.dumps(tune))
@@ -321,7 +324,7 @@ def main() -> int:  # noqa: C901
     log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}), "row entries", entries)
     prev = None
     if a.guess != "previous" and a.prev_steps > 0:  # the same mixed path from the previous-step guess
-        _, _, repp, ms_p = run("mixed", a.prev_steps, max(2, a.warmup // 4), predictor="previous")
+        _, _, repp, ms_p = run("mixed", a.prev_steps, max(2, a.warmup // 4), predictor="previous", precond=a.precond)
         prev = {"ms_per_step": round(ms_p, 4), "value": round(ns_per_day(ms_p, a.dt_fs), 3), "steps": a.prev_steps,
                 "cg_iters_per_step": round(repp["cg_iterations"] / a.prev_steps, 3)}
     errors, fp64 = None, None
@@ -365,10 +368,11 @@ def main() -> int:  # noqa: C901
         "data": "synthetic",
         "config": {
             "workload": (f"{a.waters} rigid pGM3P-25 waters ({n} atoms), {a.edge} nm cubic box, qpi+LJ(lrc), rc 0.9 nm, "
-                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the {a.guess} guess, NVE "
+                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the {a.guess} guess ({a.precond} "
+                         f"preconditioner), NVE "
                          f"dt {a.dt_fs} fs, FP32 kernels with FP64 accumulation; baseline: the FP64 path from the "
                          f"previous-step guess"),
-            "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": a.guess,
+            "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": a.guess, "precond": a.precond,
         },
         "cg_iters_per_step": round(repm["cg_iterations"] / a.steps, 3),
         "autotune": repm["autotune"],

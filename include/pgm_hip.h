@@ -66,6 +66,8 @@ typedef struct {
     int recip;          /* reciprocal space: 0 smooth PME, 1 exact Ewald sum (PeriodicPGM, ewald.py:186-470) */
     double k_tol;       /* Ewald: k-vectors with exp(-k^2/(4 beta^2)) >= k_tol, fixed at the creation box */
     int nbead;          /* beads (0 or 1: one system): copies of the topology advanced together (PGMBeads) */
+    int precond;        /* CG preconditioner: 0 Jacobi z = alpha r (the reference's), 1 the inverse 9x9 block of
+                           each 3-site molecule (intramolecular coupling; same convergence test) */
 } pgm_settings;
 
 /* Induction state (pgm_jax/md/forcefield.py:679-713 InductionState), caller-owned:

@@ -95,6 +95,7 @@ class Settings(C.Structure):
         ("recip", C.c_int),
         ("k_tol", C.c_double),
         ("nbead", C.c_int),
+        ("precond", C.c_int),
     ]
 
 

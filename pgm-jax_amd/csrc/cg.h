@@ -266,6 +266,172 @@ __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__re
     }
 }
 
+// ---------------------------------------------------------------------------- molecule-block preconditioner
+// Optional (3-site molecules): z = M_I^-1 r over the nine components of molecule I, with
+// M_I = diag(1/alpha_p) + the dipole tensors B1 I - B2 x x^T (x = r_p - r_k, minimum image) between
+// its atoms for the full Gaussian kernel erf(a r)/r -- the direct plus reciprocal intramolecular
+// coupling of the operator, periodic images left out.  The CG is the same flexible PCG with another
+// symmetric positive definite preconditioner; the convergence test stays max|alpha r| / norm.
+// M_I^-1 (81 reals, row major) per molecule by Gauss-Jordan in FP64, remade with the neighbour rows
+// (the molecules turn little in between, and any such blocks are a valid preconditioner).
+template <typename real>
+__global__ void k_mol_precond(int nmol, const double4 *__restrict__ xr, const double *__restrict__ alpha, Box box,
+                              real *__restrict__ minv) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nmol) return;
+    double M[9][18];
+    for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 18; ++c) M[r][c] = (c == r + 9) ? 1.0 : 0.0;
+    for (int p = 0; p < 3; ++p)
+        for (int c = 0; c < 3; ++c) M[3 * p + c][3 * p + c] = 1.0 / alpha[3 * I + p];
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 3; ++k) {
+            if (p == k) continue;
+            const double4 a = xr[3 * I + p], b = xr[3 * I + k];
+            double x[3] = {a.x - b.x, a.y - b.y, a.z - b.z};
+            min_image(x[0], x[1], x[2], box);
+            const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+            const double ir = 1.0 / sqrt(r2), r = r2 * ir;
+            double B[3];
+            one_B<double, 3>(1.0 / sqrt(2.0 * (a.w + b.w)), r, ir, ir * ir, B);
+            for (int u = 0; u < 3; ++u)
+                for (int v = 0; v < 3; ++v) M[3 * p + u][3 * k + v] = (u == v ? B[1] : 0.0) - B[2] * x[u] * x[v];
+        }
+    for (int col = 0; col < 9; ++col) {  // Gauss-Jordan (M is symmetric positive definite: no pivoting)
+        const double d = 1.0 / M[col][col];
+        for (int c = 0; c < 18; ++c) M[col][c] *= d;
+        for (int r = 0; r < 9; ++r) {
+            if (r == col) continue;
+            const double f = M[r][col];
+            if (f != 0.0)
+                for (int c = 0; c < 18; ++c) M[r][c] -= f * M[col][c];
+        }
+    }
+    real *out = minv + (size_t)81 * I;
+    for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 9; ++c) out[9 * r + c] = (real)M[r][9 + c];
+}
+
+// z of atom p (0..2) of the molecule whose nine residual components are sr[0..8]: rows 3p..3p+2 of M^-1.
+template <typename real>
+PGM_DEV real4<real> mol_apply(const real *__restrict__ Mi, int p, const real *sr) {
+    real v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        real s = real(0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) s += Mi[9 * (3 * p + c) + q] * sr[q];
+        v[c] = s;
+    }
+    real4<real> z;
+    z.x = v[0];
+    z.y = v[1];
+    z.z = v[2];
+    z.w = 0;
+    return z;
+}
+
+constexpr int MOLP_BLOCK = 192;  // 64 whole molecules per workgroup
+
+// k_cg_start with the molecule blocks: z = M^-1 r, p = z, partials (r . z, max |alpha r|).
+template <typename real>
+__global__ __launch_bounds__(MOLP_BLOCK) void k_cg_start_mol(int nper, const real4<real> *__restrict__ r,
+                                                             const real *__restrict__ alpha, const real *__restrict__ minv,
+                                                             real4<real> *__restrict__ z, real4<real> *__restrict__ p,
+                                                             double *__restrict__ part) {
+    __shared__ real sr[3 * MOLP_BLOCK];
+    __shared__ double lds[32];
+    const int li = blockIdx.x * MOLP_BLOCK + threadIdx.x;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
+    const bool in = li < nper;
+    real4<real> rr;
+    rr.x = rr.y = rr.z = rr.w = real(0);
+    if (in) rr = r[i];
+    sr[3 * threadIdx.x] = rr.x;
+    sr[3 * threadIdx.x + 1] = rr.y;
+    sr[3 * threadIdx.x + 2] = rr.z;
+    __syncthreads();
+    double s[1] = {0.0};
+    double m = 0.0;
+    if (in) {
+        const int q = threadIdx.x / 3, pa = threadIdx.x - 3 * q;
+        const real4<real> zz = mol_apply<real>(minv + (size_t)81 * (i / 3), pa, sr + 9 * q);
+        z[i] = zz;
+        p[i] = zz;
+        const real a = alpha[i];
+        s[0] = (double)(rr.x * zz.x) + (double)(rr.y * zz.y) + (double)(rr.z * zz.z);
+        m = fmax(fmax(fabs((double)(a * rr.x)), fabs((double)(a * rr.y))), fabs((double)(a * rr.z)));
+    }
+    block_sum<1>(s, lds);
+    m = block_max(m, lds + 16);
+    if (threadIdx.x == 0) {
+        part[2 * pb] = s[0];
+        part[2 * pb + 1] = m;
+    }
+}
+
+// k_cg_update1 with the molecule blocks: x += al p, r' = r - al Ap, z' = M^-1 r', partials
+// (z' . (r' - r), r' . z', max |alpha r'|).
+template <typename real>
+__global__ __launch_bounds__(MOLP_BLOCK) void k_cg_update1_mol(int nper, const double *__restrict__ pap_part, int np,
+                                                               const double *__restrict__ rz, int rzs, int k,
+                                                               real4<real> *__restrict__ x, real4<real> *__restrict__ r,
+                                                               real4<real> *__restrict__ z, const real4<real> *__restrict__ p,
+                                                               const real4<real> *__restrict__ Ap, const real *__restrict__ alpha,
+                                                               const real *__restrict__ minv, double *__restrict__ part,
+                                                               const CGState *__restrict__ st) {
+    const int b = blockIdx.y;
+    if (k >= st[b].conv_it) return;
+    __shared__ real sr[3 * MOLP_BLOCK];
+    __shared__ double lds[32];
+    const double pap = block_reduce_partials(pap_part + (size_t)b * np, np, 1, 0, lds, false);
+    const real al = (real)(rz[(size_t)b * rzs + k] / fmax(pap, 1e-300));
+    const int li = blockIdx.x * MOLP_BLOCK + threadIdx.x;
+    const int i = b * nper + li;
+    const int pb = b * gridDim.x + blockIdx.x;
+    const bool in = li < nper;
+    real4<real> r0, r1;
+    r0.x = r0.y = r0.z = r0.w = real(0);
+    r1 = r0;
+    if (in) {
+        const real4<real> pp = p[i], ap = Ap[i];
+        r0 = r[i];
+        real4<real> xx = x[i];
+        xx.x += al * pp.x;
+        xx.y += al * pp.y;
+        xx.z += al * pp.z;
+        x[i] = xx;
+        r1.x = r0.x - al * ap.x;
+        r1.y = r0.y - al * ap.y;
+        r1.z = r0.z - al * ap.z;
+        r1.w = 0;
+        r[i] = r1;
+    }
+    sr[3 * threadIdx.x] = r1.x;
+    sr[3 * threadIdx.x + 1] = r1.y;
+    sr[3 * threadIdx.x + 2] = r1.z;
+    __syncthreads();
+    double s[2] = {0.0, 0.0};
+    double m = 0.0;
+    if (in) {
+        const int q = threadIdx.x / 3, pa = threadIdx.x - 3 * q;
+        const real4<real> z1 = mol_apply<real>(minv + (size_t)81 * (i / 3), pa, sr + 9 * q);
+        z[i] = z1;
+        const real a = alpha[i];
+        s[0] = (double)(z1.x * (r1.x - r0.x)) + (double)(z1.y * (r1.y - r0.y)) + (double)(z1.z * (r1.z - r0.z));
+        s[1] = (double)(r1.x * z1.x) + (double)(r1.y * z1.y) + (double)(r1.z * z1.z);
+        m = fmax(fmax(fabs((double)(a * r1.x)), fabs((double)(a * r1.y))), fabs((double)(a * r1.z)));
+    }
+    block_sum<2>(s, lds);
+    m = block_max(m, lds + 16);
+    if (threadIdx.x == 0) {
+        part[3 * pb] = s[0];
+        part[3 * pb + 1] = s[1];
+        part[3 * pb + 2] = m;
+    }
+}
+
 // Per bead b: iterations it_b = min(conv_it_b, launched) and the final residual err_b[it_b]
 // (bead_it, bead_res); stats slot = the largest of the residuals (kept on the device).  One block.
 __global__ void k_record_residual(int nbead, const double *__restrict__ err, int rzs, const CGState *__restrict__ st,

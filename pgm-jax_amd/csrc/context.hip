@@ -133,6 +133,9 @@ struct pgm_ctx {
     // molecules, molecule blocks (mj, mg; rlm lanes per molecule) from the candidate molecules mrows
     bool mol3 = false;            // every bead is consecutive 3-atom molecules
     bool atom_rows_fresh = false; // rows / count / count_near compacted for the current geometry
+    bool molp = false;            // molecule-block CG preconditioner (settings.precond 1)
+    bool pinv_stale = true;       // its blocks are remade at the next evaluation after a row build
+    DevBuf pinv;                  // its inverse 9x9 blocks, 81 reals per molecule
     int layout = 0;               // 0 atom rows, 1 molecule blocks
     int rlm = 4;
     int mcap_full = 0, mcap = 0;
@@ -640,6 +643,7 @@ static int build_rows(pgm_ctx *c, const double *pos) {
         if (mc <= c->cap_full) {
             if (c->layout == 1) PGM_TRY(build_mol_rows(c));
             c->rows_valid = true;
+            c->pinv_stale = true;
             PGM_TRY(c->pos_build.ensure(sizeof(double) * 3 * (size_t)n));
             HIP_TRY(hipMemcpyAsync(c->pos_build.p, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
             c->rebuilds += 1;
@@ -674,6 +678,17 @@ static int prepare(pgm_ctx *c, const double *pos) {
     LAUNCH_CHECK();
     c->atom_rows_fresh = false;
     if (c->layout != 1) PGM_TRY(compact_atom_rows(c));
+    if (c->molp && c->pinv_stale) {  // blocks of the geometry at the row build (any SPD blocks precondition)
+        c->pinv_stale = false;
+        const int nmol = n / 3;
+        if (c->mixed)
+            k_mol_precond<float><<<nblk(nmol, 128), 128, 0, st>>>(nmol, c->xr.as<double4>(), c->alpha64.as<double>(), c->box,
+                                                                  c->pinv.as<float>());
+        else
+            k_mol_precond<double><<<nblk(nmol, 128), 128, 0, st>>>(nmol, c->xr.as<double4>(), c->alpha64.as<double>(), c->box,
+                                                                   c->pinv.as<double>());
+        LAUNCH_CHECK();
+    }
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->s.cutoff * c->s.cutoff;
@@ -1160,7 +1175,8 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     using R4 = real4<real>;
     const int nb = nblk(n, 256);
     const dim3 gB = bead_grid(c, nper);
-    const int npb = nblk(nper, 256);
+    const dim3 gM(nblk(nper, MOLP_BLOCK), c->nbead);           // molecule-block preconditioner kernels
+    const int npb = c->molp ? nblk(nper, MOLP_BLOCK) : nblk(nper, 256);  // start / update1 partials per bead
     R4 *x = c->vx.as<R4>(), *r = c->vr.as<R4>(), *z = c->vz.as<R4>(), *p = c->vp.as<R4>(), *Ap = c->vAp.as<R4>();
     R4 *b = c->vb.as<R4>(), *x0 = c->vx0.as<R4>(), *dq = c->vdq.as<R4>();
     const double *q = c->q.as<double>();
@@ -1213,7 +1229,11 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     }
     c->norm_on_device = true;
     HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
-    k_cg_start<real><<<gB, 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
+    if (c->molp)
+        k_cg_start_mol<real><<<gM, MOLP_BLOCK, 0, st>>>(nper, r, c->alpha_r.as<real>(), c->pinv.as<real>(), z, p,
+                                                         c->partB.as<double>());
+    else
+        k_cg_start<real><<<gB, 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
     k_cg_start_finish<<<B, 256, 0, st>>>(c->partB.as<double>(), npb, S, rz, err, c->rzs, c->s.dipole_tol, c->s.max_iter);
     LAUNCH_CHECK();
     const int max_iter = c->s.max_iter;
@@ -1234,8 +1254,13 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     while (launched < max_iter) {
         const int k = launched;
         PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
-        k_cg_update1<real><<<gB, 256, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
-                                               c->alpha_r.as<real>(), c->partB.as<double>(), S);
+        if (c->molp)
+            k_cg_update1_mol<real><<<gM, MOLP_BLOCK, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z,
+                                                               p, Ap, c->alpha_r.as<real>(), c->pinv.as<real>(),
+                                                               c->partB.as<double>(), S);
+        else
+            k_cg_update1<real><<<gB, 256, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
+                                                   c->alpha_r.as<real>(), c->partB.as<double>(), S);
         k_cg_update2<real><<<gB, 256, 0, st>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, c->s.dipole_tol,
                                                max_iter);
         LAUNCH_CHECK();
@@ -1630,6 +1655,20 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         c->layout = c->mol3 ? 1 : 0;
         if (c->layout == 1 && (r = alloc_mol(c, 64)) < 0) return fail(r);
     }
+    if (s->precond != 0 && s->precond != 1) {
+        g_err = "pgm_ctx_create: precond must be 0 (Jacobi) or 1 (molecule blocks)";
+        return fail(PGM_ERR_ARG);
+    }
+    if (s->precond == 1) {
+        bool m3 = nper % 3 == 0;
+        for (int i = 0; i < nper && m3; ++i) m3 = top->mol[i] == i / 3;
+        if (!m3) {
+            g_err = "pgm_ctx_create: the molecule-block preconditioner needs consecutive 3-atom molecules";
+            return fail(PGM_ERR_ARG);
+        }
+        c->molp = true;
+        if ((r = c->pinv.ensure(c->rsize * 81 * (size_t)(n / 3))) < 0) return fail(r);
+    }
     if ((r = set_box(c, H)) < 0) return fail(r);
     // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
     const double rl = s->cutoff + s->skin;
@@ -1662,7 +1701,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
                      &c->pair_part, &c->pack_part, &c->conv_part, &c->st, &c->rz, &c->err, &c->energy, &c->bead_it, &c->bead_res,
-                     &c->mrows, &c->mcount_full, &c->mcount, &c->mj, &c->mg,
+                     &c->mrows, &c->mcount_full, &c->mcount, &c->mj, &c->mg, &c->pinv,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
                      &c->loc16, &c->blocks, &c->merge_tab,

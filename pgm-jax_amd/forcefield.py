@@ -89,6 +89,7 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
     cs.recip = 1 if settings.recip == "ewald" else 0
     cs.k_tol = settings.k_tol
     cs.nbead = int(nbead)
+    cs.precond = 1 if settings.precond == "molecule" else 0
     stream = torch.cuda.current_stream()
     ctx = C.c_void_p()
     _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),

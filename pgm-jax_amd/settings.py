@@ -21,7 +21,8 @@ class MDSettings:
     ("mu4", "mu3", "previous", "none"; "learned": the MLP guess of PGMForceField.guess_config, not in
     the reference), fused, norm_refresh, peek, precision ("mixed" | "double").
     Beyond the reference's MD settings: recip ("pme" | "ewald", the exact Ewald sum of the
-    PeriodicPGM reference model) and k_tol (its k-vector truncation).
+    PeriodicPGM reference model), k_tol (its k-vector truncation) and precond ("jacobi", the
+    reference's z = alpha r, or "molecule": the inverse 9x9 block of each 3-site molecule).
     """
 
     elec: str = "qpi"
@@ -42,6 +43,7 @@ class MDSettings:
     precision: str = "mixed"
     recip: str = "pme"
     k_tol: float = 1e-12
+    precond: str = "jacobi"
 
     def __post_init__(self) -> None:
         """Validate like PGMForceField.__init__ (forcefield.py:917-1000)."""
@@ -59,6 +61,8 @@ class MDSettings:
             raise ValueError(f"recip must be 'pme' or 'ewald', got {self.recip!r}")
         if not 0.0 < self.k_tol < 1.0:
             raise ValueError("k_tol must be in (0, 1)")
+        if self.precond not in ("jacobi", "molecule"):
+            raise ValueError(f"precond must be 'jacobi' or 'molecule', got {self.precond!r}")
 
     def replace(self, **changes) -> MDSettings:
         """Return a copy with some settings changed (TypeError for an unknown name, as the reference)."""
