@@ -231,6 +231,8 @@ def main() -> int:  # noqa: C901
     ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
     ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
     ap.add_argument("--bead-evals", type=int, default=10, help="timed force evaluations of the 32-bead leg (0: off)")
+    ap.add_argument("--graph-steps", type=int, default=50,
+                    help="timed steps of the measured context again with its CG iterations replayed as hipGraphs (0: off)")
     ap.add_argument("--precond", default="molecule", choices=["jacobi", "molecule"],
                     help="CG preconditioner of the mixed paths (the FP64 baseline keeps the reference's Jacobi)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -315,6 +317,19 @@ def main() -> int:  # noqa: C901
     guess_status = ffm.guess_status() if a.guess == "learned" else None
     if guess_status:
         log("learned guess after the timed steps:", json.dumps(guess_status))
+    graphs = None
+    if a.graph_steps > 0:  # the same context and trajectory, CG iterations as hipGraphs (no per-kernel events)
+        ffm.set_graphs(True)
+        mdm.run(3, a.dt_fs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        repg = mdm.run(a.graph_steps, a.dt_fs)
+        torch.cuda.synchronize()
+        ms_g = (time.perf_counter() - t0) * 1e3 / a.graph_steps
+        ffm.set_graphs(False)
+        graphs = {"ms_per_step": round(ms_g, 4), "value": round(ns_per_day(ms_g, a.dt_fs), 3), "steps": a.graph_steps,
+                  "cg_iters_per_step": round(repg["cg_iterations"] / a.graph_steps, 3)}
+        log(f"mixed with CG hipGraphs: {ms_g:.3f} ms/step, {repg['cg_iterations'] / a.graph_steps:.2f} CG it/step")
     live = repm["kernel_timing"]
     log("live kernel timing over the timed steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
@@ -373,11 +388,13 @@ def main() -> int:  # noqa: C901
                          f"dt {a.dt_fs} fs, FP32 kernels with FP64 accumulation; baseline: the FP64 path from the "
                          f"previous-step guess"),
             "waters": a.waters, "atoms": n, "pme_grid": a.grid, "dt_fs": a.dt_fs, "guess": a.guess, "precond": a.precond,
+
         },
         "cg_iters_per_step": round(repm["cg_iterations"] / a.steps, 3),
         "autotune": repm["autotune"],
         "learned_guess": guess_status,
         "mixed_previous_guess": prev,
+        "mixed_cg_graphs": graphs,
         "fp64": fp64,
         "errors_vs_fp64": None if errors is None else {k: float(f"{v:.3e}") for k, v in errors.items()},
         "energy_drift_check": {"etot_end_kJmol": repm["etot"], "temperature_K": round(repm["temperature"], 2)},

@@ -136,6 +136,14 @@ int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy 
 int pgm_bead_count(const pgm_ctx *ctx);
 int pgm_bead_results(pgm_ctx *ctx, pgm_energy *energies, int *iterations, double *residuals);
 
+/* CG iterations as hipGraphs (no counterpart in the reference): with `on`, iteration k of every
+ * solve (the row field on the side stream, the PME chain, the gather, the two update kernels) is
+ * captured once into a graph and replayed with hipGraphLaunch; the host's pacing of the iterations
+ * and the results are unchanged.  Graphs are dropped when the box, row capacities, tiles or layout
+ * change, and not used while live kernel timing is on.  A context on the null stream (which cannot be
+ * captured) replays them on a private stream ordered after and before its work by events. */
+int pgm_set_graphs(pgm_ctx *ctx, int on);
+
 /* Layout of the CG row field (the data of PGMForceField._row_field, forcefield.py:1395-1402): 0 atom
  * rows (partner index, G1, G2 per pair in sliced ELL), 1 molecule blocks (contexts of consecutive
  * 3-atom molecules: per molecule pair the 9 atom pairs' G1, G2, `lanes` = 4, 8 or 16 lanes per

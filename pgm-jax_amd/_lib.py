@@ -33,6 +33,7 @@ EXPORTS = (
     "pgm_compute",
     "pgm_bead_count",
     "pgm_bead_results",
+    "pgm_set_graphs",
     "pgm_set_row_layout",
     "pgm_row_layout",
     "pgm_autotune",
@@ -177,6 +178,7 @@ def lib() -> C.CDLL:
     L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
     L.pgm_bead_count.argtypes = [P]
     L.pgm_bead_results.argtypes = [P, P, P, P]
+    L.pgm_set_graphs.argtypes = [P, I]
     L.pgm_set_row_layout.argtypes = [P, I, I]
     L.pgm_row_layout.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.pgm_autotune.argtypes = [P, P, I, C.POINTER(D)]

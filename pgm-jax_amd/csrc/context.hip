@@ -133,6 +133,12 @@ struct pgm_ctx {
     // molecules, molecule blocks (mj, mg; rlm lanes per molecule) from the candidate molecules mrows
     bool mol3 = false;            // every bead is consecutive 3-atom molecules
     bool atom_rows_fresh = false; // rows / count / count_near compacted for the current geometry
+    // CG iterations as hipGraphs (pgm_set_graphs): one executable per iteration index k, captured on
+    // first use and replayed; dropped whenever a launch argument they hold can change
+    bool use_graphs = false;
+    std::vector<hipGraphExec_t> it_graphs;
+    hipStream_t gstream = nullptr;              // capture / replay stream when the context runs on the null stream
+    hipEvent_t ev_g0 = nullptr, ev_g1 = nullptr;  // ordering between the two
     bool molp = false;            // molecule-block CG preconditioner (settings.precond 1)
     bool pinv_stale = true;       // its blocks are remade at the next evaluation after a row build
     DevBuf pinv;                  // its inverse 9x9 blocks, 81 reals per molecule
@@ -218,6 +224,12 @@ struct pgm_ctx {
 // since the row build.
 static double *aux_slots(const pgm_ctx *c) { return c->energy.as<double>() + 4 * c->nbead; }
 
+static void drop_graphs(pgm_ctx *c) {
+    for (hipGraphExec_t &g : c->it_graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    c->it_graphs.clear();
+}
+
 extern "C" const char *pgm_last_error(void) { return g_err.c_str(); }
 extern "C" int pgm_version(void) { return 1; }
 
@@ -297,6 +309,7 @@ template <typename T> static int upload(DevBuf &b, const T *h, size_t count, hip
 
 // ---------------------------------------------------------------------------- box
 static int set_box(pgm_ctx *c, const double *H) {
+    drop_graphs(c);
     for (int k = 0; k < 9; ++k) c->H[k] = H[k];
     if (std::fabs(H[1]) + std::fabs(H[2]) + std::fabs(H[5]) > 1e-12 * std::fabs(H[0])) {
         g_err = "box must be lower triangular (lattice vectors as rows)";
@@ -409,6 +422,7 @@ static int allow_lds(pgm_ctx *c, size_t spread, size_t gather) {
 // and a last tile at least P-1 wide; gather tiles of T16 = 2 T8 (the union of two spread tiles per axis).  No such T8: the
 // atomic spread and the per-atom gather are used.
 static int setup_tiles(pgm_ctx *c) {
+    drop_graphs(c);
     const int P = c->s.order;
     PMETiles t8{}, t16{};
     bool ok = true;
@@ -566,6 +580,7 @@ __global__ void k_pack_xr(int n, const double *__restrict__ pos, const double *_
 }
 
 static int alloc_rows(pgm_ctx *c, int cap) {
+    drop_graphs(c);
     c->cap_full = cap;
     c->cap = cap;
     PGM_TRY(c->rows_full.ensure((size_t)c->n * cap * sizeof(int)));
@@ -579,6 +594,7 @@ static int alloc_rows(pgm_ctx *c, int cap) {
 
 // Molecule-block buffers for candidate capacity mcap_full (records rounded up to rlm lanes).
 static int alloc_mol(pgm_ctx *c, int mcap_full) {
+    drop_graphs(c);
     const int nmol = c->n / 3;
     c->mcap_full = mcap_full;
     c->mcap = (mcap_full + c->rlm - 1) / c->rlm * c->rlm;
@@ -1250,20 +1266,71 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
         return m;
     };
-    PGM_TRY(read_conv(0));  // converged before any iteration?
-    while (launched < max_iter) {
-        const int k = launched;
+    // iteration k on c->stream: the field of p, then the two update kernels (all early-exit once converged)
+    auto iteration = [&](int k) -> int {
+        hipStream_t cs = c->stream;
         PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
         if (c->molp)
-            k_cg_update1_mol<real><<<gM, MOLP_BLOCK, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z,
+            k_cg_update1_mol<real><<<gM, MOLP_BLOCK, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z,
                                                                p, Ap, c->alpha_r.as<real>(), c->pinv.as<real>(),
                                                                c->partB.as<double>(), S);
         else
-            k_cg_update1<real><<<gB, 256, 0, st>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
+            k_cg_update1<real><<<gB, 256, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
                                                    c->alpha_r.as<real>(), c->partB.as<double>(), S);
-        k_cg_update2<real><<<gB, 256, 0, st>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, c->s.dipole_tol,
+        k_cg_update2<real><<<gB, 256, 0, cs>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, c->s.dipole_tol,
                                                max_iter);
         LAUNCH_CHECK();
+        return PGM_OK;
+    };
+    const bool graphs = c->use_graphs && !c->timing;
+    hipStream_t gs = st ? st : c->gstream;  // the null stream cannot be captured: a private stream, ordered by events
+    PGM_TRY(read_conv(0));  // converged before any iteration?
+    while (launched < max_iter) {
+        const int k = launched;
+        if (graphs) {
+            if ((int)c->it_graphs.size() <= k) c->it_graphs.resize(k + 1, nullptr);
+            if (gs != st) {
+                HIP_TRY(hipEventRecord(c->ev_g0, st));
+                HIP_TRY(hipStreamWaitEvent(gs, c->ev_g0, 0));
+            }
+            if (!c->it_graphs[k]) {  // capture iteration k once (on gs)
+                if (gs != st) {
+                    c->stream = gs;
+                    FFT_TRY(rocfft_execution_info_set_stream(c->info, gs));
+                }
+                HIP_TRY(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
+                const int rc = iteration(k);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(gs, &g);
+                if (gs != st) {
+                    c->stream = st;
+                    FFT_TRY(rocfft_execution_info_set_stream(c->info, st));
+                }
+                if (rc < 0) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) {
+                    g_err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+                    return PGM_ERR_HIP;
+                }
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    g_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+                    return PGM_ERR_HIP;
+                }
+                c->it_graphs[k] = ex;
+            }
+            HIP_TRY(hipGraphLaunch(c->it_graphs[k], gs));
+            if (gs != st) {
+                HIP_TRY(hipEventRecord(c->ev_g1, gs));
+                HIP_TRY(hipStreamWaitEvent(st, c->ev_g1, 0));
+            }
+        } else {
+            PGM_TRY(iteration(k));
+        }
         launched = k + 1;
         // slot `launched` is waited on at the next pass (launched >= expect - 2) or after the loop;
         // earlier reads would only add a copy and its gap to the queue
@@ -1691,6 +1758,13 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     for (auto &t : c->timed)
         for (hipEvent_t e : t.ev)
             if (e) (void)hipEventDestroy(e);
+    drop_graphs(c);
+    if (c->gstream) {
+        (void)hipStreamSynchronize(c->gstream);
+        (void)hipStreamDestroy(c->gstream);
+    }
+    if (c->ev_g0) (void)hipEventDestroy(c->ev_g0);
+    if (c->ev_g1) (void)hipEventDestroy(c->ev_g1);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fwd) rocfft_plan_destroy(c->fwd);
     if (c->bwd) rocfft_plan_destroy(c->bwd);
@@ -1721,6 +1795,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
 
 extern "C" int pgm_set_stream(pgm_ctx *c, void *stream) {
     if (!c) return PGM_ERR_ARG;
+    drop_graphs(c);
     c->stream = (hipStream_t)stream;
     if (c->info) FFT_TRY(rocfft_execution_info_set_stream(c->info, c->stream));
     return PGM_OK;
@@ -1844,6 +1919,19 @@ extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pg
 }
 
 extern "C" int pgm_bead_count(const pgm_ctx *c) { return c ? c->nbead : PGM_ERR_ARG; }
+
+extern "C" int pgm_set_graphs(pgm_ctx *c, int on) {
+    if (!c) return PGM_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (on && !c->gstream) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_g0, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_g1, hipEventDisableTiming));
+    }
+    if (!on) drop_graphs(c);
+    c->use_graphs = on != 0;
+    return PGM_OK;
+}
 
 extern "C" int pgm_set_row_layout(pgm_ctx *c, int layout, int lanes) {
     if (!c || (layout != 0 && layout != 1) || (layout == 1 && lanes != 4 && lanes != 8 && lanes != 16)) {

@@ -342,6 +342,10 @@ class PGMForceField:
         self._built_at = None
         return _autotune(self._ctx, pos, reps)
 
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay every CG iteration as a captured hipGraph (same kernels, same results)."""
+        _lib.check(_lib.lib().pgm_set_graphs(self._ctx, int(bool(on))), "pgm_set_graphs")
+
     def row_layout(self) -> tuple[str, int]:
         """Return the CG row-field layout and its lanes per row."""
         lay, lanes = C.c_int(), C.c_int()
