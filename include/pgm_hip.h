@@ -227,7 +227,7 @@ int pgm_time_kernels(pgm_ctx *ctx, const double *pos, int reps, double *ms);
  * dipoles the solves produce (refitted every harvest_every solves), plus a per-atom MLP correction
  * (16 features: the last four dipoles in a local frame of the molecule and an oxygen flag;
  * hidden_layers x 128 SiLU; 3 outputs) trained on the linear predictor's errors harvested into a
- * ring of `capacity` rows.  BF16 inference on the matrix cores, FP32 training (Adam).  The judge,
+ * ring of `capacity` rows.  BF16 inference on the matrix cores, FP32 training (Adam).  The selector,
  * at every harvest: the linear predictor is used while its error is below 0.8 x that of mu4, the
  * network on top of it while the sum's error is below 0.8 x the linear predictor's; else mu4.  The
  * CG always runs to the same tolerance.  Needs consecutive 3-site molecules.

@@ -1,9 +1,9 @@
 """CPU oracle of the pGM electrostatics hot path: TEST INFRASTRUCTURE ONLY.
 
 This package is a float64 numpy restatement of the reference's algorithm (pGM-JAX: md/kernels.py,
-md/pme.py, md/forcefield.py, ewald.py, solver.py, lj.py) and the checker against which the HIP
-library is tested.  Only `tests/`, `__graft_entry__.smoke()` and `bench.py`'s CPU-baseline leg may
-import it, and there only as the checker or as the timed CPU baseline; the product package
+md/pme.py, md/forcefield.py, ewald.py, solver.py, lj.py) and the reference results against which the
+HIP library is tested.  Only `tests/`, `__graft_entry__.smoke()` and `bench.py`'s CPU-baseline leg may
+import it, and there only to check results or as the timed CPU baseline; the product package
 (`pgm-jax_amd/`) never imports, links or executes anything from here.
 
 Modules

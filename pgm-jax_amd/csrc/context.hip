@@ -200,7 +200,7 @@ struct pgm_ctx {
     struct Guess {
         bool on = false;        // configured
         bool trained = false;   // at least one training call
-        int mode = 0;           // the judge's choice: 0 mu4, 1 fitted linear predictor, 2 linear + network
+        int mode = 0;           // the selector's choice: 0 mu4, 1 fitted linear predictor, 2 linear + network
         int nh = 3, every = 10, cap = 0;
         long long nparams = 0, rows = 0;
         unsigned long long seed = 0;
@@ -1076,8 +1076,8 @@ static int guess_alloc_batch(pgm_ctx *c, int B) {
 static const GuessCoef MU4_COEF{{4.0, -6.0, 4.0, -1.0, 0.0, 0.0, 0.0, 0.0}};
 
 // Before the solve (predictor 4, ring full): features of every atom, the network's output when
-// trained, and x0 by the judge's mode: the fitted linear predictor (+ the network), or mu4.
-// x0 of the learned guess.  The features and the network run when the judge uses the network
+// trained, and x0 by the selector's mode: the fitted linear predictor (+ the network), or mu4.
+// x0 of the learned guess.  The features and the network run when the selector uses the network
 // (mode 2) or when this solve is harvested (the harvest scores the network and stores features).
 template <typename real> static int guess_x0(pgm_ctx *c, const double *pos, real4<real> *x0, bool harvest) {
     auto &G = c->guess;
@@ -1131,7 +1131,7 @@ static bool solve_small(int K, const double *A0, const double *b0, double *x) {
 
 // After the solve of a harvest step (ring full, before the ring push): statistics of the ring
 // against mu_t (one sync), the refit of the linear predictor, the network's training rows, and the
-// judge's choice for the following steps.
+// selector's choice for the following steps.
 static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
     auto &G = c->guess;
     const int n = c->n;
@@ -1152,7 +1152,7 @@ static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
     G.err_lin = h[44] * inv;
     G.err_mu4 = h[45] * inv;
     G.err_net = G.have_y ? h[46] * inv : 0.0;
-    // judge for the following steps (errors of this harvest, measured before the refit)
+    // selector for the following steps (errors of this harvest, measured before the refit)
     int mode = 0;
     if (G.nfit > 0 && G.err_lin < 0.8 * G.err_mu4) mode = 1;
     if (mode == 1 && G.have_y && G.err_net < 0.8 * G.err_lin) mode = 2;

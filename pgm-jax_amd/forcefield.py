@@ -303,7 +303,7 @@ class PGMForceField:
 
     def guess_status(self) -> dict:
         """Rows harvested, sigma, squared errors of mu4 / the linear predictor / linear + network at the last
-        harvest, the judge's mode, trained, number of fits, the linear coefficients."""
+        harvest, the selector's mode, trained, number of fits, the linear coefficients."""
         out = (C.c_double * 16)()
         _lib.check(_lib.lib().pgm_guess_status(self._ctx, out), "pgm_guess_status")
         modes = ("mu4", "linear", "linear+network")

@@ -3,7 +3,7 @@
 Double precision must reproduce the reference's float64 goldens to rounding (energies 1e-9
 relative, forces 1e-7 of the largest, dipoles 1e-8, the same CG iteration count); mixed precision
 (float32 kernels, FP64 accumulation) must stay within 1e-6 relative in energy and 1e-5 relative
-RMS in forces of the float64 result (north_star asks |dE|/E <= 1e-7 for the mixed path in MD; the
+RMS in forces of the float64 result (the mixed path's MD target is |dE|/E <= 1e-7; the
 single-point bound here is the reference's own mixed/double agreement criterion,
 tests/test_water512.py:10-12).
 """

@@ -76,9 +76,9 @@ def test_gradients_and_adam_match_oracle():
         assert np.abs(gb - rb).max() < 1e-4 * max(np.abs(rb).max(), 1e-12)
 
 
-def test_learned_guess_in_md_converges_and_judges():
+def test_learned_guess_in_md_converges_and_selects():
     """MD with predictor "learned": the ring fills, the linear predictor is fitted at each harvest (every 5 solves),
-    the network trains on its errors; the judge picks the better guess; every solve reaches the tolerance."""
+    the network trains on its errors; the selector picks the better guess; every solve reaches the tolerance."""
     P, ff, sys_, pos, H = _ff()
     ff.guess_config(hidden_layers=3, harvest_every=5, seed=9)
     vel = P.models.maxwell_boltzmann_rigid(pos, sys_.masses, 298.0, seed=6)
@@ -103,7 +103,7 @@ def test_learned_guess_in_md_converges_and_judges():
 
 def test_linear_predictor_beats_mu4_in_cg_iterations():
     """The same 512-water trajectory from the same state: mean CG iterations with the learned guess after the fit
-    vs mu4 (the judge keeps mu4 when the fit does not help, so never worse by more than one iteration)."""
+    vs mu4 (the selector keeps mu4 when the fit does not help, so never worse by more than one iteration)."""
     import pgm_jax_amd as P
 
     sys_, pos, H = P.models.water_box(512, 2.483, seed=5)
