@@ -101,7 +101,7 @@ def pmc_traffic(kernel: str):
         return None
 
 
-def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int) -> dict:
+def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int, fused: bool = False) -> dict:
     """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
 
     Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
@@ -115,8 +115,12 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
       row field, atom rows: per row entry j (4 B) + G1, G2 (FP32, 8 B); per atom the dipole read
                  (16 B) and the field written (16 B);
       spread:    per atom positions 12 + q 4 + d 12 B, the FP32 grid written once (K^3 x 4 B) -- the
-                 tiled spread plus its block merge;
-      gather:    the FP32 grid read once (K^3 x 4 B), per atom d in 12 B and the field out 12 B.
+                 tiled spread plus its block merge (with the fused transform the merge belongs to
+                 the forward planes and the spread entry is the tiled spread alone);
+      gather:    the FP32 grid read once (K^3 x 4 B), per atom d in 12 B and the field out 12 B;
+      transform: (fused kernels, isolated timings) forward planes: the grid read and the half
+                 spectrum (K^3/2 complex x 8 B) written; x transform: the spectrum read and written
+                 and Gc read (K^3/2 x 4 B); inverse planes: the spectrum read and the grid written.
     """
     if layout == "molecule":
         row_bytes = blocks * 76 + n * 64
@@ -131,7 +135,7 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
     def gbs(b, ms):
         return b / (ms * 1e-3) / 1e9
 
-    return {
+    out = {
         "kernel": kdesc,
         "bound": "hbm",
         "achieved": round(gbs(row_bytes, row_ms), 2),
@@ -144,7 +148,8 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
         "launches": live["row_field"]["launches"],
         "row_entries": entries,
         "row_blocks": blocks,
-        "spread": {"kernels": "k_spread_tiles + k_merge_table", "achieved": round(gbs(spread_bytes, spread_ms), 2),
+        "spread": {"kernels": "k_spread_tiles" if fused else "k_spread_tiles + k_merge_table",
+                   "achieved": round(gbs(spread_bytes, spread_ms), 2),
                    "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
                    "ms_per_launch": round(spread_ms, 5)},
         "gather": {"kernels": "k_gather_tiles", "achieved": round(gbs(gather_bytes, gather_ms), 2),
@@ -152,6 +157,15 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                    "algorithmic_bytes": gather_bytes, "ms_per_launch": round(gather_ms, 5)},
         "isolated_ms": {k: round(v, 5) for k, v in kern.items()},
     }
+    if fused:
+        half = K3 // 2  # complex points of the half spectrum (to K^2 (K/2 + 1) ~ K^3 / 2)
+        tb = {"fwd_planes": K3 * 4 + half * 8, "x_conv": 2 * half * 8 + half * 4, "bwd_planes": half * 8 + K3 * 4}
+        t_ms = sum(kern[k] for k in tb)
+        out["transform"] = {"kernels": "k_pme_fwd_plane + k_pme_x_conv + k_pme_bwd_plane (isolated)",
+                            "achieved": round(gbs(sum(tb.values()), t_ms), 2),
+                            "frac": round(gbs(sum(tb.values()), t_ms) / HBM_PEAK_GBS, 5),
+                            "algorithmic_bytes": sum(tb.values()), "ms_per_launch": round(t_ms, 5)}
+    return out
 
 
 def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
@@ -366,7 +380,7 @@ def main() -> int:  # noqa: C901
     beads = bead_leg(P, a.bead_evals, 3) if a.bead_evals > 0 else None
     n = 3 * a.waters
     K3 = a.grid**3
-    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3)
+    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused")
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,

@@ -152,11 +152,23 @@ int pgm_set_graphs(pgm_ctx *ctx, int on);
 int pgm_set_row_layout(pgm_ctx *ctx, int layout, int lanes);
 int pgm_row_layout(const pgm_ctx *ctx, int *layout, int *lanes);
 
+/* Reciprocal transform of the CG field evaluations (the FFT / convolution / inverse FFT of
+ * pme.py:373-376): 0 the library's batched r2c / c2r plans with a separate convolution kernel, 1
+ * the fused kernels (the spread-block merge, the real transform along z and the transform along y
+ * per grid plane; along x with the product by the influence function and its inverse; the inverse
+ * planes), the default where available: every grid axis a power of two in 8..256, tiled spreading,
+ * K2 (K3/2 + 1) complex values within 160 KB.  Energy, force and strain evaluations keep the library
+ * plans.  pgm_pme_transform reports the transform in use and whether the fused one is available. */
+int pgm_set_pme_transform(pgm_ctx *ctx, int fused);
+int pgm_pme_transform(const pgm_ctx *ctx, int *fused, int *available);
+
 /* Autotuner (no counterpart in the reference): times, on this context's system at positions pos
- * (device), every applicable row-field layout (molecule blocks with 4, 8, 16 lanes; atom rows) and
- * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that),
- * `reps` launches each, and keeps the fastest.  report (host, 8 doubles): layout, lanes, spread tile
- * width per axis (3), row-field ms, spread + merge + FFT + gather ms, layouts tried.  Synchronises. */
+ * (device), every applicable row-field layout (molecule blocks with 4, 8, 16 lanes; atom rows),
+ * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that)
+ * and both reciprocal transforms, `reps` launches each, and keeps the fastest.  report (host, 10
+ * doubles): layout, lanes, spread tile width per axis (3), row-field ms, spread + transform + gather
+ * ms, layouts tried, fused transform chosen (1) or not (0), the other transform's ms (0 if not
+ * timed).  Synchronises. */
 int pgm_autotune(pgm_ctx *ctx, const double *pos, int reps, double *report);
 
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy

@@ -37,6 +37,8 @@ EXPORTS = (
     "pgm_set_row_layout",
     "pgm_row_layout",
     "pgm_autotune",
+    "pgm_set_pme_transform",
+    "pgm_pme_transform",
     "pgm_energy_forces",
     "pgm_strain_derivative",
     "pgm_solve_linear_induction",
@@ -182,6 +184,8 @@ def lib() -> C.CDLL:
     L.pgm_set_row_layout.argtypes = [P, I, I]
     L.pgm_row_layout.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.pgm_autotune.argtypes = [P, P, I, C.POINTER(D)]
+    L.pgm_set_pme_transform.argtypes = [P, I]
+    L.pgm_pme_transform.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
     L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]

@@ -25,7 +25,7 @@ import dataclasses
 import numpy as np
 
 from . import _lib
-from .forcefield import InductionState, _autotune, _dev, _make_context, _ptr, _torch
+from .forcefield import InductionState, _autotune, _dev, _make_context, _ptr, _set_transform, _torch, _transform
 from .settings import MDSettings, grid_size
 from .system import System
 
@@ -111,10 +111,18 @@ class PGMBeads:
         self._built_at = q.clone()
 
     def autotune(self, q, reps: int = 20) -> dict:
-        """Time every row-field layout and PME spread-tile width on the beads q (P, N, 3) and keep the fastest."""
+        """Time every row-field layout, PME spread-tile width and reciprocal transform on the beads q (P, N, 3); keep the fastest."""
         q = _dev(q, (self.P, self.n, 3))
         self._built_at = None
         return _autotune(self._ctx, q, reps)
+
+    def set_pme_transform(self, transform: str = "fused") -> None:
+        """Reciprocal transform of the CG field evaluations: "fused" kernels or the "library" FFT plans."""
+        _set_transform(self._ctx, transform)
+
+    def pme_transform(self) -> tuple[str, bool]:
+        """Return the transform in use and whether the fused one is available for this grid."""
+        return _transform(self._ctx)
 
     def bead_results(self) -> tuple[list[dict], np.ndarray, np.ndarray]:
         """Per-bead energies {"elec","vdw","total"}, CG iterations and final residuals of the last compute."""

@@ -18,6 +18,7 @@
 #include "pairs.h"
 #include "md.h"
 #include "pme.h"
+#include "fft.h"
 #include "virial.h"
 #include "ewald.h"
 #include "dense.h"
@@ -159,6 +160,11 @@ struct pgm_ctx {
     DevBuf grid, cgrid, Gc, fft_work;
     rocfft_plan fwd = nullptr, bwd = nullptr;
     rocfft_execution_info info = nullptr;
+    // fused transform of the CG field evaluations (fft.h): available for power-of-two grids with
+    // tiled spreading; pme_fused selects it over the library plans
+    bool fused_ok = false, pme_fused = false;
+    PFFTGeom pfg{};
+    DevBuf pfft_tw;
     // tiled spread (atoms sorted by grid tile)
     // spread tiles of T8 points per axis (tiles8, one fine key each) inside gather tiles of 2 T8
     // (tiles); tstart/tend by fine key, tstart16/tend16 by gather tile
@@ -467,7 +473,7 @@ static int setup_tiles(pgm_ctx *c) {
     PGM_TRY(c->tend16.ensure(sizeof(int) * B * t16.ntiles));
     PGM_TRY(c->spl.ensure(c->rsize * 6 * P * (size_t)n));
     PGM_TRY(c->blocks.ensure(c->rsize * B * t8.ntiles * t8.bsize));
-    {  // per-axis offsets of the block merge (k_merge_table)
+    {  // per-axis offsets of the block merge (k_merge_table, k_pme_fwd_plane)
         std::vector<int> tab;
         for (int a = 0; a < 3; ++a) {
             const int K = c->s.grid[a], T = t8.T[a], nt = t8.nt[a];
@@ -569,6 +575,46 @@ static int make_plans(pgm_ctx *c) {
     FFT_TRY(rocfft_execution_info_create(&c->info));
     if (c->fft_work.bytes) FFT_TRY(rocfft_execution_info_set_work_buffer(c->info, c->fft_work.p, c->fft_work.bytes));
     FFT_TRY(rocfft_execution_info_set_stream(c->info, c->stream));
+    return PGM_OK;
+}
+
+// Fused transform of the CG field evaluations (fft.h): every grid axis a power of two in 8..256,
+// the plane of K2 (K3/2 + 1) complex values within 160 KB of LDS, no Ewald sum.  Twiddles
+// W_N^t = exp(-2 pi i t / N) for N = K3/2, K3, K2, K1 in FP64, rounded to the context's precision.
+template <typename real> static int fused_lds(pgm_ctx *c) {
+    const int plane = (int)pfft_plane_lds<real>(c->pfg), xc = (int)pfft_xconv_lds<real>(c->pfg);
+    HIP_TRY(hipFuncSetAttribute((const void *)k_pme_fwd_plane<real>, hipFuncAttributeMaxDynamicSharedMemorySize, plane));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_pme_bwd_plane<real>, hipFuncAttributeMaxDynamicSharedMemorySize, plane));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_pme_x_conv<real>, hipFuncAttributeMaxDynamicSharedMemorySize, xc));
+    std::vector<real> tw;
+    int *offs[4] = {&c->pfg.twM, &c->pfg.twK3, &c->pfg.twK2, &c->pfg.twK1};
+    const int lens[4] = {c->pfg.M, c->pfg.K3, c->pfg.K2, c->pfg.K1};
+    for (int a = 0; a < 4; ++a) {
+        *offs[a] = (int)(tw.size() / 2);
+        for (int t = 0; t < lens[a]; ++t) {
+            const double ang = 2.0 * M_PI * (double)t / (double)lens[a];
+            tw.push_back((real)std::cos(ang));
+            tw.push_back((real)-std::sin(ang));
+        }
+    }
+    PGM_TRY(upload(c->pfft_tw, tw.data(), tw.size(), c->stream));
+    return PGM_OK;
+}
+
+static int setup_fused(pgm_ctx *c) {
+    c->fused_ok = false;
+    c->pme_fused = false;
+    if (c->ewald) return PGM_OK;
+    const int *K = c->s.grid;
+    for (int a = 0; a < 3; ++a)
+        if (K[a] < 8 || K[a] > 256 || (K[a] & (K[a] - 1))) return PGM_OK;
+    const PFFTGeom f{K[0], K[1], K[2], K[2] / 2, K[2] / 2 + 1, 0, 0, 0, 0};
+    const size_t plane = c->mixed ? pfft_plane_lds<float>(f) : pfft_plane_lds<double>(f);
+    if (plane > 160 * 1024) return PGM_OK;
+    c->pfg = f;
+    PGM_TRY(c->mixed ? fused_lds<float>(c) : fused_lds<double>(c));
+    c->fused_ok = true;
+    c->pme_fused = true;
     return PGM_OK;
 }
 
@@ -774,6 +820,23 @@ static void timing_end(pgm_ctx *c, int which, int k, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------- PME dispatch
+// The fused transform of the merged spread blocks: phi = c2r(Gc r2c(Q)) into c->grid (fft.h).
+template <typename real> static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st) {
+    using C = cplx<real>;
+    const PFFTGeom &f = c->pfg;
+    const int B = c->nbead;
+    const size_t plane = pfft_plane_lds<real>(f);
+    const int nth = f.K2 * f.K3 >= 8192 ? pfft_plane_threads<real>() : 256;
+    const C *tw = c->pfft_tw.as<C>();
+    k_pme_fwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, (size_t)c->tiles8.ntiles * c->tiles8.bsize, c->merge_tab.as<int>(),
+                                                         c->blocks.as<real>(), c->cgrid.as<C>(), tw, conv, iter);
+    k_pme_x_conv<real><<<dim3(nblk((long long)f.K2 * f.K3h, PFFT_COLS), B), 256, pfft_xconv_lds<real>(f), st>>>(
+        f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1, conv, iter);
+    k_pme_bwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, c->cgrid.as<C>(), c->grid.as<real>(), tw, conv, iter);
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
 template <typename real>
 static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter,
                       double *vir_part = nullptr) {
@@ -795,6 +858,10 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
         LAUNCH_CHECK();
+        if (c->pme_fused && c->fused_ok && !energy && !vir_part) {  // merge + transforms + Gc fused (fft.h)
+            timing_end(c, 1, t_sp, st);
+            return fused_transform<real>(c, conv, iter, st);
+        }
         const int *K = c->s.grid;
         k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0] * B), 128, 0, st>>>(
             K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(), conv,
@@ -1742,6 +1809,7 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     const int cap0 = (int)(1.25 * 4.0 / 3.0 * M_PI * rl * rl * rl * c->nper / c->volume) + 32;
     if ((r = alloc_rows(c, std::min(cap0, c->nper))) < 0) return fail(r);
     if ((r = make_plans(c)) < 0) return fail(r);
+    if ((r = setup_fused(c)) < 0) return fail(r);
     *out = c;
     return PGM_OK;
 }
@@ -1778,7 +1846,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
                      &c->mrows, &c->mcount_full, &c->mcount, &c->mj, &c->mg, &c->pinv,
                      &c->pos_build, &c->minv, &c->mass, &c->d2, &c->md_part, &c->md_fail, &c->heat,
                      &c->pkey, &c->pidx, &c->pskey, &c->psidx, &c->tstart, &c->tend, &c->tstart16, &c->tend16, &c->spl, &c->loc,
-                     &c->loc16, &c->blocks, &c->merge_tab,
+                     &c->loc16, &c->blocks, &c->merge_tab, &c->pfft_tw,
                      &c->pme_sort_tmp, &c->stats, &c->mol_ptr, &c->mol_atoms, &c->com, &c->off, &c->rec_dd,
                      &c->vir_pair, &c->vir_conv, &c->vir_atoms, &c->vir_cov, &c->vir_out, &c->kvec, &c->sk,
                      &c->guess.params, &c->guess.grads, &c->guess.m, &c->guess.v, &c->guess.packed, &c->guess.bias,
@@ -2471,6 +2539,8 @@ extern "C" int pgm_copy_probe(void *dst, const void *src, size_t bytes, void *st
 // Average duration [ms] of each hot kernel over `reps` launches, HIP events on the context stream:
 // out[0] block merge (tiled spread; grid clear otherwise), [1] spread (dipoles), [2] r2c FFT, [3] convolution, [4] c2r FFT, [5] row field
 // (the CG matvec's pair part), [6] gather + combine, [7] one whole CG matvec (all of the above).
+// With the fused transform: [0] 0 (the merge is part of [2]), [2] forward planes (merge, z, y),
+// [3] x transform, Gc and inverse, [4] inverse planes (y, z).
 template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int reps, double *ms) {
     using R4 = real4<real>;
     if (c->ewald) {
@@ -2528,17 +2598,41 @@ template <typename real> static int time_impl(pgm_ctx *c, const double *pos, int
                                  }));
         if (rc < 0) return rc;
     }
-    void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
-    rc = timed(2, [&] { rocfft_execute(c->fwd, in, out, c->info); });
-    if (rc < 0) return rc;
-    rc = timed(3, [&] {
-        k_convolve<real, false><<<dim3(nblk((long long)c->ncplx, 256), c->nbead), 256, 0, st>>>(
-            c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, nullptr, -1);
-    });
-    if (rc < 0) return rc;
-    void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
-    rc = timed(4, [&] { rocfft_execute(c->bwd, in2, out2, c->info); });
-    if (rc < 0) return rc;
+    if (c->pme_fused && c->fused_ok && c->tiled) {  // merge fused into the forward plane kernel
+        using C = cplx<real>;
+        const PFFTGeom &f = c->pfg;
+        const size_t plane = pfft_plane_lds<real>(f);
+        const int nth = f.K2 * f.K3 >= 8192 ? pfft_plane_threads<real>() : 256;
+        const C *tw = c->pfft_tw.as<C>();
+        ms[0] = 0.0;
+        rc = timed(2, [&] {
+            k_pme_fwd_plane<real><<<f.K1 * c->nbead, nth, plane, st>>>(f, (size_t)c->tiles8.ntiles * c->tiles8.bsize,
+                                                                        c->merge_tab.as<int>(), c->blocks.as<real>(),
+                                                                        c->cgrid.as<C>(), tw, nullptr, -1);
+        });
+        if (rc < 0) return rc;
+        rc = timed(3, [&] {
+            k_pme_x_conv<real><<<dim3(nblk((long long)f.K2 * f.K3h, PFFT_COLS), c->nbead), 256, pfft_xconv_lds<real>(f), st>>>(
+                f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1, nullptr, -1);
+        });
+        if (rc < 0) return rc;
+        rc = timed(4, [&] {
+            k_pme_bwd_plane<real><<<f.K1 * c->nbead, nth, plane, st>>>(f, c->cgrid.as<C>(), c->grid.as<real>(), tw, nullptr, -1);
+        });
+        if (rc < 0) return rc;
+    } else {
+        void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
+        rc = timed(2, [&] { rocfft_execute(c->fwd, in, out, c->info); });
+        if (rc < 0) return rc;
+        rc = timed(3, [&] {
+            k_convolve<real, false><<<dim3(nblk((long long)c->ncplx, 256), c->nbead), 256, 0, st>>>(
+                c->ncplx, K3h, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, nullptr, -1);
+        });
+        if (rc < 0) return rc;
+        void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
+        rc = timed(4, [&] { rocfft_execute(c->bwd, in2, out2, c->info); });
+        if (rc < 0) return rc;
+    }
     rc = timed(5, [&] { (void)row_field_launch<real, false>(c, dq, nullptr, -1, st); });
     if (rc < 0) return rc;
     rc = timed(6, [&] { (void)gather_combine<real>(c, dq, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1); });
@@ -2633,6 +2727,24 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
         c->tile_cap = best_cap;
         PGM_TRY(setup_tiles(c));
     }
+    // reciprocal transform: the fused kernels against the library plans, with the chosen tiles
+    double other_pme = 0.0;
+    if (c->tiled && c->fused_ok) {
+        PGM_TRY(prepare(c, pos));
+        PGM_TRY(dipoles());
+        double ms[2] = {0.0, 0.0};
+        for (int fz = 0; fz < 2; ++fz) {
+            c->pme_fused = fz == 1;
+            PGM_TRY(time_ms([&] {
+                (void)spread_fft_only<real>(c, c->vp.as<R4>());
+                (void)gather_combine<real>(c, c->vp.as<R4>(), MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), nullptr, -1);
+            }, ms[fz]));
+        }
+        c->pme_fused = ms[1] <= ms[0];
+        best_pme = ms[c->pme_fused ? 1 : 0];
+        other_pme = ms[c->pme_fused ? 0 : 1];
+        drop_graphs(c);
+    }
     c->rows_valid = false;
     PGM_TRY(prepare(c, pos));
     HIP_TRY(hipStreamSynchronize(st));
@@ -2644,12 +2756,33 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
     out[5] = best_row;
     out[6] = best_pme < 1e300 ? best_pme : 0.0;
     out[7] = (double)lays.size();
+    out[8] = c->pme_fused && c->fused_ok ? 1.0 : 0.0;
+    out[9] = other_pme;
+    return PGM_OK;
+}
+
+extern "C" int pgm_set_pme_transform(pgm_ctx *c, int fused) {
+    if (!c) return PGM_ERR_ARG;
+    if (fused && !(c->fused_ok && c->tiled)) {
+        g_err = "pgm_set_pme_transform: the fused transform needs power-of-two grids of 8..256 points per axis, tiled "
+                "spreading and a plane of K2 (K3/2 + 1) complex values within 160 KB";
+        return PGM_ERR_ARG;
+    }
+    drop_graphs(c);
+    c->pme_fused = fused != 0;
+    return PGM_OK;
+}
+
+extern "C" int pgm_pme_transform(const pgm_ctx *c, int *fused, int *available) {
+    if (!c) return PGM_ERR_ARG;
+    if (fused) *fused = c->pme_fused && c->fused_ok && c->tiled ? 1 : 0;
+    if (available) *available = c->fused_ok && c->tiled ? 1 : 0;
     return PGM_OK;
 }
 
 extern "C" int pgm_autotune(pgm_ctx *c, const double *pos, int reps, double *report) {
     if (!c || !pos || !report || reps <= 0) return PGM_ERR_ARG;
-    for (int k = 0; k < 8; ++k) report[k] = 0.0;
+    for (int k = 0; k < 10; ++k) report[k] = 0.0;
     return c->mixed ? autotune_impl<float>(c, pos, reps, report) : autotune_impl<double>(c, pos, reps, report);
 }
 

@@ -99,11 +99,23 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
 
 def _autotune(ctx, pos, reps: int) -> dict:
     """pgm_autotune on a context; return the chosen shapes and their timings."""
-    rep = (C.c_double * 8)()
+    rep = (C.c_double * 10)()
     _lib.check(_lib.lib().pgm_autotune(ctx, C.c_void_p(pos.data_ptr()), int(reps), rep), "pgm_autotune")
     return {"row_layout": ("atom", "molecule")[int(rep[0])], "row_lanes": int(rep[1]),
             "spread_tile": tuple(int(x) for x in rep[2:5]), "row_field_ms": rep[5], "pme_ms": rep[6],
-            "layouts_tried": int(rep[7])}
+            "layouts_tried": int(rep[7]), "pme_transform": ("library", "fused")[int(rep[8])],
+            "other_transform_ms": rep[9]}
+
+
+def _set_transform(ctx, transform: str) -> None:
+    code = {"library": 0, "fused": 1}[transform]
+    _lib.check(_lib.lib().pgm_set_pme_transform(ctx, code), "pgm_set_pme_transform")
+
+
+def _transform(ctx) -> tuple[str, bool]:
+    fused, avail = C.c_int(), C.c_int()
+    _lib.check(_lib.lib().pgm_pme_transform(ctx, C.byref(fused), C.byref(avail)), "pgm_pme_transform")
+    return ("library", "fused")[fused.value], bool(avail.value)
 
 
 class PGMForceField:
@@ -337,7 +349,7 @@ class PGMForceField:
         self._built_at = None
 
     def autotune(self, pos, reps: int = 20) -> dict:
-        """Time every row-field layout and PME spread-tile width on this system at pos and keep the fastest."""
+        """Time every row-field layout, PME spread-tile width and reciprocal transform at pos; keep the fastest."""
         pos = _dev(pos, (self.n, 3))
         self._built_at = None
         return _autotune(self._ctx, pos, reps)
@@ -345,6 +357,14 @@ class PGMForceField:
     def set_graphs(self, on: bool = True) -> None:
         """Replay every CG iteration as a captured hipGraph (same kernels, same results)."""
         _lib.check(_lib.lib().pgm_set_graphs(self._ctx, int(bool(on))), "pgm_set_graphs")
+
+    def set_pme_transform(self, transform: str = "fused") -> None:
+        """Reciprocal transform of the CG field evaluations: "fused" kernels or the "library" FFT plans."""
+        _set_transform(self._ctx, transform)
+
+    def pme_transform(self) -> tuple[str, bool]:
+        """Return the transform in use and whether the fused one is available for this grid."""
+        return _transform(self._ctx)
 
     def row_layout(self) -> tuple[str, int]:
         """Return the CG row-field layout and its lanes per row."""

@@ -99,8 +99,15 @@ class RigidWaterMD:
         return rep
 
     def time_kernels(self, reps: int = 20) -> dict:
-        """Average duration [ms] of the hot kernels (pgm_time_kernels) at the current positions."""
+        """Average duration [ms] of the hot kernels (pgm_time_kernels) at the current positions.
+
+        With the fused reciprocal transform the block merge is part of the forward planes
+        ("fwd_planes": merge, z and y transforms; "x_conv": x transform, influence function and
+        inverse; "bwd_planes": inverse y and z transforms)."""
         ms = (C.c_double * 8)()
         _lib.check(_lib.lib().pgm_time_kernels(self.ff._ctx, C.c_void_p(self.pos.data_ptr()), int(reps), ms), "pgm_time_kernels")
+        if self.ff.pme_transform()[0] == "fused":
+            names = ("merge", "spread", "fwd_planes", "x_conv", "bwd_planes", "row_field", "gather", "matvec")
+            return {k: v for k, v in zip(names, list(ms)) if k != "merge"}
         names = ("merge", "spread", "fft_r2c", "convolve", "fft_c2r", "row_field", "gather", "matvec")
         return dict(zip(names, list(ms)))
