@@ -203,13 +203,27 @@ __global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, cons
     const int BY = t.B[1], BZ = t.B[2];
     for (int k = lane; k < t.bsize; k += WAVE) blk[k] = real(0);
     const int s0 = tstart[key], s1 = tend[key];
-    constexpr int NP = P * P * P;
+    // this lane's stencil points p = lane + 64 u (the last pass only where p < P^3): axis indices
+    // and block offsets, the same for every atom
+    constexpr int NP = P * P * P, NPASS = (NP + WAVE - 1) / WAVE;
+    static_assert((NPASS - 1) * WAVE < NP, "every pass but the last is full");
+    int pix[NPASS], piy[NPASS], piz[NPASS], poff[NPASS];
+    const bool plast = lane + (NPASS - 1) * WAVE < NP;
+#pragma unroll
+    for (int u = 0; u < NPASS; ++u) {
+        const int p = (u < NPASS - 1 || plast) ? lane + u * WAVE : 0;
+        pix[u] = p / (P * P);
+        piy[u] = (p / P) % P;
+        piz[u] = p % P;
+        poff[u] = (pix[u] * BY + piy[u]) * BZ + piz[u];
+    }
     for (int c0 = s0; c0 < s1; c0 += SPREAD_CHUNK) {
         const int m = min(SPREAD_CHUNK, s1 - c0);
         __syncthreads();
-        for (int k = lane; k < m * 6 * P; k += WAVE) {
+        for (int k = lane; k < m * 6 * P; k += WAVE) {  // (theta, dtheta) pairs per axis point
             const int a = k / (6 * P), v = k - a * 6 * P;
-            sv[a * SW + v] = spl[(size_t)(c0 + a) * 6 * P + v];
+            const int ax = v / (2 * P), w = v - ax * 2 * P, which = w / P, i = w - which * P;
+            sv[a * SW + ax * 2 * P + 2 * i + which] = spl[(size_t)(c0 + a) * 6 * P + v];
         }
         if (lane < m) {
             const real4<real> d = dq[sidx[c0 + lane]];
@@ -227,14 +241,21 @@ __global__ __launch_bounds__(64) void k_spread_tiles(PMETiles t, PMEGeom g, cons
             const int lc = sl[a];
             const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
             const real e0 = v[6 * P], e1 = v[6 * P + 1], e2 = v[6 * P + 2], q = v[6 * P + 3];
-            for (int p = lane; p < NP; p += WAVE) {
-                const int ix = p / (P * P), iy = (p / P) % P, iz = p % P;
-                const real tx = v[ix], dx = v[P + ix];
-                const real ty = v[2 * P + iy], dy = v[3 * P + iy];
-                const real tz = v[4 * P + iz], dz = v[5 * P + iz];
-                const real w = (q * tx + e0 * dx) * ty * tz + tx * (e1 * dy * tz + e2 * ty * dz);
-                blk[((lx + ix) * BY + (ly + iy)) * BZ + (lz + iz)] += w;
+            const int base = (lx * BY + ly) * BZ + lz;
+            real w[NPASS], old[NPASS];
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u) {
+                const real tx = v[2 * pix[u]], dx = v[2 * pix[u] + 1];
+                const real ty = v[2 * P + 2 * piy[u]], dy = v[2 * P + 2 * piy[u] + 1];
+                const real tz = v[4 * P + 2 * piz[u]], dz = v[4 * P + 2 * piz[u] + 1];
+                w[u] = (q * tx + e0 * dx) * ty * tz + tx * (e1 * dy * tz + e2 * ty * dz);
             }
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u)
+                if (u < NPASS - 1 || plast) old[u] = blk[base + poff[u]];
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u)
+                if (u < NPASS - 1 || plast) blk[base + poff[u]] = old[u] + w[u];
         }
     }
     __syncthreads();
@@ -431,13 +452,23 @@ __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, con
     const int o0 = u0 * t.T[0], o1 = u1 * t.T[1], o2 = u2 * t.T[2];
     const int K1 = g.K[0], K2 = g.K[1], K3 = g.K[2];
     phi += (size_t)bead * K1 * K2 * K3;
-    for (int k = tid; k < t.bsize; k += blockDim.x) {
-        const int bz = k % BZ, by = (k / BZ) % BY, bx = k / (BZ * BY);
-        int gx = o0 + bx, gy = o1 + by, gz = o2 + bz;
-        gx -= gx >= K1 ? K1 : 0;
-        gy -= gy >= K2 ? K2 : 0;
-        gz -= gz >= K3 ? K3 : 0;
-        blk[k] = phi[((size_t)gx * K2 + gy) * K3 + gz];
+    // block load, four loads in flight per thread before the LDS stores
+    const int nth = blockDim.x, bs = t.bsize;
+    for (int k0 = tid; k0 < bs; k0 += 4 * nth) {
+        real v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * nth < bs ? k0 + u * nth : 0;
+            const int bz = k % BZ, by = (k / BZ) % BY, bx = k / (BZ * BY);
+            int gx = o0 + bx, gy = o1 + by, gz = o2 + bz;
+            gx -= gx >= K1 ? K1 : 0;
+            gy -= gy >= K2 ? K2 : 0;
+            gz -= gz >= K3 ? K3 : 0;
+            v[u] = phi[((size_t)gx * K2 + gy) * K3 + gz];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k0 + u * nth < bs) blk[k0 + u * nth] = v[u];
     }
     (void)BX;
     __syncthreads();
