@@ -162,13 +162,21 @@ int pgm_row_layout(const pgm_ctx *ctx, int *layout, int *lanes);
 int pgm_set_pme_transform(pgm_ctx *ctx, int fused);
 int pgm_pme_transform(const pgm_ctx *ctx, int *fused, int *available);
 
+/* Where a CG field evaluation starts the row field on its side stream (no counterpart in the
+ * reference, whose field is one XLA program): 0 before the PME spread, 1 after it, 2 after the
+ * forward planes of the fused transform.  Only the overlap changes, not the results; the autotuner
+ * picks the fastest. */
+int pgm_set_row_fork(pgm_ctx *ctx, int stage);
+
 /* Autotuner (no counterpart in the reference): times, on this context's system at positions pos
  * (device), every applicable row-field layout (molecule blocks with 4, 8, 16 lanes; atom rows),
- * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that)
- * and both reciprocal transforms, `reps` launches each, and keeps the fastest.  report (host, 10
- * doubles): layout, lanes, spread tile width per axis (3), row-field ms, spread + transform + gather
- * ms, layouts tried, fused transform chosen (1) or not (0), the other transform's ms (0 if not
- * timed).  Synchronises. */
+ * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that),
+ * both reciprocal transforms and the point where a CG field evaluation starts the row field on its
+ * side stream (before the spread, after it, after the forward planes of the fused transform),
+ * `reps` launches each, and keeps the fastest.  report (host, 12 doubles): layout, lanes, spread
+ * tile width per axis (3), row-field ms, spread + transform + gather ms, layouts tried, fused
+ * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0, 1, 2),
+ * whole field-evaluation ms.  Synchronises. */
 int pgm_autotune(pgm_ctx *ctx, const double *pos, int reps, double *report);
 
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy

@@ -39,6 +39,7 @@ EXPORTS = (
     "pgm_autotune",
     "pgm_set_pme_transform",
     "pgm_pme_transform",
+    "pgm_set_row_fork",
     "pgm_energy_forces",
     "pgm_strain_derivative",
     "pgm_solve_linear_induction",
@@ -186,6 +187,7 @@ def lib() -> C.CDLL:
     L.pgm_autotune.argtypes = [P, P, I, C.POINTER(D)]
     L.pgm_set_pme_transform.argtypes = [P, I]
     L.pgm_pme_transform.argtypes = [P, C.POINTER(I), C.POINTER(I)]
+    L.pgm_set_row_fork.argtypes = [P, I]
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
     L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]

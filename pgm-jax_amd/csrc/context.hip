@@ -5,6 +5,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <functional>
 #include <climits>
 #include <random>
 #include <cmath>
@@ -163,6 +164,9 @@ struct pgm_ctx {
     // fused transform of the CG field evaluations (fft.h): available for power-of-two grids with
     // tiled spreading; pme_fused selects it over the library plans
     bool fused_ok = false, pme_fused = false;
+    // where a CG field evaluation forks the row field onto the side stream: 0 before the spread,
+    // 1 after the spread, 2 after the forward planes (fused transform only); chosen by pgm_autotune
+    int fork_stage = 0;
     PFFTGeom pfg{};
     DevBuf pfft_tw;
     // tiled spread (atoms sorted by grid tile)
@@ -821,7 +825,8 @@ static void timing_end(pgm_ctx *c, int which, int k, hipStream_t st) {
 
 // ---------------------------------------------------------------------------- PME dispatch
 // The fused transform of the merged spread blocks: phi = c2r(Gc r2c(Q)) into c->grid (fft.h).
-template <typename real> static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st) {
+template <typename real>
+static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st, const std::function<int()> *fork = nullptr) {
     using C = cplx<real>;
     const PFFTGeom &f = c->pfg;
     const int B = c->nbead;
@@ -830,6 +835,7 @@ template <typename real> static int fused_transform(pgm_ctx *c, const int *conv,
     const C *tw = c->pfft_tw.as<C>();
     k_pme_fwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, (size_t)c->tiles8.ntiles * c->tiles8.bsize, c->merge_tab.as<int>(),
                                                          c->blocks.as<real>(), c->cgrid.as<C>(), tw, conv, iter);
+    if (fork && c->fork_stage == 2) PGM_TRY((*fork)());
     k_pme_x_conv<real><<<dim3(nblk((long long)f.K2 * f.K3h, PFFT_COLS), B), 256, pfft_xconv_lds<real>(f), st>>>(
         f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1, conv, iter);
     k_pme_bwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, c->cgrid.as<C>(), c->grid.as<real>(), tw, conv, iter);
@@ -837,9 +843,10 @@ template <typename real> static int fused_transform(pgm_ctx *c, const int *conv,
     return PGM_OK;
 }
 
+// fork: called once the spread (fork_stage 1) or the forward planes (fork_stage 2) are enqueued.
 template <typename real>
 static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter,
-                      double *vir_part = nullptr) {
+                      double *vir_part = nullptr, const std::function<int()> *fork = nullptr) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int P = c->s.order;
@@ -860,8 +867,10 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
         LAUNCH_CHECK();
         if (c->pme_fused && c->fused_ok && !energy && !vir_part) {  // merge + transforms + Gc fused (fft.h)
             timing_end(c, 1, t_sp, st);
-            return fused_transform<real>(c, conv, iter, st);
+            if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
+            return fused_transform<real>(c, conv, iter, st, fork);
         }
+        if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
         const int *K = c->s.grid;
         k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0] * B), 128, 0, st>>>(
             K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(), conv,
@@ -976,15 +985,22 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     hipStream_t st = c->stream;
     const int *conv = iter >= 0 ? &c->st.as<CGState>()->conv_it : nullptr;
     // fork: the row field on the side stream, concurrently with spread -> FFT -> convolve -> FFT
-    HIP_TRY(hipEventRecord(c->ev_fork, st));
-    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-    const int t_row = timing_begin(c, 0, c->side);
-    if (withq)
-        PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side)));
-    else
-        PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side)));
-    timing_end(c, 0, t_row, c->side);
-    HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    // (from the start, or after the spread / the forward planes: fork_stage)
+    const std::function<int()> fork = [&]() -> int {
+        HIP_TRY(hipEventRecord(c->ev_fork, st));
+        HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        const int t_row = timing_begin(c, 0, c->side);
+        if (withq)
+            PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side)));
+        else
+            PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side)));
+        timing_end(c, 0, t_row, c->side);
+        HIP_TRY(hipEventRecord(c->ev_join, c->side));
+        return PGM_OK;
+    };
+    const bool late = c->fork_stage > 0 && c->tiled && !c->ewald &&
+                      (c->fork_stage == 1 || (c->pme_fused && c->fused_ok));
+    if (!late) PGM_TRY(fork());
     const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
     if (c->ewald) {
         const int t_sp = timing_begin(c, 1, st);
@@ -1005,7 +1021,7 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
         timing_end(c, 2, t_ga, st);
         return PGM_OK;
     }
-    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter));
+    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter, nullptr, late ? &fork : nullptr));
     HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     const int t_ga = timing_begin(c, 2, st);
     PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter));
@@ -2745,6 +2761,24 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
         other_pme = ms[c->pme_fused ? 0 : 1];
         drop_graphs(c);
     }
+    // fork point of the row field in a whole CG field evaluation (row field beside the PME chain)
+    double best_fork = 1e300;
+    int fork_best = 0;
+    if (c->tiled && !c->ewald) {
+        PGM_TRY(prepare(c, pos));
+        PGM_TRY(dipoles());
+        const int stages = c->pme_fused && c->fused_ok ? 3 : 2;
+        for (int fs = 0; fs < stages; ++fs) {
+            c->fork_stage = fs;
+            double ms = 0;
+            PGM_TRY(time_ms([&] {
+                (void)field_eval<real>(c, c->vp.as<R4>(), false, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), -1);
+            }, ms));
+            if (ms < best_fork) best_fork = ms, fork_best = fs;
+        }
+        c->fork_stage = fork_best;
+        drop_graphs(c);
+    }
     c->rows_valid = false;
     PGM_TRY(prepare(c, pos));
     HIP_TRY(hipStreamSynchronize(st));
@@ -2758,6 +2792,8 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
     out[7] = (double)lays.size();
     out[8] = c->pme_fused && c->fused_ok ? 1.0 : 0.0;
     out[9] = other_pme;
+    out[10] = c->fork_stage;
+    out[11] = best_fork < 1e300 ? best_fork : 0.0;
     return PGM_OK;
 }
 
@@ -2770,6 +2806,18 @@ extern "C" int pgm_set_pme_transform(pgm_ctx *c, int fused) {
     }
     drop_graphs(c);
     c->pme_fused = fused != 0;
+    if (!c->pme_fused && c->fork_stage == 2) c->fork_stage = 1;
+    return PGM_OK;
+}
+
+extern "C" int pgm_set_row_fork(pgm_ctx *c, int stage) {
+    if (!c || stage < 0 || stage > 2) return PGM_ERR_ARG;
+    if (stage == 2 && !(c->pme_fused && c->fused_ok && c->tiled)) {
+        g_err = "pgm_set_row_fork: stage 2 (after the forward planes) needs the fused transform";
+        return PGM_ERR_ARG;
+    }
+    drop_graphs(c);
+    c->fork_stage = stage;
     return PGM_OK;
 }
 
@@ -2782,7 +2830,7 @@ extern "C" int pgm_pme_transform(const pgm_ctx *c, int *fused, int *available) {
 
 extern "C" int pgm_autotune(pgm_ctx *c, const double *pos, int reps, double *report) {
     if (!c || !pos || !report || reps <= 0) return PGM_ERR_ARG;
-    for (int k = 0; k < 10; ++k) report[k] = 0.0;
+    for (int k = 0; k < 12; ++k) report[k] = 0.0;
     return c->mixed ? autotune_impl<float>(c, pos, reps, report) : autotune_impl<double>(c, pos, reps, report);
 }
 

@@ -92,3 +92,25 @@ def test_fused_transform_on_beads():
         assert abs(int(its[b]) - r.iterations) <= 1, (b, its[b], r.iterations)
         assert abs(per[b]["total"] - r.energy["total"]) <= 1e-7 * abs(r.energy["total"])
         assert _rel_rms(forces[b].cpu().numpy(), r.forces.cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_row_fork_point_changes_nothing(precision):
+    """The row field's start on the side stream (before / after the spread, after the forward planes)
+    only moves the overlap: fields, iterations and energies are bitwise the same."""
+    import pgm_jax_amd as P
+
+    top, pos, H = systems.water_box(512, 2.483, seed=9)
+    ff = P.PGMForceField(_system(top), H, P.MDSettings(precision=precision, cutoff=0.9, ewald_beta=4.0,
+                                                       pme_grid=(32, 32, 32), pme_order=6, dipole_tol=1e-7))
+    d = np.random.default_rng(5).normal(scale=0.01, size=pos.shape)
+    out = []
+    for stage in ("start", "after_spread", "after_forward_planes"):
+        ff.set_row_fork(stage)
+        r = ff.compute(pos, H)
+        out.append((ff.field(pos, d).cpu().numpy(), r.iterations, r.energy["total"]))
+    for f, it, e in out[1:]:
+        assert np.array_equal(f, out[0][0]) and it == out[0][1] and e == out[0][2]
+    ff.set_pme_transform("library")
+    with pytest.raises(P.PGMError):
+        ff.set_row_fork("after_forward_planes")
