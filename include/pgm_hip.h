@@ -164,8 +164,8 @@ int pgm_pme_transform(const pgm_ctx *ctx, int *fused, int *available);
 
 /* Where a CG field evaluation starts the row field on its side stream (no counterpart in the
  * reference, whose field is one XLA program): 0 before the PME spread, 1 after it, 2 after the
- * forward planes of the fused transform.  Only the overlap changes, not the results; the autotuner
- * picks the fastest. */
+ * forward planes of the fused transform, 3 before the spread with the forward planes waiting for
+ * the row field.  Only the overlap changes, not the results; the autotuner picks the fastest. */
 int pgm_set_row_fork(pgm_ctx *ctx, int stage);
 
 /* Autotuner (no counterpart in the reference): times, on this context's system at positions pos
@@ -175,8 +175,8 @@ int pgm_set_row_fork(pgm_ctx *ctx, int stage);
  * side stream (before the spread, after it, after the forward planes of the fused transform),
  * `reps` launches each, and keeps the fastest.  report (host, 12 doubles): layout, lanes, spread
  * tile width per axis (3), row-field ms, spread + transform + gather ms, layouts tried, fused
- * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0, 1, 2),
- * whole field-evaluation ms.  Synchronises. */
+ * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0..3,
+ * pgm_set_row_fork), whole field-evaluation ms.  Synchronises. */
 int pgm_autotune(pgm_ctx *ctx, const double *pos, int reps, double *report);
 
 /* PGMForceField._energy_forces / energy_fixed_mu (forcefield.py:2249-2292, :2362-2430): energy

@@ -165,7 +165,8 @@ struct pgm_ctx {
     // tiled spreading; pme_fused selects it over the library plans
     bool fused_ok = false, pme_fused = false;
     // where a CG field evaluation forks the row field onto the side stream: 0 before the spread,
-    // 1 after the spread, 2 after the forward planes (fused transform only); chosen by pgm_autotune
+    // 1 after the spread, 2 after the forward planes, 3 before the spread with the forward planes
+    // waiting for it (2 and 3: fused transform only); chosen by pgm_autotune
     int fork_stage = 0;
     PFFTGeom pfg{};
     DevBuf pfft_tw;
@@ -833,6 +834,7 @@ static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st
     const size_t plane = pfft_plane_lds<real>(f);
     const int nth = f.K2 * f.K3 >= 8192 ? pfft_plane_threads<real>() : 256;
     const C *tw = c->pfft_tw.as<C>();
+    if (fork && c->fork_stage == 3) HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));  // planes after the row field
     k_pme_fwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, (size_t)c->tiles8.ntiles * c->tiles8.bsize, c->merge_tab.as<int>(),
                                                          c->blocks.as<real>(), c->cgrid.as<C>(), tw, conv, iter);
     if (fork && c->fork_stage == 2) PGM_TRY((*fork)());
@@ -998,8 +1000,9 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
         HIP_TRY(hipEventRecord(c->ev_join, c->side));
         return PGM_OK;
     };
-    const bool late = c->fork_stage > 0 && c->tiled && !c->ewald &&
-                      (c->fork_stage == 1 || (c->pme_fused && c->fused_ok));
+    // stage 3: forked at the start, joined before the forward planes (the row field beside the spread)
+    const bool fused = c->tiled && !c->ewald && c->pme_fused && c->fused_ok;
+    const bool late = c->tiled && !c->ewald && (c->fork_stage == 1 || (c->fork_stage == 2 && fused));
     if (!late) PGM_TRY(fork());
     const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
     if (c->ewald) {
@@ -1021,7 +1024,7 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
         timing_end(c, 2, t_ga, st);
         return PGM_OK;
     }
-    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter, nullptr, late ? &fork : nullptr));
+    PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter, nullptr, late || (fused && c->fork_stage == 3) ? &fork : nullptr));
     HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     const int t_ga = timing_begin(c, 2, st);
     PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter));
@@ -2767,7 +2770,7 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
     if (c->tiled && !c->ewald) {
         PGM_TRY(prepare(c, pos));
         PGM_TRY(dipoles());
-        const int stages = c->pme_fused && c->fused_ok ? 3 : 2;
+        const int stages = c->pme_fused && c->fused_ok ? 4 : 2;
         for (int fs = 0; fs < stages; ++fs) {
             c->fork_stage = fs;
             double ms = 0;
@@ -2806,14 +2809,14 @@ extern "C" int pgm_set_pme_transform(pgm_ctx *c, int fused) {
     }
     drop_graphs(c);
     c->pme_fused = fused != 0;
-    if (!c->pme_fused && c->fork_stage == 2) c->fork_stage = 1;
+    if (!c->pme_fused && c->fork_stage >= 2) c->fork_stage = c->fork_stage == 2 ? 1 : 0;
     return PGM_OK;
 }
 
 extern "C" int pgm_set_row_fork(pgm_ctx *c, int stage) {
-    if (!c || stage < 0 || stage > 2) return PGM_ERR_ARG;
-    if (stage == 2 && !(c->pme_fused && c->fused_ok && c->tiled)) {
-        g_err = "pgm_set_row_fork: stage 2 (after the forward planes) needs the fused transform";
+    if (!c || stage < 0 || stage > 3) return PGM_ERR_ARG;
+    if (stage >= 2 && !(c->pme_fused && c->fused_ok && c->tiled)) {
+        g_err = "pgm_set_row_fork: stages 2 and 3 need the fused transform";
         return PGM_ERR_ARG;
     }
     drop_graphs(c);

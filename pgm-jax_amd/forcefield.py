@@ -97,6 +97,9 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
     return ctx, stream
 
 
+FORKS = ("start", "after_spread", "after_forward_planes", "start_join_before_planes")
+
+
 def _autotune(ctx, pos, reps: int) -> dict:
     """pgm_autotune on a context; return the chosen shapes and their timings."""
     rep = (C.c_double * 12)()
@@ -104,7 +107,7 @@ def _autotune(ctx, pos, reps: int) -> dict:
     return {"row_layout": ("atom", "molecule")[int(rep[0])], "row_lanes": int(rep[1]),
             "spread_tile": tuple(int(x) for x in rep[2:5]), "row_field_ms": rep[5], "pme_ms": rep[6],
             "layouts_tried": int(rep[7]), "pme_transform": ("library", "fused")[int(rep[8])],
-            "other_transform_ms": rep[9], "row_fork": ("start", "after_spread", "after_forward_planes")[int(rep[10])],
+            "other_transform_ms": rep[9], "row_fork": FORKS[int(rep[10])],
             "field_eval_ms": rep[11]}
 
 
@@ -368,8 +371,9 @@ class PGMForceField:
         return _transform(self._ctx)
 
     def set_row_fork(self, stage: str = "start") -> None:
-        """Start the CG row field beside the PME chain at "start", "after_spread" or "after_forward_planes"."""
-        code = {"start": 0, "after_spread": 1, "after_forward_planes": 2}[stage]
+        """Start the CG row field beside the PME chain at "start", "after_spread", "after_forward_planes" or
+        "start_join_before_planes" (the forward planes wait for it)."""
+        code = FORKS.index(stage)
         _lib.check(_lib.lib().pgm_set_row_fork(self._ctx, code), "pgm_set_row_fork")
 
     def row_layout(self) -> tuple[str, int]:

@@ -96,8 +96,8 @@ def test_fused_transform_on_beads():
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_row_fork_point_changes_nothing(precision):
-    """The row field's start on the side stream (before / after the spread, after the forward planes)
-    only moves the overlap: fields, iterations and energies are bitwise the same."""
+    """The row field's start on the side stream (before / after the spread, after the forward planes,
+    joined before them) only moves the overlap: fields, iterations and energies are bitwise the same."""
     import pgm_jax_amd as P
 
     top, pos, H = systems.water_box(512, 2.483, seed=9)
@@ -105,7 +105,7 @@ def test_row_fork_point_changes_nothing(precision):
                                                        pme_grid=(32, 32, 32), pme_order=6, dipole_tol=1e-7))
     d = np.random.default_rng(5).normal(scale=0.01, size=pos.shape)
     out = []
-    for stage in ("start", "after_spread", "after_forward_planes"):
+    for stage in ("start", "after_spread", "after_forward_planes", "start_join_before_planes"):
         ff.set_row_fork(stage)
         r = ff.compute(pos, H)
         out.append((ff.field(pos, d).cpu().numpy(), r.iterations, r.energy["total"]))
