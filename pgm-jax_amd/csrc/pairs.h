@@ -553,11 +553,18 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 // kept out of the common path: the closed form (both screenings with (a r)^2, (beta r)^2 >= 1) is
 // evaluated for every pair at once, the other pairs are marked and then evaluated one per lane per
 // round, and the own molecule's record (J == I, six close pairs) is computed by six lanes together.
+//
+// With rows != nullptr the same pass also writes the compacted atom rows of the force pass (what
+// k_compact_rows makes from the full atom rows): for atom 3I + p every 3J + k with r^2 < rc^2,
+// the near pairs (r^2 < 1.05 * 2 (R_i^2 + R_j^2) or beta^2 r^2 < 1.05, FP64) from the front of the
+// row and the others from the back, in (k, candidate) order per wave chunk; count / count_near
+// as k_compact_rows gives them.
 template <typename real>
 __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
                                                      int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
                                                      double rc2, real beta, int *__restrict__ mj, real *__restrict__ mg,
-                                                     int *__restrict__ mcount) {
+                                                     int *__restrict__ mcount, int *__restrict__ rows, int acap,
+                                                     int *__restrict__ acount, int *__restrict__ acount_near, double beta2) {
     const int lane = threadIdx.x % WAVE;
     const int I = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (I >= nmol) return;
@@ -565,11 +572,13 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
     const int cf = min(mcount_full[I], mcap_full);
     const int *cand = mrows + (size_t)I * mcap_full;
     int kept = 0, self_slot = -1;
+    int on[3] = {0, 0, 0}, of[3] = {0, 0, 0};  // atom rows: near / far entries so far
+    const unsigned long long below = (1ull << lane) - 1ull;
     for (int e0 = 0; e0 < cf; e0 += WAVE) {
         const int e = e0 + lane;
         bool keep = false, own = false;
         int J = 0, code = MOL_OWN_IMAGES;
-        unsigned near = 0u;
+        unsigned near = 0u, akeep = 0u, anear = 0u;
         real G1[9], G2[9];
         double4 b[3];
 #pragma unroll
@@ -591,8 +600,11 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                     double dx = a[p].x - b[k].x, dy = a[p].y - b[k].y, dz = a[p].z - b[k].z;
                     int n[3];
                     min_image_n(dx, dy, dz, box, n);
-                    if (dx * dx + dy * dy + dz * dz >= rc2) continue;
+                    const double r2d = dx * dx + dy * dy + dz * dz;
+                    if (r2d >= rc2) continue;
                     keep = true;
+                    akeep |= 1u << (3 * p + k);
+                    if (r2d < 1.05 * 2.0 * (a[p].w + b[k].w) || beta2 * r2d < 1.05) anear |= 1u << (3 * p + k);
                     if (own) continue;  // the own molecule's pairs come after the compaction
                     if (first) {
                         n0[0] = n[0], n0[1] = n[1], n0[2] = n[2];
@@ -636,7 +648,23 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                     }
             }
         }
-        const unsigned long long below = (1ull << lane) - 1ull;
+        if (rows) {  // atom rows of the force pass
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                int *rw = rows + (size_t)(3 * I + p) * acap;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const unsigned bit = 1u << (3 * p + k);
+                    const bool kn = (akeep & bit) && (anear & bit), kf = (akeep & bit) && !(anear & bit);
+                    const unsigned long long mn = __ballot(kn), mf = __ballot(kf);
+                    const int pn = on[p] + __popcll(mn & below), pf = of[p] + __popcll(mf & below);
+                    if (kn && pn + of[p] < acap) rw[pn] = 3 * J + k;  // the far part starts at acap - of
+                    if (kf && on[p] + pf < acap) rw[acap - 1 - pf] = 3 * J + k;
+                    on[p] += __popcll(mn);
+                    of[p] += __popcll(mf);
+                }
+            }
+        }
         const unsigned long long m = __ballot(keep);
         const int slot_e = kept + __popcll(m & below);
         if (keep && slot_e < mcap) {
@@ -678,6 +706,12 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
         }
     }
     if (lane == 0) mcount[I] = kept < mcap ? kept : mcap;
+    if (rows && lane < 3) {
+        const int o = on[lane], f = of[lane];
+        const int tot = o + f < acap ? o + f : acap;
+        acount_near[3 * I + lane] = o < tot ? o : tot;
+        acount[3 * I + lane] = tot;
+    }
 }
 
 // Row field of the molecule blocks: out_i for the three atoms of I (the same sum as k_row_field over
