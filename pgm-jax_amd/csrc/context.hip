@@ -759,21 +759,20 @@ static int prepare(pgm_ctx *c, const double *pos) {
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->s.cutoff * c->s.cutoff;
-        // the same pass compacts the atom rows of the force pass (k_compact_rows not needed)
+        // FP32: the same pass compacts the atom rows of the force pass (k_compact_rows not needed)
         const double beta2 = c->s.ewald_beta * c->s.ewald_beta;
         if (c->mixed)
-            k_mol_tensors<float><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
+            k_mol_tensors<float, true><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
                                                                c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2,
                                                                (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(),
                                                                c->mcount.as<int>(), c->rows.as<int>(), c->cap,
                                                                c->count.as<int>(), c->count_near.as<int>(), beta2);
         else
-            k_mol_tensors<double><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
-                                                                c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2, c->s.ewald_beta,
-                                                                c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(),
-                                                                c->rows.as<int>(), c->cap, c->count.as<int>(),
-                                                                c->count_near.as<int>(), beta2);
-        c->atom_rows_fresh = true;
+            k_mol_tensors<double, false><<<nblk(nmol, 4), 256, 0, st>>>(
+                nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full, c->mcap, c->rlm, c->xr.as<double4>(), c->box,
+                rc2, c->s.ewald_beta, c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(), nullptr, 0, nullptr, nullptr,
+                beta2);
+        c->atom_rows_fresh = c->mixed;
     } else if (c->mixed)
         k_pair_tensors<float><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,

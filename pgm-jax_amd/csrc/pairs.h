@@ -554,12 +554,13 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 // evaluated for every pair at once, the other pairs are marked and then evaluated one per lane per
 // round, and the own molecule's record (J == I, six close pairs) is computed by six lanes together.
 //
-// With rows != nullptr the same pass also writes the compacted atom rows of the force pass (what
+// With ROWS the same pass also writes the compacted atom rows of the force pass (what
 // k_compact_rows makes from the full atom rows): for atom 3I + p every 3J + k with r^2 < rc^2,
 // the near pairs (r^2 < 1.05 * 2 (R_i^2 + R_j^2) or beta^2 r^2 < 1.05, FP64) from the front of the
 // row and the others from the back, in (k, candidate) order per wave chunk; count / count_near
-// as k_compact_rows gives them.
-template <typename real>
+// as k_compact_rows gives them.  (Used for FP32; the FP64 variant keeps its registers for the
+// tensors and leaves the atom rows to k_compact_rows.)
+template <typename real, bool ROWS>
 __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
                                                      int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
                                                      double rc2, real beta, int *__restrict__ mj, real *__restrict__ mg,
@@ -603,8 +604,10 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                     const double r2d = dx * dx + dy * dy + dz * dz;
                     if (r2d >= rc2) continue;
                     keep = true;
-                    akeep |= 1u << (3 * p + k);
-                    if (r2d < 1.05 * 2.0 * (a[p].w + b[k].w) || beta2 * r2d < 1.05) anear |= 1u << (3 * p + k);
+                    if (ROWS) {
+                        akeep |= 1u << (3 * p + k);
+                        if (r2d < 1.05 * 2.0 * (a[p].w + b[k].w) || beta2 * r2d < 1.05) anear |= 1u << (3 * p + k);
+                    }
                     if (own) continue;  // the own molecule's pairs come after the compaction
                     if (first) {
                         n0[0] = n[0], n0[1] = n[1], n0[2] = n[2];
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                     }
             }
         }
-        if (rows) {  // atom rows of the force pass
+        if (ROWS) {  // atom rows of the force pass
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 int *rw = rows + (size_t)(3 * I + p) * acap;
@@ -706,7 +709,7 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
         }
     }
     if (lane == 0) mcount[I] = kept < mcap ? kept : mcap;
-    if (rows && lane < 3) {
+    if (ROWS && lane < 3) {
         const int o = on[lane], f = of[lane];
         const int tot = o + f < acap ? o + f : acap;
         acount_near[3 * I + lane] = o < tot ? o : tot;
