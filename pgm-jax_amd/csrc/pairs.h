@@ -107,8 +107,12 @@ __global__ void k_build_rows(int n, int nper, const double *__restrict__ pos, Bo
 // The same rows built cell by cell: one workgroup per cell (bead * ncell + cell, atoms in sorted
 // order), its atoms the owners (one thread each, in chunks of blockDim), the neighbouring cells'
 // positions staged through LDS in chunks of blockDim and tested by every owner.  Cells and atoms are
-// visited in k_build_rows's order, so the rows are identical; each position is read once per
-// workgroup instead of once per owner.
+// visited in k_build_rows's order; each position is read once per workgroup instead of once per
+// owner.  With at least three cells per axis (cell heights >= the list cutoff) the only image of j
+// within the cutoff of i lies in the neighbour cell as visited, so positions are wrapped into the
+// box (r - floor(r H^-1) H, the cell's own fractional coordinates) and the neighbour cell's lattice
+// shift is added when staging: one subtraction per pair instead of a minimum image.  Fewer cells
+// per axis keep the per-pair minimum image.
 __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const double *__restrict__ pos, Box box, int3 nc,
                                                           const int *__restrict__ satom, const int *__restrict__ cstart,
                                                           const int *__restrict__ cend, double rc2, int cap,
@@ -128,16 +132,26 @@ __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const 
         lo[a] = ncs[a] >= 3 ? -1 : 0;
         hiO[a] = ncs[a] >= 3 ? 1 : ncs[a] - 1;
     }
+    const bool shifted = ncs[0] >= 3 && ncs[1] >= 3 && ncs[2] >= 3;
+    // position wrapped into the box: r - sum_a floor(u_a) h_a, u = r H^-1
+    auto wrapped = [&](int j, double &x, double &y, double &z) {
+        x = pos[3 * j];
+        y = pos[3 * j + 1];
+        z = pos[3 * j + 2];
+        if (!shifted) return;
+        const double *hi = box.hinv, *h = box.h;
+        const double n0 = floor(x * hi[0] + y * hi[3] + z * hi[6]), n1 = floor(x * hi[1] + y * hi[4] + z * hi[7]),
+                     n2 = floor(x * hi[2] + y * hi[5] + z * hi[8]);
+        x -= n0 * h[0] + n1 * h[3] + n2 * h[6];
+        y -= n0 * h[1] + n1 * h[4] + n2 * h[7];
+        z -= n0 * h[2] + n1 * h[5] + n2 * h[8];
+    };
     for (int o0s = own0; o0s < own1; o0s += blockDim.x) {
         const int s = o0s + threadIdx.x;
         const bool owner = s < own1;
         const int i = owner ? satom[s] : 0;
         double xi = 0.0, yi = 0.0, zi = 0.0;
-        if (owner) {
-            xi = pos[3 * i];
-            yi = pos[3 * i + 1];
-            zi = pos[3 * i + 2];
-        }
+        if (owner) wrapped(i, xi, yi, zi);
         int cnt = 0;
         int *row = rows + (size_t)i * cap;
         for (int q0 = lo[0]; q0 <= hiO[0]; ++q0) {
@@ -148,28 +162,53 @@ __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const 
                     const int c2 = (c[2] + q2 + ncs[2]) % ncs[2];
                     const int nb = bead * ncell_bead + (c0 * nc.y + c1) * nc.z + c2;
                     const int s0 = cstart[nb], s1 = cend[nb];
+                    // lattice shift of the neighbour cell as visited (wrap across a box face)
+                    double shx = 0.0, shy = 0.0, shz = 0.0;
+                    if (shifted) {
+                        const int q[3] = {q0, q1, q2};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const int w = c[a] + q[a] < 0 ? -1 : (c[a] + q[a] >= ncs[a] ? 1 : 0);
+                            shx += w * box.h[3 * a];
+                            shy += w * box.h[3 * a + 1];
+                            shz += w * box.h[3 * a + 2];
+                        }
+                    }
                     for (int t0 = s0; t0 < s1; t0 += blockDim.x) {
                         const int m = min((int)blockDim.x, s1 - t0);
                         __syncthreads();
                         if ((int)threadIdx.x < m) {
                             const int j = satom[t0 + threadIdx.x];
+                            double x, y, z;
+                            wrapped(j, x, y, z);
                             sj[threadIdx.x] = j;
-                            sx[threadIdx.x] = pos[3 * j];
-                            sy[threadIdx.x] = pos[3 * j + 1];
-                            sz[threadIdx.x] = pos[3 * j + 2];
+                            sx[threadIdx.x] = x + shx;
+                            sy[threadIdx.x] = y + shy;
+                            sz[threadIdx.x] = z + shz;
                         }
                         __syncthreads();
-                        if (owner)
-                            for (int u = 0; u < m; ++u) {
-                                const int j = sj[u];
-                                if (j == i) continue;
-                                double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
-                                min_image(dx, dy, dz, box);
-                                if (dx * dx + dy * dy + dz * dz < rc2) {
-                                    if (cnt < cap) row[cnt] = j;
-                                    ++cnt;
+                        if (owner) {
+                            if (shifted) {
+                                for (int u = 0; u < m; ++u) {
+                                    const double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
+                                    if (dx * dx + dy * dy + dz * dz < rc2 && sj[u] != i) {
+                                        if (cnt < cap) row[cnt] = sj[u];
+                                        ++cnt;
+                                    }
+                                }
+                            } else {
+                                for (int u = 0; u < m; ++u) {
+                                    const int j = sj[u];
+                                    if (j == i) continue;
+                                    double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
+                                    min_image(dx, dy, dz, box);
+                                    if (dx * dx + dy * dy + dz * dz < rc2) {
+                                        if (cnt < cap) row[cnt] = j;
+                                        ++cnt;
+                                    }
                                 }
                             }
+                        }
                     }
                 }
             }
