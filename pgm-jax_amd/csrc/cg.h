@@ -275,41 +275,63 @@ __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__re
 // M_I^-1 (81 reals, row major) per molecule by Gauss-Jordan in FP64, remade with the neighbour rows
 // (the molecules turn little in between, and any such blocks are a valid preconditioner).
 template <typename real>
-__global__ void k_mol_precond(int nmol, const double4 *__restrict__ xr, const double *__restrict__ alpha, Box box,
+__global__ __launch_bounds__(128) void k_mol_precond(int nmol, const double4 *__restrict__ xr, const double *__restrict__ alpha, Box box,
                               real *__restrict__ minv) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nmol) return;
-    double M[9][18];
+    // every loop unrolled, so the 9 x 9 matrix stays in registers (no scratch)
+    double M[9][9];
+#pragma unroll
     for (int r = 0; r < 9; ++r)
-        for (int c = 0; c < 18; ++c) M[r][c] = (c == r + 9) ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) M[r][c] = 0.0;
+    double4 at[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) at[p] = xr[3 * I + p];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const double ia = 1.0 / alpha[3 * I + p];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[3 * p + c][3 * p + c] = ia;
+    }
+#pragma unroll
     for (int p = 0; p < 3; ++p)
-        for (int c = 0; c < 3; ++c) M[3 * p + c][3 * p + c] = 1.0 / alpha[3 * I + p];
-    for (int p = 0; p < 3; ++p)
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (p == k) continue;
-            const double4 a = xr[3 * I + p], b = xr[3 * I + k];
+            const double4 a = at[p], b = at[k];
             double x[3] = {a.x - b.x, a.y - b.y, a.z - b.z};
             min_image(x[0], x[1], x[2], box);
             const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
             const double ir = 1.0 / sqrt(r2), r = r2 * ir;
             double B[3];
             one_B<double, 3>(1.0 / sqrt(2.0 * (a.w + b.w)), r, ir, ir * ir, B);
+#pragma unroll
             for (int u = 0; u < 3; ++u)
+#pragma unroll
                 for (int v = 0; v < 3; ++v) M[3 * p + u][3 * k + v] = (u == v ? B[1] : 0.0) - B[2] * x[u] * x[v];
         }
-    for (int col = 0; col < 9; ++col) {  // Gauss-Jordan (M is symmetric positive definite: no pivoting)
+    // in-place Gauss-Jordan inversion (M symmetric positive definite: no pivoting)
+#pragma unroll
+    for (int col = 0; col < 9; ++col) {
         const double d = 1.0 / M[col][col];
-        for (int c = 0; c < 18; ++c) M[col][c] *= d;
+        M[col][col] = 1.0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) M[col][c] *= d;
+#pragma unroll
         for (int r = 0; r < 9; ++r) {
             if (r == col) continue;
             const double f = M[r][col];
-            if (f != 0.0)
-                for (int c = 0; c < 18; ++c) M[r][c] -= f * M[col][c];
+            M[r][col] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) M[r][c] -= f * M[col][c];
         }
     }
     real *out = minv + (size_t)81 * I;
+#pragma unroll
     for (int r = 0; r < 9; ++r)
-        for (int c = 0; c < 9; ++c) out[9 * r + c] = (real)M[r][9 + c];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) out[9 * r + c] = (real)M[r][c];
 }
 
 // z of atom p (0..2) of the molecule whose nine residual components are sr[0..8]: rows 3p..3p+2 of M^-1.

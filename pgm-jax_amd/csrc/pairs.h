@@ -546,14 +546,30 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const int m0 = (I / nmol_bead) * nmol_bead;
+    // the three rows in rounds of four entries per lane and atom, all twelve loads issued first
+    int cnt3[3], cmax = 0;
+#pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const int i = 3 * I + p;
-        const int c = min(count_full[i], cap_full);
-        const int *row = rows_full + (size_t)i * cap_full;
-        for (int e = lane; e < c; e += WAVE) {
-            const int J = row[e] / 3 - m0;
-            atomicOr(&bits[J >> 5], 1u << (J & 31));
-        }
+        cnt3[p] = min(count_full[3 * I + p], cap_full);
+        cmax = max(cmax, cnt3[p]);
+    }
+    for (int e0 = 0; e0 < cmax; e0 += 4 * WAVE) {
+        int v[3][4];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + lane + u * WAVE;
+                v[p][u] = e < cnt3[p] ? rows_full[(size_t)(3 * I + p) * cap_full + e] : -1;
+            }
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (v[p][u] >= 0) {
+                    const int J = v[p][u] / 3 - m0;
+                    atomicOr(&bits[J >> 5], 1u << (J & 31));
+                }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
