@@ -179,7 +179,7 @@ __global__ void k_key_bounds(int n, const int *__restrict__ skey, int shift, int
 // cover the P^3 stencil points of one atom at a time, so no two lanes touch one point and the sum
 // is free of atomics and bitwise reproducible (an LDS float add, ds_add_f32, in place of the read and
 // write measured three times slower).
-constexpr int SPREAD_CHUNK = 16;  // atoms staged at a time (small: the LDS per wave bounds the occupancy)
+constexpr int SPREAD_CHUNK = 8;  // atoms staged at a time: 10 KB of LDS per wave, 16 waves (all 8^3 tiles of 128^3) per CU
 
 template <typename real, int P>
 PGM_DEV constexpr int spread_lds_atoms() {
