@@ -123,6 +123,7 @@ struct pgm_ctx {
     double sum_q = 0, sum_q2 = 0;
     double lj_sb = 0;  // sum_{i,j} 2 eps_ij rmin_ij^6 (LJ tail)
     DevBuf q, alpha64, alpha_r, rh, se, mol, R2;
+    DevBuf ljp;  // (rmin/2, sqrt eps, molecule, 0) per atom, double4 (the force pass)
     DevBuf cstart, ci, cj, cc, jstart, jlist;
     // geometry: the positions of the current evaluation (caller's device buffer) and xr
     const double *pos = nullptr;
@@ -1520,13 +1521,13 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
         LAUNCH_CHECK();
         k_pair_forces<real, true><<<gpair, 256, 0, st>>>(
             nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-            (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
+            (real)c->s.ewald_beta, c->ljp.as<double4>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), c->off.as<double>(),
             c->vir_pair.as<double>());
     } else {
         k_pair_forces<real, false><<<gpair, 256, 0, st>>>(
             nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-            (real)c->s.ewald_beta, c->rh.as<double>(), c->se.as<double>(), c->mol.as<int>(), c->g_el.as<double>(),
+            (real)c->s.ewald_beta, c->ljp.as<double4>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
     }
     LAUNCH_CHECK();
@@ -1691,6 +1692,16 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = upload(c->rh, c->h_rh.data(), n, st)) < 0) return fail(r);
     if ((r = upload(c->se, c->h_se.data(), n, st)) < 0) return fail(r);
     if ((r = upload(c->mol, c->h_mol.data(), n, st)) < 0) return fail(r);
+    {  // (rmin/2, sqrt eps, molecule, 0) per atom for the force pass
+        std::vector<double> lj(4 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            lj[4 * (size_t)i] = c->h_rh[i];
+            lj[4 * (size_t)i + 1] = c->h_se[i];
+            lj[4 * (size_t)i + 2] = (double)c->h_mol[i];
+            lj[4 * (size_t)i + 3] = 0.0;
+        }
+        if ((r = upload(c->ljp, lj.data(), lj.size(), st)) < 0) return fail(r);
+    }
     if ((r = upload(c->R2, c->h_R2.data(), n, st)) < 0) return fail(r);
     {  // molecules as CSR over atom indices (for centres of mass)
         int mmax = -1;
@@ -1861,7 +1872,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (c->fwd) rocfft_plan_destroy(c->fwd);
     if (c->bwd) rocfft_plan_destroy(c->bwd);
     if (c->info) rocfft_execution_info_destroy(c->info);
-    DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
+    DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->ljp, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
                      &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,

@@ -383,7 +383,9 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
 //   e_ik  = q_i q_k G0 + (q_i d_k.x - q_k d_i.x) G1 - G2 (d_i.x)(d_k.x) + G1 d_i.d_k
 //   de/dx = x [-q_i q_k G1 - t G2 + G3 (d_i.x)(d_k.x) - G2 d_i.d_k] + q_i G1 d_k - q_k G1 d_i
 //           - G2 (d_i (d_k.x) + d_k (d_i.x)),   t = q_i d_k.x - q_k d_i.x
-//   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2.
+//   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2,
+//   one FP64 reciprocal of r^2 per pair; the partner's (rmin/2, sqrt eps, molecule) in one 32-byte
+//   load (ljp).
 // Outputs (FP64): g_el = sum de/dx, g_lj, the row field sum de/dd_i, and per-block energy sums
 // (each pair counted in both rows: the caller halves them).  With VIRIAL, also the per-block sums
 // of (KE de/dx + de_LJ/dx) (x) X, X = x - o_i + o_k with o the offsets of the atoms from their
@@ -417,15 +419,16 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
             fx += (double)(cfield * x + G[1] * dk.x); \
             fy += (double)(cfield * y + G[1] * dk.y); \
             fz += (double)(cfield * z + G[1] * dk.z); \
-            const double eps = sei * se[j]; \
+            const double4 ljk = ljp[j]; \
+            const double eps = sei * ljk.y; \
             double l0 = 0.0, l1 = 0.0, l2 = 0.0; \
-            if (mol[j] != mi && eps != 0.0) { \
-                const double rmin = rhi + rh[j]; \
-                const double rr2 = dx * dx + dy * dy + dz * dz; \
-                const double s2 = rmin * rmin / rr2; \
+            if ((int)ljk.z != mi && eps != 0.0) { \
+                const double rmin = rhi + ljk.x; \
+                const double ir2 = 1.0 / (dx * dx + dy * dy + dz * dz); \
+                const double s2 = rmin * rmin * ir2; \
                 const double s6 = s2 * s2 * s2; \
                 el += eps * (s6 * s6 - 2.0 * s6); \
-                const double dl = eps * 12.0 * (s6 - s6 * s6) / rr2; \
+                const double dl = eps * 12.0 * (s6 - s6 * s6) * ir2; \
                 l0 = dl * dx; \
                 l1 = dl * dy; \
                 l2 = dl * dz; \
@@ -445,8 +448,7 @@ template <typename real, bool VIRIAL>
 __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__restrict__ rows, const int *__restrict__ count,
                                                      const int *__restrict__ count_near, int cap,
                                                      const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
-                                                     real beta, const double *__restrict__ rh, const double *__restrict__ se,
-                                                     const int *__restrict__ mol, double *__restrict__ g_el,
+                                                     real beta, const double4 *__restrict__ ljp, double *__restrict__ g_el,
                                                      double *__restrict__ g_lj, double *__restrict__ rowf,
                                                      double *__restrict__ epart, const double *__restrict__ off,
                                                      double *__restrict__ wpart) {
@@ -461,8 +463,9 @@ __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__rest
         const double4 a = xr[i];
         const real4<real> di = dq[i];
         const real qi = di.w;
-        const double rhi = rh[i], sei = se[i];
-        const int mi = mol[i];
+        const double4 lji = ljp[i];
+        const double rhi = lji.x, sei = lji.y;
+        const int mi = (int)lji.z;
         const int c = count[i];
         const int *row = rows + (size_t)i * cap;
         double oi[3] = {0.0, 0.0, 0.0};
