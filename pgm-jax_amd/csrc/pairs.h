@@ -213,10 +213,14 @@ __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const 
                 }
             }
         }
-        if (owner) {
-            count[i] = cnt;
-            atomicMax(maxcount, cnt);
-        }
+        if (owner) count[i] = cnt;
+        // the largest count: one atomic per wave, and only above the value already there (a
+        // same-address atomic per atom serialises in one L2 channel)
+        int m = owner ? cnt : 0;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, WAVE));
+        if (threadIdx.x % WAVE == 0 && m > __hip_atomic_load(maxcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(maxcount, m);
     }
 }
 
@@ -565,14 +569,17 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
                 const int e = e0 + lane + u * WAVE;
                 v[p][u] = e < cnt3[p] ? rows_full[(size_t)(3 * I + p) * cap_full + e] : -1;
             }
+        // consecutive entries of a row are often the atoms of one molecule (and its neighbours in
+        // index): a lane skips a molecule its left neighbour sets, which keeps most same-word LDS
+        // atomics out of one instruction
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (v[p][u] >= 0) {
-                    const int J = v[p][u] / 3 - m0;
-                    atomicOr(&bits[J >> 5], 1u << (J & 31));
-                }
+            for (int u = 0; u < 4; ++u) {
+                const int J = v[p][u] >= 0 ? v[p][u] / 3 - m0 : -1;
+                const int Jl = __shfl_up(J, 1, WAVE);
+                if (J >= 0 && (lane == 0 || Jl != J)) atomicOr(&bits[J >> 5], 1u << (J & 31));
+            }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -600,7 +607,7 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
     }
     if (lane == 0) {
         mcount[I] = total;
-        atomicMax(maxcount, total);
+        if (total > __hip_atomic_load(maxcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(maxcount, total);
     }
 }
 
