@@ -950,7 +950,7 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
     const real cself = (real)c_self(c);
     if (c->tiled) {
         const PMETiles &t = c->tiles;
-        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<c->nbead * t.ntiles, 256, c->rsize * t.bsize, st>>>(
+        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<c->nbead * t.ntiles, GATHER_THREADS, c->rsize * t.bsize, st>>>(
                                      t, c->pg, c->tstart16.as<int>(), c->tend16.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
                                      c->loc16.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
                                      extra, cself, mode, out, part, conv, iter)));

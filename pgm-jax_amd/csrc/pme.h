@@ -426,14 +426,16 @@ PGM_DEV void combine_out(int mode, int i, real rx, real ry, real rz, const real4
     out[i] = o;
 }
 
-// Tiled gather + combine: one 256-thread workgroup per grid tile loads the potential on the
+// Tiled gather + combine: one 512-thread workgroup per grid tile loads the potential on the
 // tile's (T + P - 1)^3 block (periodic wrap) into LDS once, then every atom binned to the tile
-// (thread per atom, sorted order s, atom sidx[s]) sums its 216 stencil values from LDS with the
-// spline weights stored by k_pme_splines, and combine_out writes its result.  Per-block partials
+// (two neighbouring threads per atom, sorted order s, atom sidx[s]: the first (P+1)/2 x planes of
+// its stencil and the rest, summed with one shuffle) sums its stencil values from LDS with the
+// spline weights stored by k_pme_splines, and the first thread's combine_out writes its result.  Per-block partials
 // of MODE_FIELD / MODE_AV go to part[tile].  Block b * t.ntiles + tile serves tile `tile` of bead
 // b (potential at phi + b K1 K2 K3).
+constexpr int GATHER_THREADS = 512;
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather_tiles(PMETiles t, PMEGeom g, const int *__restrict__ tstart,
                                                       const int *__restrict__ tend, const int *__restrict__ sidx,
                                                       const real *__restrict__ spl, const int *__restrict__ loc,
                                                       const real *__restrict__ phi, const real4<real> *__restrict__ dq,
@@ -474,7 +476,9 @@ __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, con
     __syncthreads();
     const int s0 = tstart[tile], s1 = tend[tile];
     double acc[1] = {0.0};
-    for (int s = s0 + tid; s < s1; s += blockDim.x) {
+    const int half = tid & 1;
+    constexpr int PH = (P + 1) / 2;  // x planes of the first thread of a pair
+    for (int s = s0 + (tid >> 1); s < s1; s += blockDim.x >> 1) {  // both threads of a pair: same s
         const real *v = spl + (size_t)s * 6 * P;
         real t1[P], d1[P], t2[P], d2[P], t3[P], d3[P];
 #pragma unroll
@@ -490,7 +494,12 @@ __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, con
         const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = lc >> 20;
         real g1 = 0, g2 = 0, g3 = 0;
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) {
+        for (int ixh = 0; ixh < PH; ++ixh) {
+            const int ix = half ? PH + ixh : ixh;
+            if (ix >= P) break;
+            // spline weights picked with compile-time indices (registers, not scratch)
+            const int ih = PH + ixh < P ? PH + ixh : P - 1;
+            const real d1x = half ? d1[ih] : d1[ixh], t1x = half ? t1[ih] : t1[ixh];
             real s_tt = 0, s_dt = 0, s_td = 0;
 #pragma unroll
             for (int iy = 0; iy < P; ++iy) {
@@ -506,10 +515,14 @@ __global__ __launch_bounds__(256) void k_gather_tiles(PMETiles t, PMEGeom g, con
                 s_dt += lt * d2[iy];
                 s_td += ld * t2[iy];
             }
-            g1 += s_tt * d1[ix];
-            g2 += s_dt * t1[ix];
-            g3 += s_td * t1[ix];
+            g1 += s_tt * d1x;
+            g2 += s_dt * t1x;
+            g3 += s_td * t1x;
         }
+        g1 += __shfl_xor(g1, 1, WAVE);
+        g2 += __shfl_xor(g2, 1, WAVE);
+        g3 += __shfl_xor(g3, 1, WAVE);
+        if (half) continue;
         const double *es = g.escale;
         const real rx = (real)es[0] * g1 + (real)es[1] * g2 + (real)es[2] * g3;
         const real ry = (real)es[3] * g1 + (real)es[4] * g2 + (real)es[5] * g3;
