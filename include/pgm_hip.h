@@ -126,6 +126,13 @@ int pgm_pme_energy(pgm_ctx *ctx, const double *pos, int with_charges, const doub
 int pgm_compute(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces,
                 int *iterations, double *residual);
 
+/* PGMForceField.energy (forcefield.py:2562-2626), the Monte Carlo barostat's trial energy: the
+ * induced dipoles solved by CG from ind->mu (no predictor; the right-hand side and the
+ * convergence norm evaluated afresh), written back to ind->mu; ind->hist, count and norm are
+ * left as they are, as is the learned guess's record.  Energies only (no forces). */
+int pgm_trial_energy(pgm_ctx *ctx, const double *pos, pgm_induction *ind, pgm_energy *energy, int *iterations,
+                     double *residual);
+
 /* Beads (settings.nbead > 1; PGMBeads.compute over the force beads, pgm_jax/md/pimd.py:1104-1172,
  * with every bead's force field in the same launches instead of vmap / lax.map chunks,
  * pimd.py:1068-1102): positions, forces, mu and hist are (nbead, N, 3) with N = top->n_atoms; every

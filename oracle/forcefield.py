@@ -374,6 +374,24 @@ class ForceField:
             W += tv.T @ v
         return W
 
+    def energy(self, pos: np.ndarray, H: np.ndarray, ind: InductionState) -> tuple:
+        """Energy only, for Monte Carlo barostat trials (forcefield.py:2562-2626): CG from ind.mu (no
+        predictor; the field b and the norm mean|alpha b| evaluated afresh); returns (E_total, mu, it).
+        The state's history is not touched."""
+        pos = np.asarray(pos, float)
+        H = np.asarray(H, float)
+        g = self.geometry(pos, H)
+        p = self.perm_dipoles(pos, H)
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        A = self.operator(g, S, Gk)
+        b = self.field(g, S, Gk, self.top.q, p)
+        norm = float(np.mean(np.abs(self.top.alpha[:, None] * b))) + 1e-300
+        x0 = np.asarray(ind.mu, float)
+        mu, it, _ = self.cg(A, x0, b - A(x0), norm)
+        e = self.energy_forces(pos, H, mu, g, S, Gk)[0]["total"]
+        return e, mu, it
+
     def energy_fixed_mu(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray) -> float:
         """Return the total energy [kJ/mol] at fixed induced dipoles (forcefield.py:2249-2292)."""
         g = self.geometry(pos, H)

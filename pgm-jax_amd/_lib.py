@@ -31,6 +31,7 @@ EXPORTS = (
     "pgm_field",
     "pgm_pme_energy",
     "pgm_compute",
+    "pgm_trial_energy",
     "pgm_bead_count",
     "pgm_bead_results",
     "pgm_set_graphs",
@@ -179,6 +180,7 @@ def lib() -> C.CDLL:
     L.pgm_field.argtypes = [P, P, I, P, P]
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]
     L.pgm_compute.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), P, C.POINTER(I), C.POINTER(D)]
+    L.pgm_trial_energy.argtypes = [P, P, C.POINTER(Induction), C.POINTER(Energy), C.POINTER(I), C.POINTER(D)]
     L.pgm_bead_count.argtypes = [P]
     L.pgm_bead_results.argtypes = [P, P, P, P]
     L.pgm_set_graphs.argtypes = [P, I]

@@ -70,6 +70,19 @@ __global__ void k_predict(int n, const double *__restrict__ hist, double c0, dou
     x0[i] = r;
 }
 
+// x0 = mu (the initial guess of a trial solve: the dipoles passed in, no predictor).
+template <typename real>
+__global__ void k_mu_to_x0(int n, const double *__restrict__ mu, real4<real> *__restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    real4<real> r;
+    r.x = (real)mu[3 * i];
+    r.y = (real)mu[3 * i + 1];
+    r.z = (real)mu[3 * i + 2];
+    r.w = 0;
+    x0[i] = r;
+}
+
 // dq = (p + x0, q or 0): the dipoles and charges a field evaluation spreads.
 template <typename real>
 __global__ void k_pack_dq(int n, const double *__restrict__ p, const real4<real> *__restrict__ x0, const double *__restrict__ q,

@@ -1277,7 +1277,11 @@ static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
 // its own CG in the same launches; the host waits for the slowest, and iterations / residual are
 // the largest over the beads (per bead: bead_it, bead_res).  Bead normalisers stay on the device.
 template <typename real>
-static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual, bool quiet = false) {
+// trial: the CG starts from ind->mu (no predictor, the right-hand side and its norm evaluated
+// afresh), and neither the predictor history nor the learned guess's records change
+// (PGMForceField.energy, forcefield.py:2562-2626).
+static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *residual, bool quiet = false,
+                      bool trial = false) {
     const int n = c->n, B = c->nbead, nper = c->nper;
     hipStream_t st = c->stream;
     using R4 = real4<real>;
@@ -1299,7 +1303,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         case 3: cf[0] = 1; K = 1; break;
         default: K = 0;
     }
-    const bool learned = c->s.predictor == 4 && c->guess.on;
+    const bool learned = !trial && c->s.predictor == 4 && c->guess.on;
     if (c->s.predictor == 4) {  // learned guess; mu4 / previous-step until the histories are there
         if (ind->count >= 4) {
             K = 4;
@@ -1308,9 +1312,12 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
             K = 1;
         }
     }
-    const bool have = K > 0 && ind->count >= K;
+    const bool have = trial || (K > 0 && ind->count >= K);
     const bool use_net = learned && c->guess.nring >= GUESS_K;
-    if (use_net) {
+    if (trial) {
+        k_mu_to_x0<real><<<nb, 256, 0, st>>>(n, ind->mu, x0);
+        LAUNCH_CHECK();
+    } else if (use_net) {
         PGM_TRY(guess_x0<real>(c, c->pos, x0, ind->count % c->guess.every == 0));
     } else if (have) {
         if (c->s.predictor == 4 && K == 4) {
@@ -1319,7 +1326,7 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         k_predict<real><<<nb, 256, 0, st>>>(n, ind->hist, cf[0], cf[1], cf[2], cf[3], x0);
         LAUNCH_CHECK();
     }
-    const bool fused = have && c->s.fused && (c->s.norm_refresh <= 0 || ind->count % c->s.norm_refresh != 0) &&
+    const bool fused = !trial && have && c->s.fused && (c->s.norm_refresh <= 0 || ind->count % c->s.norm_refresh != 0) &&
                        (B == 1 || c->norm_on_device);
     if (fused) {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), x0, q, dq);
@@ -1448,19 +1455,19 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         G.head = (G.head + 1) % GUESS_K;
         G.nring = std::min(G.nring + 1, GUESS_K);
     }
-    k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
+    if (!trial) k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
     double *slot = c->stats.as<double>() + (c->stats_slot % c->stats_cap);
     k_record_residual<<<1, 256, 0, st>>>(B, err, c->rzs, S, launched, c->bead_it.as<int>(), c->bead_res.as<double>(), slot);
     LAUNCH_CHECK();
     c->stats_slot += 1;
-    ind->count += 1;
+    if (!trial) ind->count += 1;
     if (iterations) *iterations = it;
     if (!quiet) {
         HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, slot, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 1, &S->norm, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (residual) *residual = c->h_pin_dbl[0];
-        ind->norm = c->h_pin_dbl[1];
+        if (!trial) ind->norm = c->h_pin_dbl[1];
         return (it >= max_iter && c->h_pin_dbl[0] > c->s.dipole_tol) ? PGM_NOT_CONVERGED : PGM_OK;
     }
     return (it >= max_iter) ? PGM_NOT_CONVERGED : PGM_OK;
@@ -2019,6 +2026,26 @@ extern "C" int pgm_compute(pgm_ctx *c, const double *pos, pgm_induction *ind, pg
     if (!c || !pos || !ind || !ind->mu || !ind->hist) return PGM_ERR_ARG;
     return c->mixed ? compute_api<float>(c, pos, ind, energy, forces, iterations, residual)
                     : compute_api<double>(c, pos, ind, energy, forces, iterations, residual);
+}
+
+template <typename real>
+static int energy_api(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, int *iterations, double *residual) {
+    const int pr = prepare(c, pos);
+    if (pr < 0) return pr;
+    k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
+                                                            c->cc.as<double>(), c->pperm.as<double>());
+    LAUNCH_CHECK();
+    const int rs = solve_impl<real>(c, ind, iterations, residual, false, true);
+    if (rs < 0) return rs;
+    PGM_TRY(energy_forces_impl<real>(c, ind->mu, energy, nullptr));
+    return rs != PGM_OK ? rs : pr;
+}
+
+extern "C" int pgm_trial_energy(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, int *iterations,
+                                double *residual) {
+    if (!c || !pos || !ind || !ind->mu) return PGM_ERR_ARG;
+    return c->mixed ? energy_api<float>(c, pos, ind, energy, iterations, residual)
+                    : energy_api<double>(c, pos, ind, energy, iterations, residual);
 }
 
 extern "C" int pgm_bead_count(const pgm_ctx *c) { return c ? c->nbead : PGM_ERR_ARG; }

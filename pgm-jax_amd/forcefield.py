@@ -195,6 +195,25 @@ class PGMForceField:
         energy = {"elec": e.elec, "vdw": e.vdw, "total": e.total}
         return Result(energy, forces, new, it.value, res.value, st == _lib.PGM_ROW_OVERFLOW)
 
+    def energy(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None) -> tuple:
+        """Return (total energy [kJ/mol], state, CG iterations, overflow) for Monte Carlo barostat trials
+        (forcefield.py:2562-2626): the dipoles are solved from ind.mu (no predictor) into a new mu;
+        the state's history, count and norm are those passed in."""
+        if params is not None:
+            raise NotImplementedError("parameter pytrees are not part of the HIP path")
+        pos = _dev(pos, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        ind = self.init_induction() if ind is None else ind
+        mu = ind.mu.clone()  # the caller's state stays as it was (a rejected trial keeps it)
+        cind = _lib.Induction(mu.data_ptr(), ind.hist.data_ptr(), ind.count, ind.norm)
+        e = _lib.Energy()
+        it = C.c_int()
+        res = C.c_double()
+        st = _lib.check(_lib.lib().pgm_trial_energy(self._ctx, C.c_void_p(pos.data_ptr()), C.byref(cind), C.byref(e), C.byref(it),
+                                              C.byref(res)), "pgm_trial_energy")
+        return e.total, InductionState(mu, ind.hist, ind.count, ind.norm), it.value, st == _lib.PGM_ROW_OVERFLOW
+
     def energy_forces(self, pos, mu, H=None) -> tuple[dict, object]:
         """Return (energy parts, forces) at fixed induced dipoles mu (forcefield.py:2249-2292, :2362-2430)."""
         torch = _torch()
