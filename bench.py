@@ -101,7 +101,8 @@ def pmc_traffic(kernel: str):
         return None
 
 
-def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int, fused: bool = False) -> dict:
+def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int, fused: bool = False,
+                lanes: int = 16) -> dict:
     """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
 
     Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
@@ -142,7 +143,7 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs(row_bytes, row_ms) / HBM_PEAK_GBS, 5),
-        "traffic": pmc_traffic(kname),
+        "traffic": pmc_traffic(kname if layout != "molecule" or lanes == 16 else f"{kname}_{lanes}_lanes"),
         "algorithmic_bytes": row_bytes,
         "ms_per_launch": round(row_ms, 5),
         "launches": live["row_field"]["launches"],
@@ -348,7 +349,7 @@ def main() -> int:  # noqa: C901
     log("live kernel timing over the timed steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
     entries = ffm.row_entries()
-    layout = ffm.row_layout()[0]
+    layout, lanes = ffm.row_layout()
     blocks = ffm.row_blocks()
     log("kernels (ms):", json.dumps({k: round(v, 5) for k, v in kern.items()}), "row entries", entries)
     prev = None
@@ -380,7 +381,7 @@ def main() -> int:  # noqa: C901
     beads = bead_leg(P, a.bead_evals, 3) if a.bead_evals > 0 else None
     n = 3 * a.waters
     K3 = a.grid**3
-    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused")
+    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused", lanes)
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,
