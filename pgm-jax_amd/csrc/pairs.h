@@ -118,8 +118,7 @@ __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const 
                                                           const int *__restrict__ cend, double rc2, int cap,
                                                           int *__restrict__ rows, int *__restrict__ count,
                                                           int *__restrict__ maxcount) {
-    __shared__ double sx[256], sy[256], sz[256];
-    __shared__ int sj[256];
+    __shared__ double4 sp[256];  // staged candidate: position (shifted) and index (as a double)
     const int cell = blockIdx.x;
     const int bead = cell / ncell_bead, lc = cell - bead * ncell_bead;
     const int own0 = cstart[cell], own1 = cend[cell];
@@ -181,26 +180,37 @@ __global__ __launch_bounds__(256) void k_build_rows_cells(int ncell_bead, const 
                             const int j = satom[t0 + threadIdx.x];
                             double x, y, z;
                             wrapped(j, x, y, z);
-                            sj[threadIdx.x] = j;
-                            sx[threadIdx.x] = x + shx;
-                            sy[threadIdx.x] = y + shy;
-                            sz[threadIdx.x] = z + shz;
+                            sp[threadIdx.x] = make_double4(x + shx, y + shy, z + shz, (double)j);
                         }
                         __syncthreads();
                         if (owner) {
                             if (shifted) {
-                                for (int u = 0; u < m; ++u) {
-                                    const double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
-                                    if (dx * dx + dy * dy + dz * dz < rc2 && sj[u] != i) {
-                                        if (cnt < cap) row[cnt] = sj[u];
-                                        ++cnt;
+                                // eight candidates' distances at a time (their LDS reads in flight
+                                // together), then the hits appended in candidate order
+                                for (int u0 = 0; u0 < m; u0 += 8) {
+                                    double d2[8];
+                                    int jj[8];
+#pragma unroll
+                                    for (int k = 0; k < 8; ++k) {
+                                        const int u = u0 + k < m ? u0 + k : m - 1;
+                                        const double4 c4 = sp[u];
+                                        jj[k] = (int)c4.w;
+                                        const double dx = xi - c4.x, dy = yi - c4.y, dz = zi - c4.z;
+                                        d2[k] = dx * dx + dy * dy + dz * dz;
                                     }
+#pragma unroll
+                                    for (int k = 0; k < 8; ++k)
+                                        if (u0 + k < m && d2[k] < rc2 && jj[k] != i) {
+                                            if (cnt < cap) row[cnt] = jj[k];
+                                            ++cnt;
+                                        }
                                 }
                             } else {
                                 for (int u = 0; u < m; ++u) {
-                                    const int j = sj[u];
+                                    const double4 c4 = sp[u];
+                                    const int j = (int)c4.w;
                                     if (j == i) continue;
-                                    double dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
+                                    double dx = xi - c4.x, dy = yi - c4.y, dz = zi - c4.z;
                                     min_image(dx, dy, dz, box);
                                     if (dx * dx + dy * dy + dz * dz < rc2) {
                                         if (cnt < cap) row[cnt] = j;
