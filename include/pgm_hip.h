@@ -41,7 +41,8 @@ typedef struct pgm_ctx pgm_ctx;
  * unit(r_{cov_j[m]} - r_{cov_i[m]}) (pgm_jax/md/forcefield.py:1032-1042). */
 typedef struct {
     int n_atoms;
-    const double *q, *radius, *alpha, *lj_rmin_half, *lj_sqrt_eps;
+    const double *q, *radius, *alpha, *lj_rmin_half, *lj_sqrt_eps;  /* alpha >= 0; 0: not polarizable (its
+                                                                      * induced dipole stays 0, forcefield.py:767-780) */
     const int *mol;
     int n_cov;
     const int *cov_i, *cov_j;

@@ -46,6 +46,14 @@ SQRT_PI = math.sqrt(math.pi)
 PRED = {"mu3": (3.0, -3.0, 1.0), "mu4": (4.0, -6.0, 4.0, -1.0)}  # forcefield.py:605
 
 
+def div_alpha(x, alpha):
+    """x / alpha, 0 where alpha = 0 (forcefield.py:767-780 `_div_alpha`): atoms without polarizability
+    keep a zero induced dipole (z = alpha r never moves it) and add nothing to |mu|^2 / (2 alpha)."""
+    alpha = np.asarray(alpha, float)
+    pol = alpha != 0
+    return np.where(pol, np.asarray(x, float) / np.where(pol, alpha, 1.0), 0.0)
+
+
 @dataclasses.dataclass(frozen=True)
 class Settings:
     """The MDSettings fields of the hot path (forcefield.py:142-454 defaults)."""
@@ -165,8 +173,8 @@ class ForceField:
         return -(self.row_field(g, q, d) + rec - self.c_self * d)
 
     def operator(self, g: dict, S: dict, Gk: np.ndarray):
-        """Return A: v -> v/alpha - field(0, v) (forcefield.py:1694-1712)."""
-        inv_a = 1.0 / self.top.alpha[:, None]
+        """Return A: v -> v/alpha - field(0, v) (forcefield.py:1694-1712), v/alpha = 0 where alpha = 0."""
+        inv_a = div_alpha(1.0, self.top.alpha[:, None])
         return lambda v: v * inv_a - self.field(g, S, Gk, None, v)
 
     def cg(self, A, x: np.ndarray, r: np.ndarray, norm: float, tol=None, peek=None, max_iter=None) -> tuple:
@@ -221,7 +229,7 @@ class ForceField:
             x0h = sum(ci * ind.hist[k] for k, ci in enumerate(c))
             if s.fused and have and ind.count % s.norm_refresh != 0:
                 x0 = x0h
-                r0 = self.field(g, S, Gk, q, p + x0h) - x0h / a64
+                r0 = self.field(g, S, Gk, q, p + x0h) - div_alpha(x0h, a64)
                 norm = ind.norm
             else:
                 x0, r0, norm = plain(x0h, have)
@@ -273,7 +281,7 @@ class ForceField:
         u_rec = self.pme.energy(S, Gk, t.q, d)
         u_self = -(self.b0 / SQRT_PI) * np.sum(t.q**2) - 0.5 * self.c_self * np.sum(d * d)
         u_bg = -math.pi * np.sum(t.q) ** 2 / (2.0 * abs(np.linalg.det(H)) * self.b0**2)
-        u_pol = np.sum(mu * mu / (2.0 * t.alpha[:, None]))
+        u_pol = np.sum(div_alpha(mu * mu, 2.0 * t.alpha[:, None]))
         return KE * (u_rec + u_self + u_bg + u_pol)
 
     def energy_forces(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, g: dict, S: dict, Gk: np.ndarray) -> tuple:

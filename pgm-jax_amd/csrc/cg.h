@@ -70,6 +70,17 @@ __global__ void k_predict(int n, const double *__restrict__ hist, double c0, dou
     x0[i] = r;
 }
 
+// x0 = 0 where alpha = 0: atoms without polarizability keep a zero induced dipole
+// (forcefield.py:767-780).
+template <typename real>
+__global__ void k_mask_alpha0(int n, const real *__restrict__ alpha, real4<real> *__restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || alpha[i] != real(0)) return;
+    real4<real> z;
+    z.x = z.y = z.z = z.w = real(0);
+    x0[i] = z;
+}
+
 // x0 = mu (the initial guess of a trial solve: the dipoles passed in, no predictor).
 template <typename real>
 __global__ void k_mu_to_x0(int n, const double *__restrict__ mu, real4<real> *__restrict__ x0) {
@@ -565,7 +576,7 @@ __global__ __launch_bounds__(256) void k_energy_pack(int nper, const double *__r
         v.w = (real)q[i];
         dq[i] = v;
         s[0] = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        s[1] = m2 / (2.0 * alpha[i]);
+        s[1] = alpha[i] != 0.0 ? m2 / (2.0 * alpha[i]) : 0.0;  // 0 for atoms without polarizability
     }
     block_sum<2>(s, lds);
     if (threadIdx.x == 0) {

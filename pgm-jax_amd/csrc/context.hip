@@ -135,6 +135,7 @@ struct pgm_ctx {
     // CG row field layout: atom rows in sliced ELL (jell, G1, G2; RL lanes per atom) or, for 3-site
     // molecules, molecule blocks (mj, mg; rlm lanes per molecule) from the candidate molecules mrows
     bool mol3 = false;            // every bead is consecutive 3-atom molecules
+    bool alpha0 = false;          // some atom has polarizability 0 (its induced dipole stays 0)
     bool atom_rows_fresh = false; // rows / count / count_near compacted for the current geometry
     // CG iterations as hipGraphs (pgm_set_graphs): one executable per iteration index k, captured on
     // first use and replayed; dropped whenever a launch argument they hold can change
@@ -1319,6 +1320,10 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         LAUNCH_CHECK();
     } else if (use_net) {
         PGM_TRY(guess_x0<real>(c, c->pos, x0, ind->count % c->guess.every == 0));
+        if (c->alpha0) {  // the network's guess keeps non-polarizable atoms at 0
+            k_mask_alpha0<real><<<nb, 256, 0, st>>>(n, c->alpha_r.as<real>(), x0);
+            LAUNCH_CHECK();
+        }
     } else if (have) {
         if (c->s.predictor == 4 && K == 4) {
             cf[0] = 4, cf[1] = -6, cf[2] = 4, cf[3] = -1;
@@ -1681,11 +1686,12 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     for (int i = 0; i < nper; ++i) {  // per bead
         c->sum_q += c->h_q[i];
         c->sum_q2 += c->h_q[i] * c->h_q[i];
-        if (!(c->h_alpha[i] > 0)) {
-            g_err = "pgm_ctx_create: every polarizability must be positive";
+        if (!(c->h_alpha[i] >= 0)) {
+            g_err = "pgm_ctx_create: polarizabilities must be >= 0 (0: not polarizable)";
             delete c;
             return PGM_ERR_ARG;
         }
+        if (c->h_alpha[i] == 0) c->alpha0 = true;
     }
     c->lj_sb = lj_sum_b(c);
     auto fail = [&](int r) {
@@ -1833,6 +1839,10 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     }
     if (s->precond != 0 && s->precond != 1) {
         g_err = "pgm_ctx_create: precond must be 0 (Jacobi) or 1 (molecule blocks)";
+        return fail(PGM_ERR_ARG);
+    }
+    if (s->precond == 1 && c->alpha0) {
+        g_err = "pgm_ctx_create: the molecule-block preconditioner needs every polarizability > 0";
         return fail(PGM_ERR_ARG);
     }
     if (s->precond == 1) {

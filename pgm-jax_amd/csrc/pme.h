@@ -396,7 +396,7 @@ PGM_DEV void combine_out(int mode, int i, real rx, real ry, real rz, const real4
     const real tz = rw.z + rz - cself * d.z;
     real4<real> o;
     o.w = 0;
-    const real ia = real(1) / alpha[i];
+    const real ia = alpha[i] != real(0) ? real(1) / alpha[i] : real(0);  // 0: not polarizable
     if (mode == MODE_FIELD) {
         o.x = -tx;
         o.y = -ty;
