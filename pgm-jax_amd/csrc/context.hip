@@ -1509,7 +1509,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                                             c->pack_part.as<double>());
     LAUNCH_CHECK();
     if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));  // the force pass runs on the atom rows
-    const int npair = nblk((long long)nper * ROWL, 256);       // pair partials per bead
+    const int npair = nblk((long long)nper * rowl<real>(), 256);  // pair partials per bead
     const dim3 gpair(npair, B);
     const int nconv = c->ewald ? nblk(c->nk, EW_TILE) : nblk((long long)c->ncplx, 256);
     const bool vir = strain != nullptr;
