@@ -593,6 +593,7 @@ template <typename real> static int fused_lds(pgm_ctx *c) {
     HIP_TRY(hipFuncSetAttribute((const void *)k_pme_fwd_plane<real>, hipFuncAttributeMaxDynamicSharedMemorySize, plane));
     HIP_TRY(hipFuncSetAttribute((const void *)k_pme_bwd_plane<real>, hipFuncAttributeMaxDynamicSharedMemorySize, plane));
     HIP_TRY(hipFuncSetAttribute((const void *)k_pme_x_conv<real>, hipFuncAttributeMaxDynamicSharedMemorySize, xc));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_pme_x_conv<real, true>, hipFuncAttributeMaxDynamicSharedMemorySize, xc));
     std::vector<real> tw;
     int *offs[4] = {&c->pfg.twM, &c->pfg.twK3, &c->pfg.twK2, &c->pfg.twK1};
     const int lens[4] = {c->pfg.M, c->pfg.K3, c->pfg.K2, c->pfg.K1};
@@ -832,9 +833,11 @@ static void timing_end(pgm_ctx *c, int which, int k, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------- PME dispatch
-// The fused transform of the merged spread blocks: phi = c2r(Gc r2c(Q)) into c->grid (fft.h).
+// The fused transform of the merged spread blocks: phi = c2r(Gc r2c(Q)) into c->grid (fft.h); with
+// epart also the reciprocal energy partials, conv_partials(c) per bead.
 template <typename real>
-static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st, const std::function<int()> *fork = nullptr) {
+static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st, const std::function<int()> *fork = nullptr,
+                           double *epart = nullptr) {
     using C = cplx<real>;
     const PFFTGeom &f = c->pfg;
     const int B = c->nbead;
@@ -845,11 +848,26 @@ static int fused_transform(pgm_ctx *c, const int *conv, int iter, hipStream_t st
     k_pme_fwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, (size_t)c->tiles8.ntiles * c->tiles8.bsize, c->merge_tab.as<int>(),
                                                          c->blocks.as<real>(), c->cgrid.as<C>(), tw, conv, iter);
     if (fork && c->fork_stage == 2) PGM_TRY((*fork)());
-    k_pme_x_conv<real><<<dim3(nblk((long long)f.K2 * f.K3h, PFFT_COLS), B), 256, pfft_xconv_lds<real>(f), st>>>(
-        f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1, conv, iter);
+    const dim3 gx(nblk((long long)f.K2 * f.K3h, PFFT_COLS), B);
+    if (epart)
+        k_pme_x_conv<real, true><<<gx, 256, pfft_xconv_lds<real>(f), st>>>(f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1,
+                                                                          conv, iter, epart);
+    else
+        k_pme_x_conv<real><<<gx, 256, pfft_xconv_lds<real>(f), st>>>(f, c->cgrid.as<C>(), c->Gc.as<real>(), tw + f.twK1, conv,
+                                                                     iter);
     k_pme_bwd_plane<real><<<f.K1 * B, nth, plane, st>>>(f, c->cgrid.as<C>(), c->grid.as<real>(), tw, conv, iter);
     LAUNCH_CHECK();
     return PGM_OK;
+}
+
+// The PME transform runs fused (fft.h) when the grid allows it and no strain derivative is asked for.
+static bool use_fused(const pgm_ctx *c, bool vir) { return c->tiled && c->pme_fused && c->fused_ok && !vir; }
+
+// Reciprocal energy partials per bead that spread_fft(energy = true) leaves in c->conv_part (vir:
+// with the strain derivative).
+static int conv_partials(const pgm_ctx *c, bool vir) {
+    if (c->ewald) return nblk(c->nk, EW_TILE);
+    return use_fused(c, vir) ? nblk((long long)c->pfg.K2 * c->pfg.K3h, PFFT_COLS) : nblk((long long)c->ncplx, 256);
 }
 
 // fork: called once the spread (fork_stage 1) or the forward planes (fork_stage 2) are enqueued.
@@ -874,10 +892,10 @@ static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy
                                 c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
         }
         LAUNCH_CHECK();
-        if (c->pme_fused && c->fused_ok && !energy && !vir_part) {  // merge + transforms + Gc fused (fft.h)
+        if (use_fused(c, vir_part != nullptr)) {  // merge + transforms + Gc fused (fft.h)
             timing_end(c, 1, t_sp, st);
             if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
-            return fused_transform<real>(c, conv, iter, st, fork);
+            return fused_transform<real>(c, conv, iter, st, fork, energy ? c->conv_part.as<double>() : nullptr);
         }
         if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
         const int *K = c->s.grid;
@@ -1511,8 +1529,8 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));  // the force pass runs on the atom rows
     const int npair = nblk((long long)nper * rowl<real>(), 256);  // pair partials per bead
     const dim3 gpair(npair, B);
-    const int nconv = c->ewald ? nblk(c->nk, EW_TILE) : nblk((long long)c->ncplx, 256);
     const bool vir = strain != nullptr;
+    const int nconv = conv_partials(c, vir);
     if (vir) {
         PGM_TRY(c->com.ensure(sizeof(double) * 3 * (size_t)c->nmol_all));
         PGM_TRY(c->off.ensure(sizeof(double) * 3 * (size_t)n));
@@ -1821,7 +1839,8 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = c->grid.ensure(c->rsize * c->ngrid * nbead)) < 0) return fail(r);
     if ((r = c->cgrid.ensure(2 * c->rsize * c->ncplx * nbead)) < 0) return fail(r);
     if ((r = c->conv_part.ensure(sizeof(double) * nbead *
-                                 (std::max(nblk((long long)c->ncplx, 256), nblk(c->nk, EW_TILE)) + 16))) < 0)
+                                 (std::max({nblk((long long)c->ncplx, 256), nblk(c->nk, EW_TILE),
+                                            nblk((long long)s->grid[1] * (s->grid[2] / 2 + 1), PFFT_COLS)}) + 16))) < 0)
         return fail(r);
     if (c->ewald && (r = c->sk.ensure(sizeof(double2) * std::max(c->nk, 1))) < 0) return fail(r);
     if ((r = setup_tiles(c)) < 0) return fail(r);
@@ -1998,7 +2017,7 @@ static int pme_api(pgm_ctx *c, const double *pos, int withq, const double *d, do
     }
     // energy: sum of the convolution partials
     k_energy_final<<<1, 256, 0, c->stream>>>(c->pair_part.as<double>(), 0, c->pack_part.as<double>(), 0, c->conv_part.as<double>(),
-                                             nblk((long long)c->ncplx, 256), 0.0, 0.0, 0.0, c->energy.as<double>());
+                                             conv_partials(c, false), 0.0, 0.0, 0.0, c->energy.as<double>());
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->energy.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

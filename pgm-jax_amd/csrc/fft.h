@@ -309,14 +309,18 @@ template <typename real> inline size_t pfft_xconv_lds(const PFFTGeom &f) {
 }
 
 // Along x: columns c = y K3h + kz in [c0, c0 + 16) of bead b: transform, multiply by Gc, inverse.
-// Launch: (ceil(K2 K3h / 16), B) blocks of 256 threads, pfft_xconv_lds bytes of dynamic LDS;
-// twx = the K1 twiddles.
-template <typename real>
+// With ENERGY also the FP64 sum of w Gc |F Q|^2 over the block's columns (w = 1/2 on the kz = 0 and
+// Nyquist planes of the half spectrum, 1 elsewhere: the sum of k_convolve's ENERGY branch) to
+// epart[b * gridDim.x + blockIdx.x].  Launch: (ceil(K2 K3h / 16), B) blocks of 256 threads,
+// pfft_xconv_lds bytes of dynamic LDS; twx = the K1 twiddles.
+template <typename real, bool ENERGY = false>
 __global__ __launch_bounds__(256) void k_pme_x_conv(PFFTGeom f, cplx<real> *__restrict__ spec, const real *__restrict__ Gc,
-                                                    const cplx<real> *__restrict__ twx, const int *__restrict__ conv_it, int iter) {
+                                                    const cplx<real> *__restrict__ twx, const int *__restrict__ conv_it, int iter,
+                                                    double *__restrict__ epart = nullptr) {
     using C = cplx<real>;
     const int bead = blockIdx.y;
     if (cg_done(conv_it, bead, iter)) return;
+    __shared__ double red[16];
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C *col = reinterpret_cast<C *>(lds_raw);
     const int K1 = f.K1;
@@ -340,6 +344,7 @@ __global__ __launch_bounds__(256) void k_pme_x_conv(PFFTGeom f, cplx<real> *__re
     }
     __syncthreads();
     fft_lds<false>(col, K1, nc, PFFT_COLS, 1, tws);
+    double e[1] = {0.0};
     for (int p0 = threadIdx.x; p0 < nv; p0 += 4 * nth) {
         real g[4];
 #pragma unroll
@@ -350,8 +355,20 @@ __global__ __launch_bounds__(256) void k_pme_x_conv(PFFTGeom f, cplx<real> *__re
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int p = p0 + u * nth;
-            if (p < nv) col[p] = cmk<C>(col[p].x * g[u], col[p].y * g[u]);
+            if (p < nv) {
+                const C v = col[p];
+                if (ENERGY) {  // g = 0 on the padding columns
+                    const int kz = (c0 + (p % PFFT_COLS)) % f.K3h;
+                    const double w = (kz == 0 || (f.K3 % 2 == 0 && kz == f.K3h - 1)) ? 0.5 : 1.0;
+                    e[0] += w * (double)g[u] * ((double)v.x * v.x + (double)v.y * v.y);
+                }
+                col[p] = cmk<C>(v.x * g[u], v.y * g[u]);
+            }
         }
+    }
+    if (ENERGY) {
+        block_sum<1>(e, red);
+        if (threadIdx.x == 0) epart[(size_t)bead * gridDim.x + blockIdx.x] = e[0];
     }
     __syncthreads();
     fft_lds<true>(col, K1, nc, PFFT_COLS, 1, tws);
