@@ -162,13 +162,37 @@ __global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, 
     loc16[s] = l16[0] | (l16[1] << 10) | (l16[2] << 20);
 }
 
-// start / end of every key >> shift in the sorted keys (gather tiles: shift 3)
-__global__ void k_key_bounds(int n, const int *__restrict__ skey, int shift, int *__restrict__ start, int *__restrict__ end) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int k = skey[s] >> shift;
-    if (s == 0 || (skey[s - 1] >> shift) != k) start[k] = s;
-    if (s == n - 1 || (skey[s + 1] >> shift) != k) end[k] = s + 1;
+// The atom ranges of every spread tile and every gather tile from the sorted keys, in one launch and
+// without clearing first: thread i < n8 takes spread tile i (the keys equal to i), thread n8 + j
+// (j < n16) gather tile j (the keys with key >> shift == j).  Each bound is a binary search (first
+// position whose key is >= the tile's first key), so an empty tile gets an empty range.
+__global__ void k_tile_bounds(int n, const int *__restrict__ skey, int n8, int *__restrict__ start8, int *__restrict__ end8,
+                              int n16, int shift, int *__restrict__ start16, int *__restrict__ end16) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8 + n16) return;
+    const bool fine = i < n8;
+    const int k = fine ? i : i - n8;
+    const int k0 = fine ? k : k << shift, k1 = fine ? k + 1 : (k + 1) << shift;
+    int lo = 0, hi = n;  // first position with skey >= k0
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (skey[m] < k0) lo = m + 1;
+        else hi = m;
+    }
+    const int s0 = lo;
+    hi = n;  // first position with skey >= k1, at or after s0
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (skey[m] < k1) lo = m + 1;
+        else hi = m;
+    }
+    if (fine) {
+        start8[k] = s0;
+        end8[k] = lo;
+    } else {
+        start16[k] = s0;
+        end16[k] = lo;
+    }
 }
 
 // Tiled spread: one wave per spread tile (fine key; block b * t.ntiles + key for bead b) accumulates
