@@ -160,13 +160,13 @@ int pgm_set_graphs(pgm_ctx *ctx, int on);
 int pgm_set_row_layout(pgm_ctx *ctx, int layout, int lanes);
 int pgm_row_layout(const pgm_ctx *ctx, int *layout, int *lanes);
 
-/* Reciprocal transform of the CG field evaluations (the FFT / convolution / inverse FFT of
+/* Reciprocal transform of the CG field, energy and force evaluations (the FFT / convolution / inverse FFT of
  * pme.py:373-376): 0 the library's batched r2c / c2r plans with a separate convolution kernel, 1
  * the fused kernels (the spread-block merge, the real transform along z and the transform along y
  * per grid plane; along x with the product by the influence function and its inverse; the inverse
  * planes), the default where available: every grid axis a power of two in 8..256, tiled spreading,
- * K2 (K3/2 + 1) complex values within 160 KB.  Energy, force and strain evaluations keep the library
- * plans.  pgm_pme_transform reports the transform in use and whether the fused one is available. */
+ * K2 (K3/2 + 1) complex values within 160 KB; in an energy evaluation the x-transform kernel also
+ * sums the reciprocal energy.  Strain evaluations keep the library plans.  pgm_pme_transform reports the transform in use and whether the fused one is available. */
 int pgm_set_pme_transform(pgm_ctx *ctx, int fused);
 int pgm_pme_transform(const pgm_ctx *ctx, int *fused, int *available);
 

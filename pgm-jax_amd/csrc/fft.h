@@ -1,4 +1,4 @@
-// fft.h -- the PME reciprocal-space convolution of one CG field evaluation as three fused kernels.
+// fft.h -- the PME reciprocal-space convolution of one field (or energy) evaluation as three fused kernels.
 //
 // phi = c2r(Gc * r2c(Q)) over a K1 x K2 x K3 grid (z = K3 fastest; half spectrum of K3h = K3/2 + 1
 // points along z, the layout [x][y][kz] of the library's FFT plans), pme.py:373-376, with each K a
@@ -7,10 +7,10 @@
 //                    k_merge_table), the real transform along z (a K3/2-point complex transform of
 //                    (x_2m, x_2m+1) and the split into the half spectrum), the transform along y;
 //   k_pme_x_conv     per 16 (y, kz) columns: the transform along x, the product with Gc, the
-//                    inverse transform along x;
+//                    inverse transform along x (with ENERGY also the sum of w Gc |F Q|^2);
 //   k_pme_bwd_plane  one workgroup per x-plane: the inverse along y, the merge of the half
 //                    spectrum into a K3/2-point complex sequence, the inverse along z -> real plane.
-// Every transform runs in LDS as radix-8/4/2 Stockham stages (autosort, in place: a stage reads all
+// Every transform runs in LDS as radix-16/8/4/2 Stockham stages (autosort, in place: a stage reads all
 // its inputs into registers, then writes) with twiddles exp(-2 pi i t / N) tabulated in FP64 on the
 // host.  Unnormalised in both directions, like the library plans it replaces.
 #pragma once

@@ -382,7 +382,7 @@ class PGMForceField:
         _lib.check(_lib.lib().pgm_set_graphs(self._ctx, int(bool(on))), "pgm_set_graphs")
 
     def set_pme_transform(self, transform: str = "fused") -> None:
-        """Reciprocal transform of the CG field evaluations: "fused" kernels or the "library" FFT plans."""
+        """Reciprocal transform of the CG field, energy and force evaluations: "fused" kernels or the "library" FFT plans."""
         _set_transform(self._ctx, transform)
 
     def pme_transform(self) -> tuple[str, bool]:

@@ -1,9 +1,10 @@
 """The fused reciprocal transform of the CG field evaluations against the library FFT plans (GPU).
 
 Both compute phi = c2r(Gc r2c(Q)) (pme.py:373-376) from the same merged spread blocks; the fused
-kernels transform in LDS (radix-8/4/2 Stockham stages, FP64-tabulated twiddles), the library plans
+kernels transform in LDS (radix-16/8/4/2 Stockham stages, FP64-tabulated twiddles), the library plans
 with rocFFT.  Fields agree to rounding: 1e-12 relative RMS in FP64, 2e-6 in FP32; full evaluations
-take the same CG iterations (FP64) and energies agree to 1e-11 / 1e-7 relative.  Grids of unequal
+take the same CG iterations (FP64) and energies agree to 1e-11 / 1e-7 relative (the fused
+leg sums the reciprocal energy in the x-transform kernel, the library leg in the convolution kernel).  Grids of unequal
 axes check the axis bookkeeping; grids with an axis that is not a power of two keep the library
 plans and refuse the fused transform.
 """
