@@ -629,17 +629,29 @@ __global__ void k_forces_final(int n, const double *__restrict__ pos, Box box, c
 __global__ void k_energy_final(const double *__restrict__ pair_part, int npair, const double *__restrict__ pack_part, int npack,
                                const double *__restrict__ conv_part, int nconv, double ke_const, double tail, double cself,
                                double *__restrict__ out) {
-    __shared__ double lds[32];
+    __shared__ double lds[5 * 16];
     const int b = blockIdx.x;
-    pair_part += (size_t)b * npair * 2;
-    pack_part += (size_t)b * npack * 2;
+    const double2 *pp = reinterpret_cast<const double2 *>(pair_part) + (size_t)b * npair;
+    const double2 *pk = reinterpret_cast<const double2 *>(pack_part) + (size_t)b * npack;
     conv_part += (size_t)b * nconv;
     out += 4 * b;
-    const double e_el = block_reduce_partials(pair_part, npair, 2, 0, lds, false);
-    const double e_lj = block_reduce_partials(pair_part, npair, 2, 1, lds, false);
-    const double d2 = block_reduce_partials(pack_part, npack, 2, 0, lds, false);
-    const double upol = block_reduce_partials(pack_part, npack, 2, 1, lds, false);
-    const double urec = block_reduce_partials(conv_part, nconv, 1, 0, lds, false);
+    // one pass over the three partial arrays, five sums per thread, then one block sum
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int nt = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < npair; k0 += 4 * nt) {
+        double2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = k0 + u * nt < npair ? pp[k0 + u * nt] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[0] += x[u].x, v[1] += x[u].y;
+    }
+    for (int k = threadIdx.x; k < npack; k += nt) {
+        const double2 x = pk[k];
+        v[2] += x.x, v[3] += x.y;
+    }
+    for (int k = threadIdx.x; k < nconv; k += nt) v[4] += conv_part[k];
+    block_sum<5>(v, lds);
+    const double e_el = v[0], e_lj = v[1], d2 = v[2], upol = v[3], urec = v[4];
     if (threadIdx.x == 0) {
         const double elec = KE * 0.5 * e_el + KE * (urec - 0.5 * cself * d2 + upol) + ke_const;
         const double vdw = 0.5 * e_lj + tail;
