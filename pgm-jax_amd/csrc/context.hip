@@ -530,7 +530,7 @@ template <typename real> static int pme_bin(pgm_ctx *c, const double *pos) {
     PGM_TRY(c->pme_sort_tmp.ensure(tmp));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->pme_sort_tmp.p, tmp, c->pkey.as<int>(), c->pskey.as<int>(), c->pidx.as<int>(),
                                                c->psidx.as<int>(), n, 0, bits, st));
-    k_tile_bounds<<<nblk(nk8 + nk16, 256), 256, 0, st>>>(n, c->pskey.as<int>(), nk8, c->tstart.as<int>(), c->tend.as<int>(), nk16,
+    k_tile_bounds<<<nblk(n + 1, 256), 256, 0, st>>>(n, c->pskey.as<int>(), nk8, c->tstart.as<int>(), c->tend.as<int>(), nk16,
                                                          3, c->tstart16.as<int>(), c->tend16.as<int>());
     LAUNCH_CHECK();
     ORDER_SWITCH(c->s.order, (k_pme_splines<real, PO><<<nblk(n, 256), 256, 0, st>>>(n, pos, c->pg, t8, c->psidx.as<int>(),

@@ -163,35 +163,23 @@ __global__ void k_pme_splines(int n, const double *__restrict__ pos, PMEGeom g, 
 }
 
 // The atom ranges of every spread tile and every gather tile from the sorted keys, in one launch and
-// without clearing first: thread i < n8 takes spread tile i (the keys equal to i), thread n8 + j
-// (j < n16) gather tile j (the keys with key >> shift == j).  Each bound is a binary search (first
-// position whose key is >= the tile's first key), so an empty tile gets an empty range.
+// without clearing first.  Thread s in [0, n] looks at the keys either side of position s (key -1
+// before the first, the tile count after the last): position s is the start of every tile k with
+// key[s-1] < k <= key[s] and the end of tile k - 1, so every tile's start and end is written once
+// and an empty tile gets an empty range.  Spread tiles are the keys, gather tiles key >> shift.
 __global__ void k_tile_bounds(int n, const int *__restrict__ skey, int n8, int *__restrict__ start8, int *__restrict__ end8,
                               int n16, int shift, int *__restrict__ start16, int *__restrict__ end16) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n8 + n16) return;
-    const bool fine = i < n8;
-    const int k = fine ? i : i - n8;
-    const int k0 = fine ? k : k << shift, k1 = fine ? k + 1 : (k + 1) << shift;
-    int lo = 0, hi = n;  // first position with skey >= k0
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (skey[m] < k0) lo = m + 1;
-        else hi = m;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > n) return;
+    const int p8 = s > 0 ? skey[s - 1] : -1, c8 = s < n ? skey[s] : n8;
+    for (int k = p8 + 1; k <= c8; ++k) {
+        if (k < n8) start8[k] = s;
+        if (k > 0 && k <= n8) end8[k - 1] = s;
     }
-    const int s0 = lo;
-    hi = n;  // first position with skey >= k1, at or after s0
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (skey[m] < k1) lo = m + 1;
-        else hi = m;
-    }
-    if (fine) {
-        start8[k] = s0;
-        end8[k] = lo;
-    } else {
-        start16[k] = s0;
-        end16[k] = lo;
+    const int p16 = s > 0 ? skey[s - 1] >> shift : -1, c16 = s < n ? skey[s] >> shift : n16;
+    for (int k = p16 + 1; k <= c16; ++k) {
+        if (k < n16) start16[k] = s;
+        if (k > 0 && k <= n16) end16[k - 1] = s;
     }
 }
 
