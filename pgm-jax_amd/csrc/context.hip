@@ -737,6 +737,9 @@ static int compact_atom_rows(pgm_ctx *c) {
 // blocks the records are the compacted form; the atom rows are compacted only when asked for.
 static int prepare(pgm_ctx *c, const double *pos) {
     if (!c->rows_valid) PGM_TRY(build_rows(c, pos));
+    // Captured CG iterations of the per-atom PME and Ewald paths hold the position pointer as a
+    // kernel argument: positions at another address need new captures.
+    if (pos != c->pos) drop_graphs(c);
     c->pos = pos;
     const int n = c->n;
     hipStream_t st = c->stream;
@@ -1361,7 +1364,9 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
         LAUNCH_CHECK();
         PGM_TRY(field_eval<real>(c, x0, false, MODE_PLAIN, b, r, nullptr, -1));
     }
-    c->norm_on_device = true;
+    // A trial solve tests convergence against its own geometry's normaliser but leaves the state's
+    // as it was (forcefield.py:2612-2624): the next solve reloads ind->norm.
+    c->norm_on_device = !trial;
     HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
     if (c->molp)
         k_cg_start_mol<real><<<gM, MOLP_BLOCK, 0, st>>>(nper, r, c->alpha_r.as<real>(), c->pinv.as<real>(), z, p,
@@ -1844,11 +1849,13 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         g_err = "pgm_ctx_create: beads need a tiled PME grid (per axis at least three tiles of P-1..8 points)";
         return fail(PGM_ERR_ARG);
     }
-    {  // molecule blocks of the CG row field when every molecule is 3 consecutive atoms (and the
-       // candidate bitmap of a bead fits 64 KB of LDS per block)
+    {  // molecule blocks of the CG row field when every molecule is 3 consecutive atoms, the
+       // candidate bitmap of a bead fits 64 KB of LDS per block (k_mol_rows) and every molecule
+       // index of all beads fits the 26-bit partner field of a record word (k_mol_tensors);
+       // otherwise the atom rows
         bool m3 = nper % 3 == 0;
         for (int i = 0; i < nper && m3; ++i) m3 = top->mol[i] == i / 3;
-        c->mol3 = m3 && (nper / 3 + 31) / 32 <= 4096;
+        c->mol3 = m3 && (nper / 3 + 31) / 32 <= 4096 && (long long)c->n / 3 < (1LL << 26);
         c->layout = c->mol3 ? 1 : 0;
         if (c->layout == 1 && (r = alloc_mol(c, 64)) < 0) return fail(r);
     }
@@ -1998,6 +2005,7 @@ extern "C" int pgm_field(pgm_ctx *c, const double *pos, int with_charges, const 
 template <typename real>
 static int pme_api(pgm_ctx *c, const double *pos, int withq, const double *d, double *e_rec, double *grad_d) {
     using R4 = real4<real>;
+    if (pos != c->pos) drop_graphs(c);
     c->pos = pos;
     PGM_TRY(pme_bin<real>(c, pos));
     const int nb = nblk(c->n, 256);
@@ -2098,6 +2106,7 @@ extern "C" int pgm_set_row_layout(pgm_ctx *c, int layout, int lanes) {
         return PGM_ERR_ARG;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured iterations hold the previous layout's kernels and row tensors
     c->layout = layout;
     if (layout == 1) {
         c->rlm = lanes;
@@ -2532,6 +2541,16 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
     // half the skin minus a margin that covers one step's motion.
     const double trigger = 0.5 * c->s.skin - 0.01;
     bool disp_pending = false;
+    if (nsteps > 0 && c->rows_valid) {
+        // Earlier calls (or the caller) may have moved the atoms since the row build: the first step
+        // of every call tests the displacement of the current positions, read back synchronously.
+        k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
+        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, aux_slots(c) + 1);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, aux_slots(c) + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        disp_pending = true;
+    }
     for (int k = 0; k < nsteps; ++k) {
         if (disp_pending) {  // the read-back queued last step is complete: the CG waited on later work
             if (!c->rows_valid || c->h_pin_dbl[9] > trigger) {
