@@ -16,7 +16,8 @@ reference's); beside it: the same mixed path from the previous-step guess, the p
 (Jacobi, previous-step guess: the reference's algorithm; "fp64", the baseline of the metric), the energy /
 force / dipole errors of mixed vs FP64 at the final configuration, the HBM roofline of the
 dominant kernel (the CG matvec's row field; HIP events around every launch of the timed steps)
-with the PME spread / gather beside it, and the float64 numpy oracle on the host (cpu_baseline).
+with the PME spread / gather beside it, and the float64 C++/OpenMP restatement of the same algorithm on
+every host core (cpu_baseline, oracle/cpu_ref.cpp) from the same equilibrated state.
 
 Beads (--bead-evals > 0): 32 beads of 1,024 waters (3.94 nm box, PME 48^3, mixed, previous-step
 guess), force evaluations of all beads in one context (every stage one launch for all beads,
@@ -60,28 +61,70 @@ def ns_per_day(ms_per_step: float, dt_fs: float) -> float:
     return 86400.0 / (ms_per_step * 1e-3) * dt_fs * 1e-6
 
 
-def cpu_baseline(waters: int) -> dict:
-    """Time the float64 numpy oracle on one force evaluation of a 4,096-water box (same settings, PME 64^3)."""
-    sys.path.insert(0, ROOT)
-    from oracle import systems
-    from oracle.forcefield import ForceField, Settings
+def _host_cores() -> int:
+    """Cores this process may use: the affinity set, capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
 
-    nw, edge, K = 4096, 4.966, 64
-    top, pos, H = systems.water_box(nw, edge, seed=3)
-    ff = ForceField(top, H, Settings(pme_grid=(K,) * 3, dipole_tol=1e-7, max_iter=100, predictor="none"))
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos, vel, masses, H, grid: int, steps: int = 6, timed: int = 2) -> dict:
+    """Time the float64 C++/OpenMP restatement (oracle/cpu_ref.cpp) on every host core: `steps` NVE
+    steps of the bench's own system from its equilibrated state; the first steps fill the mu4
+    predictor's history (the first four solve from alpha b), the last `timed` are the sample."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    from oracle import systems
+    from oracle import md as omd
+    from oracle.cpu_ref import CpuRef
+    from oracle.forcefield import Settings
+
+    n = len(pos)
+    top, _, _ = systems.water_box(n // 3, float(H[0, 0]), seed=3)  # the topology of n // 3 waters
+    cores = _host_cores()
+    c = CpuRef(top, H, Settings(cutoff=0.9, ewald_beta=4.0, pme_grid=(grid,) * 3, pme_order=6, lj_lrc=True, dipole_tol=1e-7,
+                                max_iter=200), threads=cores)
+    x = np.ascontiguousarray(pos, float).copy()
+    v = np.ascontiguousarray(vel, float).copy()
+    d2 = np.ascontiguousarray(omd.constraint_lengths2(x))
     t0 = time.perf_counter()
-    res = ff.compute(pos, H)
-    sec = time.perf_counter() - t0
-    ms_step = sec * 1e3 * waters / nw  # O(N) force field: scaled by the number of waters
+    F = np.ascontiguousarray(c.compute(x)["forces"])
+    first = time.perf_counter() - t0
+    per, its = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rep = c.md(x, v, F, masses, d2, 1, 1e-3)
+        per.append(time.perf_counter() - t0)
+        its.append(rep["cg_iterations"])
+    ms = 1e3 * float(np.mean(per[-timed:]))
     return {
-        "value": round(ns_per_day(ms_step, 1.0), 6),
+        "value": round(ns_per_day(ms, 1.0), 6),
         "unit": "ns/day",
-        "cores": 1,
+        "cores": c.threads,
         "kind": "port",
-        "sample": (f"float64 numpy oracle, one force evaluation (rows, PME 64^3 order 6, CG to 1e-7 from alpha*b: "
-                   f"{res.iterations} it, forces) of 4,096 waters took {sec:.2f} s on 1 host core; scaled x{waters / nw:g} "
-                   f"to {waters} waters -> {ms_step / 1e3:.1f} s/step"),
-        "seconds": round(sec, 3),
+        "sample": (f"float64 C++/OpenMP restatement of the same algorithm (oracle/cpu_ref.cpp: cell-list rows, PME "
+                   f"{grid}^3 order 6, Jacobi CG to 1e-7 with the mu4 predictor, SHAKE/RATTLE NVE 1 fs), {n // 3} waters from the "
+                   f"bench's equilibrated state, {c.threads} threads on '{_cpu_model()}': {steps} steps after a first force "
+                   f"evaluation ({first:.1f} s), CG iterations per step {its}; value from the mean of the last {timed} steps "
+                   f"({ms / 1e3:.2f} s/step).  Reference's published CPU numbers (JAX CPU path): 3.14 ns/day on 32 cores for "
+                   f"512 waters (docs/efield.md:261), 40.3 ms/step on 16 cores for 1,536 atoms (docs/interfaces.md:235)"),
+        "seconds_per_step": [round(t, 3) for t in per],
+        "cpu_model": _cpu_model(),
     }
 
 
@@ -102,7 +145,7 @@ def pmc_traffic(kernel: str):
 
 
 def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, n: int, K3: int, fused: bool = False,
-                lanes: int = 16) -> dict:
+                lanes: int = 16, floor_us: float | None = None) -> dict:
     """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
 
     Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
@@ -158,6 +201,10 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                    "algorithmic_bytes": gather_bytes, "ms_per_launch": round(gather_ms, 5)},
         "isolated_ms": {k: round(v, 5) for k, v in kern.items()},
     }
+    if floor_us is not None:  # the spread / gather bytes at the launch floor: their HBM fraction's ceiling
+        out["launch_floor_us"] = round(floor_us, 3)
+        out["spread"]["ceiling_frac_at_launch_floor"] = round(gbs(spread_bytes, floor_us * 1e-3) / HBM_PEAK_GBS, 4)
+        out["gather"]["ceiling_frac_at_launch_floor"] = round(gbs(gather_bytes, floor_us * 1e-3) / HBM_PEAK_GBS, 4)
     if fused:
         half = K3 // 2  # complex points of the half spectrum (to K^2 (K/2 + 1) ~ K^3 / 2)
         tb = {"fwd_planes": K3 * 4 + half * 8, "x_conv": 2 * half * 8 + half * 4, "bwd_planes": half * 8 + K3 * 4}
@@ -167,6 +214,18 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                             "frac": round(gbs(sum(tb.values()), t_ms) / HBM_PEAK_GBS, 5),
                             "algorithmic_bytes": sum(tb.values()), "ms_per_launch": round(t_ms, 5)}
     return out
+
+
+def ctypes_floor(P, reps: int = 2000) -> float:
+    """Microseconds per back-to-back launch of an empty kernel on torch's stream (pgm_launch_floor)."""
+    import ctypes
+
+    import torch
+
+    us = ctypes.c_double()
+    P._lib.check(P._lib.lib().pgm_launch_floor(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), reps,
+                                               ctypes.byref(us)), "pgm_launch_floor")
+    return us.value
 
 
 def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
@@ -185,6 +244,8 @@ def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
     gen = torch.Generator(device="cuda").manual_seed(5)
     moves = [torch.randn(q0.shape, generator=gen, device="cuda", dtype=torch.float64) * 2e-4 for _ in range(warmup + evals)]
 
+    kern = {}
+
     def batched():
         pb = P.PGMBeads(sys_, H, s, beads=beads)
         log("beads autotune", json.dumps(pb.autotune(q0, reps=5)))
@@ -198,7 +259,9 @@ def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
             _, U, e = pb.compute(q, H, e)
             its += e.iters
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / evals, its / evals
+        ms = (time.perf_counter() - t0) * 1e3 / evals
+        kern.update(pb.time_kernels(q, reps=20))  # one launch of each kernel for all beads
+        return ms, its / evals
 
     def sequential():
         ffs = [P.PGMForceField(sys_, H, s) for _ in range(beads)]
@@ -223,8 +286,19 @@ def bead_leg(P, evals: int, warmup: int, beads: int = 32) -> dict:
     log(f"beads: {beads} x {nw} waters, one context: {ms_b:.3f} ms per evaluation of all beads, {it_b:.2f} CG it (max over beads)")
     ms_s, it_s = sequential()
     log(f"beads: {beads} single-bead contexts in turn: {ms_s:.3f} ms per evaluation of all beads, {it_s:.2f} CG it (max)")
+    n3 = 3 * nw
+    spread_bytes = beads * (n3 * 28 + K**3 * 4)
+    gather_bytes = beads * (K**3 * 4 + n3 * 24)
+    sp, ga = kern.get("spread", 0.0), kern.get("gather", 0.0)
     return {"workload": f"{beads} beads x {nw} rigid pGM3P-25 waters ({3 * nw} atoms per bead), {edge} nm box, PME {K}^3 "
                         f"order 6, mixed, previous-step guess, CG tol 1e-7; force evaluation of every bead",
+            "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+            "spread_all_beads": {"algorithmic_bytes": spread_bytes, "ms_per_launch": round(sp, 5),
+                                 "achieved": round(spread_bytes / (sp * 1e-3) / 1e9, 2) if sp else None,
+                                 "frac": round(spread_bytes / (sp * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if sp else None},
+            "gather_all_beads": {"algorithmic_bytes": gather_bytes, "ms_per_launch": round(ga, 5),
+                                 "achieved": round(gather_bytes / (ga * 1e-3) / 1e9, 2) if ga else None,
+                                 "frac": round(gather_bytes / (ga * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if ga else None},
             "evals": evals, "ms_per_eval_batched": round(ms_b, 4), "ms_per_eval_sequential": round(ms_s, 4),
             "speedup": round(ms_s / ms_b, 3), "cg_iters_batched": round(it_b, 3), "cg_iters_sequential": round(it_s, 3)}
 
@@ -310,7 +384,7 @@ def main() -> int:  # noqa: C901
             md = P.RigidWaterMD(ff, pos, vel)
         tune = ff.autotune(md.pos, reps=10)  # row-field layout and spread tiles for this context (untimed)
         log(precision, predictor, "autotune", json.dumps(tune))
-        md.run(warmup, a.dt_fs)
+        rep0 = md.run(warmup, a.dt_fs)
         torch.cuda.synchronize()
         log(precision, predictor, "timed", steps)
         if live_timing:
@@ -325,13 +399,16 @@ def main() -> int:  # noqa: C901
         log(precision, f"{ms:.3f} ms/step, {rep['cg_iterations'] / steps:.2f} CG it/step, T {rep['temperature']:.1f} K, "
             f"Etot {rep['etot']:.3f}, rebuilds {rep['row_rebuilds']}")
         rep["autotune"] = tune
+        rep["etot_start"] = rep0["etot"]
         return ff, md, rep, ms
 
     # the measured path: mixed precision with the guess (the equilibrated context keeps the trained network)
     ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, live_timing=True, predictor=a.guess, ff=ffe, md=mde)
     guess_status = ffm.guess_status() if a.guess == "learned" else None
+    guess_label = a.guess
     if guess_status:
         log("learned guess after the timed steps:", json.dumps(guess_status))
+        guess_label = f"learned ({guess_status['mode']} at the end of the timed steps)"
     graphs = None
     if a.graph_steps > 0:  # the same context and trajectory, CG iterations as hipGraphs (no per-kernel events)
         ffm.set_graphs(True)
@@ -379,9 +456,11 @@ def main() -> int:  # noqa: C901
                 "speedup_mixed_vs_fp64": round(ms_d / ms_m, 3)}
 
     beads = bead_leg(P, a.bead_evals, 3) if a.bead_evals > 0 else None
+    floor_us = ctypes_floor(P)
+    log(f"launch floor: {floor_us:.2f} us per empty-kernel launch")
     n = 3 * a.waters
     K3 = a.grid**3
-    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused", lanes)
+    roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused", lanes, floor_us)
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,
@@ -398,7 +477,7 @@ def main() -> int:  # noqa: C901
         "data": "synthetic",
         "config": {
             "workload": (f"{a.waters} rigid pGM3P-25 waters ({n} atoms), {a.edge} nm cubic box, qpi+LJ(lrc), rc 0.9 nm, "
-                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the {a.guess} guess ({a.precond} "
+                         f"beta 4.0/nm, PME {a.grid}^3 order 6, CG tol 1e-7 from the {guess_label} guess ({a.precond} "
                          f"preconditioner), NVE "
                          f"dt {a.dt_fs} fs, FP32 kernels with FP64 accumulation; baseline: the FP64 path from the "
                          f"previous-step guess"),
@@ -412,14 +491,17 @@ def main() -> int:  # noqa: C901
         "mixed_cg_graphs": graphs,
         "fp64": fp64,
         "errors_vs_fp64": None if errors is None else {k: float(f"{v:.3e}") for k, v in errors.items()},
-        "energy_drift_check": {"etot_end_kJmol": repm["etot"], "temperature_K": round(repm["temperature"], 2)},
+        "energy_drift_check": {"etot_start_kJmol": repm["etot_start"], "etot_end_kJmol": repm["etot"],
+                               "drift_kJmol": repm["etot"] - repm["etot_start"],
+                               "drift_per_ekin": (repm["etot"] - repm["etot_start"]) / repm["ekin"],
+                               "steps": a.steps, "ekin_kJmol": repm["ekin"], "temperature_K": round(repm["temperature"], 2)},
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "beads": beads,
         "roofline": roofline,
     }
     if not a.no_cpu_baseline:
-        log("cpu baseline (numpy oracle)")
-        out["cpu_baseline"] = cpu_baseline(a.waters)
+        log("cpu baseline (C++/OpenMP float64 restatement, all host cores)")
+        out["cpu_baseline"] = cpu_baseline(pos, vel, sys_.masses, H, a.grid)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -69,6 +69,10 @@ typedef struct {
     int nbead;          /* beads (0 or 1: one system): copies of the topology advanced together (PGMBeads) */
     int precond;        /* CG preconditioner: 0 Jacobi z = alpha r (the reference's), 1 the inverse 9x9 block of
                            each 3-site molecule (intramolecular coupling; same convergence test) */
+    double elec_cutoff; /* real-space electrostatics cutoff [nm] (Cutoffs.elec_cutoff, pmemd's es_cutoff); 0: the
+                           same as cutoff.  Electrostatic pair terms (CG row field, energies, forces, strain) stop
+                           at elec_cutoff, LJ at cutoff (pgm_jax/md/forcefield.py:44-58, :1197-1238); the rows
+                           span the larger of the two */
 } pgm_settings;
 
 /* Induction state (pgm_jax/md/forcefield.py:679-713 InductionState), caller-owned:
@@ -207,6 +211,7 @@ int pgm_strain_derivative(pgm_ctx *ctx, const double *pos, const double *mu, con
 int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu, double tol,
                                int max_iter, void *stream, int *iterations, double *residual);
 
+
 /* MD state (caller-owned device buffers, float64 (N,3)) and the report of pgm_md_run. */
 typedef struct {
     double *pos, *vel, *forces;
@@ -290,6 +295,10 @@ int pgm_kernel_timing(pgm_ctx *ctx, double *avg_ms, long long *launches);
 /* Measurement hook (no counterpart in the reference): a plain copy of `bytes` (a multiple of 16)
  * between device buffers on `stream`, the known-size kernel that calibrates HBM traffic counters. */
 int pgm_copy_probe(void *dst, const void *src, size_t bytes, void *stream);
+
+/* Measurement hook, no counterpart in the reference: microseconds per launch of `reps` back-to-back
+ * launches of an empty one-workgroup kernel on `stream` (hipStream_t), between HIP events. */
+int pgm_launch_floor(void *stream, int reps, double *us);
 
 #ifdef __cplusplus
 }

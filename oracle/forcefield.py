@@ -19,6 +19,10 @@ Restated functions (reference file:line):
   compute             forcefield.py:2489-2560        one force evaluation -> Result
   strain_derivative   forcefield.py:2628-2695        molecular dE/d eps at fixed mu (+ tail impulse)
 
+With elec_cutoff (< or > cutoff) the electrostatic pair terms stop at elec_cutoff and LJ at cutoff, the
+pairs spanning both (the reference's split rows, forcefield.py:44-58, :1197-1238, :2355); the LJ tail
+keeps cutoff.
+
 Every pair is kept once (i < j); the reference keeps it in both rows and halves the sums, which is
 the same arithmetic in float64.  Only the "qpi" electrostatics with "lj" van der Waals, the PME
 long-range treatment, the Jacobi preconditioner and nonzero polarizabilities are restated (the
@@ -64,6 +68,7 @@ class Settings:
     pme_spacing: float = 0.08
     pme_order: int = 6
     lj_lrc: bool = True
+    elec_cutoff: float | None = None
     dipole_tol: float = 1e-5
     max_iter: int = 50
     predictor: str = "mu4"
@@ -111,17 +116,23 @@ class ForceField:
         self.c_self = 4.0 * self.b0**3 / (3.0 * SQRT_PI)
         grid = settings.pme_grid or grid_size(H, settings.pme_spacing)
         self.pme = PME(grid, settings.pme_order, self.b0)
-        self.rc = float(settings.cutoff)
+        self.rc = float(settings.cutoff)  # van der Waals cutoff (and the LJ tail's)
+        # electrostatics cutoff (forcefield.py:44-58, :475): the real-space pair terms stop there,
+        # LJ at rc; the pairs span both
+        self.rc_e = self.rc if settings.elec_cutoff is None else float(settings.elec_cutoff)
+        self.rc_pair = max(self.rc, self.rc_e)
 
     # ------------------------------------------------------------------ geometry
     def geometry(self, pos: np.ndarray, H: np.ndarray) -> dict:
         """Return the pairs within rc with their kernels G0..G3 and LJ parameters (forcefield.py:1340-1380)."""
         t = self.top
-        g = pairs(pos, H, self.rc, t.mol, t.first)
+        g = pairs(pos, H, self.rc_pair, t.mol, t.first)
         i, j = g["i"], g["j"]
         a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
-        g["G"] = pair_kernels(a, self.b0, g["r"], 4)
-        g["wv"] = np.where(g["intra"], 0.0, 1.0)  # rigid topology: intramolecular vdW weight 0
+        inside = g["r"] < self.rc_e  # split rows (forcefield.py:1197-1238): electrostatics within rc_e only
+        g["G"] = tuple(Gn * inside for Gn in pair_kernels(a, self.b0, g["r"], 4))
+        # rigid topology: intramolecular vdW weight 0; van der Waals only within rc
+        g["wv"] = np.where(g["intra"] | (g["r"] >= self.rc), 0.0, 1.0)
         g["rmin"] = t.lj_rmin_half[i] + t.lj_rmin_half[j]
         g["eps"] = t.lj_sqrt_eps[i] * t.lj_sqrt_eps[j]
         return g
@@ -406,3 +417,62 @@ class ForceField:
         S = self.pme.setup(pos, H)
         Gk = self.pme.influence(H)
         return self.energy_forces(pos, H, mu, g, S, Gk)[0]["total"]
+
+    # ------------------------------------------------------------------ the solve's backward pass
+    def potential_from_dipoles(self, g: dict, S: dict, Gk: np.ndarray, lam: np.ndarray) -> np.ndarray:
+        """Return phi_j = dU/dq_j of dipoles lam alone (N,) [e/nm]: sum_k G1 lam_k . (r_j - r_k) over the
+        pairs plus the reciprocal potential sum_stencil theta1 theta2 theta3 dE_rec/dQ."""
+        i, j, x = g["i"], g["j"], g["x"]
+        G1 = g["G"][1]
+        phi = np.zeros(self.top.n)
+        np.add.at(phi, i, G1 * np.sum(lam[j] * x, -1))
+        np.add.at(phi, j, -G1 * np.sum(lam[i] * x, -1))
+        grid = self.pme.potential(S, Gk, np.zeros(self.top.n), lam)
+        P = grid.reshape(-1)[self.pme._flat(S)].reshape(len(phi), *(self.pme.order,) * 3)
+        t = [S["th"][:, a] for a in range(3)]
+        return phi + np.einsum("nabc,na,nb,nc->n", P, t[0], t[1], t[2], optimize=True)
+
+    def dipole_vjp(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, mu_bar: np.ndarray, tol: float = 1e-12) -> dict:
+        """The backward pass of the differentiable solve (forcefield.py:1853-1915): the adjoint CG
+        A lam = mu_bar (same operator, x0 = 0, no peek), then lam pulled back through the residual
+        r = b(theta) - A(theta) mu at fixed mu.  With E(mu) = U(q, p + mu) + sum |mu|^2 / (2 alpha) and
+        r = -dE/dmu, theta_bar = lam . dr/dtheta = -d/dtheta (lam . dE/dmu):
+
+          q_bar_j     = -phi_j(lam)                      (potential of the dipoles lam at j)
+          alpha_bar_i = mu_i . lam_i / alpha_i^2
+          p_bar_i     = field_i(0, lam)                  (the permanent dipoles; cov_bar by the frames)
+          R_bar_i     = 2 R_i sum_j a_ij^3 d/da [dlam e_ij],  dG_n/da = (2/sqrt(pi)) (2a^2)^n exp(-a^2 r^2)
+
+        where dlam e_ij is the derivative of the pair energy along d -> d + eps lam.  Returns per-atom
+        arrays and cov_bar (per covalent dipole) with lam and the CG iteration count."""
+        t = self.top
+        pos, H = np.asarray(pos, float), np.asarray(H, float)
+        g = self.geometry(pos, H)
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        A = self.operator(g, S, Gk)
+        a64 = t.alpha[:, None]
+        norm = float(np.mean(np.abs(a64 * mu_bar))) + 1e-300
+        lam, it, _ = self.cg(A, np.zeros_like(mu_bar), np.array(mu_bar, float), norm, tol=tol, peek=0.0)
+        q_bar = -self.potential_from_dipoles(g, S, Gk, lam)
+        alpha_bar = np.sum(mu * lam, -1) / t.alpha**2
+        p_bar = self.field(g, S, Gk, None, lam)
+        # radii: pair sums of the a-derivative of the lam-directional pair energy
+        d = self.perm_dipoles(pos, H) + mu
+        i, j, x, r = g["i"], g["j"], g["x"], g["r"]
+        a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
+        inside = r < self.rc_e
+        c = (2.0 / SQRT_PI) * np.exp(-(a * r) ** 2) * inside
+        H1, H2 = c * 2.0 * a * a, c * (2.0 * a * a) ** 2
+        qi, qk = t.q[i], t.q[j]
+        li, lk, di, dk = lam[i], lam[j], d[i], d[j]
+        lix, lkx, dix, dkx = (np.sum(v * x, -1) for v in (li, lk, di, dk))
+        de = (qi * lkx - qk * lix) * H1 - H2 * (lix * dkx + dix * lkx) + H1 * (np.sum(li * dk, -1) + np.sum(di * lk, -1))
+        R_bar = np.zeros(t.n)
+        np.add.at(R_bar, i, 2.0 * t.radius[i] * a**3 * de)
+        np.add.at(R_bar, j, 2.0 * t.radius[j] * a**3 * de)
+        v = min_image(pos[t.cov_j] - pos[t.cov_i], H)
+        u = v / np.linalg.norm(v, axis=-1, keepdims=True)
+        cov_bar = np.sum(p_bar[t.cov_i] * u, -1)
+        return {"lam": lam, "iterations": it, "q": q_bar, "alpha": alpha_bar, "p": p_bar, "radius": R_bar, "cov": cov_bar}
+

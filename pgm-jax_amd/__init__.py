@@ -8,16 +8,16 @@ FP64 accumulation ("mixed").  The kernels live in libpgm_hip.so (csrc/, C ABI in
 called through ctypes; there is no CPU fallback.
 """
 
-from .settings import MDSettings, grid_size
+from .settings import MDSettings, elec_cutoff_settings, ewald_beta_for, grid_size
 from .system import Molecule, System
 from .forcefield import InductionState, PGMForceField, Result
 from ._lib import PGMError
 
-__all__ = ["MDSettings", "grid_size", "Molecule", "System", "PGMForceField", "InductionState", "Result", "PGMError"]
+__all__ = ["MDSettings", "elec_cutoff_settings", "ewald_beta_for", "grid_size", "Molecule", "System", "PGMForceField", "InductionState", "Result", "PGMError"]
 from .md import Langevin, RigidWaterMD  # noqa: E402
 from .periodic import PeriodicModel, PeriodicPGM, pressure_bar  # noqa: E402
 from .beads import PGMBeads, PGMBeadState  # noqa: E402
-from . import models, solver  # noqa: E402
+from . import amber, driver, models, solver  # noqa: E402
 
 __all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "PGMBeads", "PGMBeadState",
-            "models", "solver"]
+            "amber", "driver", "models", "solver"]

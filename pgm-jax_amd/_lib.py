@@ -59,6 +59,7 @@ EXPORTS = (
     "pgm_guess_train",
     "pgm_guess_status",
     "pgm_copy_probe",
+    "pgm_launch_floor",
 )
 
 
@@ -101,6 +102,7 @@ class Settings(C.Structure):
         ("k_tol", C.c_double),
         ("nbead", C.c_int),
         ("precond", C.c_int),
+        ("elec_cutoff", C.c_double),
     ]
 
 
@@ -176,6 +178,7 @@ def lib() -> C.CDLL:
     L.pgm_row_blocks.argtypes = [P]
     L.pgm_row_blocks.restype = C.c_longlong
     L.pgm_copy_probe.argtypes = [P, P, C.c_size_t, P]
+    L.pgm_launch_floor.argtypes = [P, I, C.POINTER(D)]
     L.pgm_perm_dipoles.argtypes = [P, P, P]
     L.pgm_field.argtypes = [P, P, I, P, P]
     L.pgm_pme_energy.argtypes = [P, P, I, P, C.POINTER(D), P]

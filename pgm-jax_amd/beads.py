@@ -124,6 +124,19 @@ class PGMBeads:
         """Return the transform in use and whether the fused one is available for this grid."""
         return _transform(self._ctx)
 
+    def time_kernels(self, q, reps: int = 20) -> dict:
+        """Average duration [ms] of the hot kernels of one launch for all beads at q (P, N, 3) (pgm_time_kernels):
+        "spread", "row_field", "gather", the transform kernels and "matvec" (one field evaluation)."""
+        q = _dev(q, (self.P, self.n, 3))
+        self._ensure_rows(q)
+        ms = (C.c_double * 8)()
+        _lib.check(_lib.lib().pgm_time_kernels(self._ctx, C.c_void_p(q.data_ptr()), int(reps), ms), "pgm_time_kernels")
+        if self.pme_transform()[0] == "fused":
+            names = ("merge", "spread", "fwd_planes", "x_conv", "bwd_planes", "row_field", "gather", "matvec")
+            return {k: v for k, v in zip(names, list(ms)) if k != "merge"}
+        names = ("merge", "spread", "fft_r2c", "convolve", "fft_c2r", "row_field", "gather", "matvec")
+        return dict(zip(names, list(ms)))
+
     def bead_results(self) -> tuple[list[dict], np.ndarray, np.ndarray]:
         """Per-bead energies {"elec","vdw","total"}, CG iterations and final residuals of the last compute."""
         P = self.P

@@ -105,6 +105,9 @@ struct pgm_ctx {
     // each has its own rows, PME grid, CG state and energies, all advanced by the same launches
     int nbead = 1, nper = 0;
     pgm_settings s{};
+    // cutoffs [nm]: electrostatic pair terms within rc_e, LJ within rc_v (the settings' cutoff), rows
+    // (cell lists, candidates, compacted force-pass rows) within rc_pair = max(rc_e, rc_v)
+    double rc_e = 0, rc_v = 0, rc_pair = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;           // second stream: the pair rows run beside the PME chain
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -338,8 +341,8 @@ static int set_box(pgm_ctx *c, const double *H) {
     for (int b = 0; b < 3; ++b)
         for (int a = 0; a < 3; ++a) c->pg.escale[3 * b + a] = Hi[3 * b + a] * c->s.grid[a];
     for (int a = 0; a < 3; ++a) c->pg.K[a] = c->s.grid[a];
-    // cell grid: plane heights / (cutoff + skin)
-    const double rl = c->s.cutoff + c->s.skin;
+    // cell grid: plane heights / (row cutoff + skin)
+    const double rl = c->rc_pair + c->s.skin;
     double ax[3] = {H[0], H[1], H[2]}, bx[3] = {H[3], H[4], H[5]}, cx[3] = {H[6], H[7], H[8]};
     auto cross = [](const double *u, const double *v, double *w) {
         w[0] = u[1] * v[2] - u[2] * v[1];
@@ -696,7 +699,7 @@ static int build_rows(pgm_ctx *c, const double *pos) {
     k_fill_int<<<nblk(ncell, 256), 256, 0, st>>>(ncell, c->cell_end.as<int>(), 0);
     k_cell_bounds<<<nblk(n, 256), 256, 0, st>>>(n, c->skey.as<int>(), c->cell_start.as<int>(), c->cell_end.as<int>());
     LAUNCH_CHECK();
-    const double rl = c->s.cutoff + c->s.skin;
+    const double rl = c->rc_pair + c->s.skin;
     for (int attempt = 0; attempt < 3; ++attempt) {
         k_fill_int<<<1, 64, 0, st>>>(1, c->maxcount.as<int>(), 0);
         k_build_rows_cells<<<ncell, 256, 0, st>>>(ncell / c->nbead, pos, c->box, c->nc, c->satom.as<int>(), c->cell_start.as<int>(),
@@ -725,7 +728,7 @@ static int build_rows(pgm_ctx *c, const double *pos) {
 static int compact_atom_rows(pgm_ctx *c) {
     const int n = c->n;
     k_compact_rows<<<nblk(n, 4), 256, 0, c->stream>>>(n, c->xr.as<double4>(), c->box, c->rows_full.as<int>(),
-                                                      c->count_full.as<int>(), c->cap_full, c->s.cutoff * c->s.cutoff,
+                                                      c->count_full.as<int>(), c->cap_full, c->rc_pair * c->rc_pair,
                                                       c->s.ewald_beta * c->s.ewald_beta, c->cap, c->rows.as<int>(),
                                                       c->count.as<int>(), c->count_near.as<int>());
     LAUNCH_CHECK();
@@ -760,29 +763,29 @@ static int prepare(pgm_ctx *c, const double *pos) {
     }
     if (c->layout == 1) {
         const int nmol = n / 3;
-        const double rc2 = c->s.cutoff * c->s.cutoff;
+        const double rc2 = c->rc_pair * c->rc_pair, rce2 = c->rc_e * c->rc_e;
         // FP32: the same pass compacts the atom rows of the force pass (k_compact_rows not needed)
         const double beta2 = c->s.ewald_beta * c->s.ewald_beta;
         if (c->mixed)
             k_mol_tensors<float, true><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
-                                                               c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2,
+                                                               c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2, rce2,
                                                                (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(),
                                                                c->mcount.as<int>(), c->rows.as<int>(), c->cap,
                                                                c->count.as<int>(), c->count_near.as<int>(), beta2);
         else
             k_mol_tensors<double, false><<<nblk(nmol, 4), 256, 0, st>>>(
                 nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full, c->mcap, c->rlm, c->xr.as<double4>(), c->box,
-                rc2, c->s.ewald_beta, c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(), nullptr, 0, nullptr, nullptr,
+                rc2, rce2, c->s.ewald_beta, c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(), nullptr, 0, nullptr, nullptr,
                 beta2);
         c->atom_rows_fresh = c->mixed;
     } else if (c->mixed)
         k_pair_tensors<float><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
-            (float)c->s.ewald_beta, c->jell.as<int>(), c->G1.as<float>(), c->G2.as<float>());
+            (float)c->s.ewald_beta, c->rc_e * c->rc_e, c->jell.as<int>(), c->G1.as<float>(), c->G2.as<float>());
     else
         k_pair_tensors<double><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
-            c->s.ewald_beta, c->jell.as<int>(), c->G1.as<double>(), c->G2.as<double>());
+            c->s.ewald_beta, c->rc_e * c->rc_e, c->jell.as<int>(), c->G1.as<double>(), c->G2.as<double>());
     LAUNCH_CHECK();
     if (c->ewald) return PGM_OK;
     return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos);
@@ -1552,13 +1555,13 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
         LAUNCH_CHECK();
         k_pair_forces<real, true><<<gpair, 256, 0, st>>>(
             nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-            (real)c->s.ewald_beta, c->ljp.as<double4>(), c->g_el.as<double>(),
+            (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), c->off.as<double>(),
             c->vir_pair.as<double>());
     } else {
         k_pair_forces<real, false><<<gpair, 256, 0, st>>>(
             nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-            (real)c->s.ewald_beta, c->ljp.as<double4>(), c->g_el.as<double>(),
+            (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
     }
     LAUNCH_CHECK();
@@ -1634,6 +1637,10 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         g_err = "pgm_ctx_create: invalid cutoff, skin, ewald_beta or max_iter";
         return PGM_ERR_ARG;
     }
+    if (!(s->elec_cutoff >= 0)) {
+        g_err = "pgm_ctx_create: elec_cutoff must be >= 0 (0: the cutoff)";
+        return PGM_ERR_ARG;
+    }
     if (s->recip != 0 && s->recip != 1) {
         g_err = "pgm_ctx_create: recip must be 0 (PME) or 1 (Ewald)";
         return PGM_ERR_ARG;
@@ -1656,6 +1663,9 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     c->nper = top->n_atoms;
     c->n = nbead * top->n_atoms;
     c->s = *s;
+    c->rc_v = s->cutoff;
+    c->rc_e = s->elec_cutoff > 0 ? s->elec_cutoff : s->cutoff;
+    c->rc_pair = std::max(c->rc_e, c->rc_v);
     c->ewald = s->recip == 1;
     if (c->ewald) {  // half-space index set with 0 < |k| < kcut at the creation box (pgm_jax/ewald.py:129-158)
         const double kcut = 2.0 * s->ewald_beta * std::sqrt(-std::log(s->k_tol));
@@ -1878,8 +1888,8 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         if ((r = c->pinv.ensure(c->rsize * 81 * (size_t)(n / 3))) < 0) return fail(r);
     }
     if ((r = set_box(c, H)) < 0) return fail(r);
-    // initial row capacity from the density (a sphere of cutoff + skin), grown on overflow
-    const double rl = s->cutoff + s->skin;
+    // initial row capacity from the density (a sphere of the row cutoff + skin), grown on overflow
+    const double rl = c->rc_pair + s->skin;
     const int cap0 = (int)(1.25 * 4.0 / 3.0 * M_PI * rl * rl * rl * c->nper / c->volume) + 32;
     if ((r = alloc_rows(c, std::min(cap0, c->nper))) < 0) return fail(r);
     if ((r = make_plans(c)) < 0) return fail(r);
@@ -2567,10 +2577,14 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, aux_slots(c) + 1, sizeof(double), hipMemcpyDeviceToHost, st));
         disp_pending = true;
         {
+            const int rebuilds_before = c->rebuilds;
             const int r = c->mixed ? compute_api<float>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true)
                                    : compute_api<double>(c, s->pos, &s->ind, &e, s->forces, &it, &res, true);
             if (r < 0) return r;
             if (r == PGM_NOT_CONVERGED) status = r;
+            // rows rebuilt inside (they were invalid, e.g. after a box change): the displacement just
+            // queued was measured against the previous build, and the new build is at these positions
+            if (c->rebuilds != rebuilds_before) disp_pending = false;
         }
         it_total += it;
         it_max = std::max(it_max, it);
@@ -2638,6 +2652,31 @@ extern "C" int pgm_copy_probe(void *dst, const void *src, size_t bytes, void *st
     const size_t n4 = bytes / 16;
     k_copy_probe<<<(unsigned)((n4 + 255) / 256), 256, 0, (hipStream_t)stream>>>(n4, (const float4 *)src, (float4 *)dst);
     LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+// Launch floor: `reps` back-to-back launches of a one-workgroup kernel that does nothing, between two
+// HIP events on `stream`; *us = microseconds per launch (the time below which no kernel of a stream
+// can go, whatever its bytes).
+__global__ void k_empty() {}
+
+extern "C" int pgm_launch_floor(void *stream, int reps, double *us) {
+    if (reps <= 0 || !us) return PGM_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int r = 0; r < 16; ++r) k_empty<<<1, 64, 0, st>>>();
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) k_empty<<<1, 64, 0, st>>>();
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    LAUNCH_CHECK();
+    *us = 1e3 * (double)ms / reps;
     return PGM_OK;
 }
 

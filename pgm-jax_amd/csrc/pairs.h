@@ -294,7 +294,7 @@ PGM_DEV size_t ell(int e, int i, int n) { return (size_t)(e / RL) * ((size_t)n *
 template <typename real>
 __global__ __launch_bounds__(256) void k_pair_tensors(int n, const int *__restrict__ rows, const int *__restrict__ count,
                                                       const int *__restrict__ count_near, int cap,
-                                                      const double4 *__restrict__ xr, Box box, real beta,
+                                                      const double4 *__restrict__ xr, Box box, real beta, double rce2,
                                                       int *__restrict__ jell, real *__restrict__ G1, real *__restrict__ G2) {
     const int lane = threadIdx.x % RL;
     const int i = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
@@ -309,11 +309,13 @@ __global__ __launch_bounds__(256) void k_pair_tensors(int n, const int *__restri
         min_image(dx, dy, dz, box);
         const real x = (real)dx, y = (real)dy, z = (real)dz;
         const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
-        real G[3];
-        if (e < cn)
-            pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
-        else
-            pair_G_far<real, 3>(aij, beta, x * x + y * y + z * z, G);
+        real G[3] = {real(0), real(0), real(0)};
+        if (dx * dx + dy * dy + dz * dz < rce2) {  // electrostatics within the elec cutoff only
+            if (e < cn)
+                pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);
+            else
+                pair_G_far<real, 3>(aij, beta, x * x + y * y + z * z, G);
+        }
         const size_t q = ell(e, i, n);
         jell[q] = j;
         G1[q] = G[1];
@@ -402,6 +404,8 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
 //   LJ (intermolecular only): eps (s^12 - 2 s^6), s = rmin/r, (1/r) de/dr = 12 eps (s^6 - s^12)/r^2,
 //   one FP64 reciprocal of r^2 per pair; the partner's (rmin/2, sqrt eps, molecule) in one 32-byte
 //   load (ljp).
+// Electrostatic terms stop at the elec cutoff (r^2 < rce2), LJ at the van der Waals cutoff (r^2 < rcv2;
+// pgm_jax/md/forcefield.py:1197-1238, 2355): the rows span both.
 // Outputs (FP64): g_el = sum de/dx, g_lj, the row field sum de/dd_i, and per-block energy sums
 // (each pair counted in both rows: the caller halves them).  With VIRIAL, also the per-block sums
 // of (KE de/dx + de_LJ/dx) (x) X, X = x - o_i + o_k with o the offsets of the atoms from their
@@ -412,9 +416,10 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
             min_image(dx, dy, dz, box); \
             const real x = (real)dx, y = (real)dy, z = (real)dz; \
             const real r2 = x * x + y * y + z * z; \
+            const double r2d = dx * dx + dy * dy + dz * dz; \
             const real aij = rsqrt_(real(2) * (real)(a.w + b.w)); \
-            real G[4]; \
-            GFUN<real, 4>(aij, beta, r2, G); \
+            real G[4] = {real(0), real(0), real(0), real(0)}; \
+            if (r2d < rce2) GFUN<real, 4>(aij, beta, r2, G); \
             const real4<real> dk = dq[j]; \
             const real qk = dk.w; \
             const real dix = di.x * x + di.y * y + di.z * z; \
@@ -438,9 +443,9 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
             const double4 ljk = ljp[j]; \
             const double eps = sei * ljk.y; \
             double l0 = 0.0, l1 = 0.0, l2 = 0.0; \
-            if ((int)ljk.z != mi && eps != 0.0) { \
+            if ((int)ljk.z != mi && eps != 0.0 && r2d < rcv2) { \
                 const double rmin = rhi + ljk.x; \
-                const double ir2 = 1.0 / (dx * dx + dy * dy + dz * dz); \
+                const double ir2 = 1.0 / r2d; \
                 const double s2 = rmin * rmin * ir2; \
                 const double s6 = s2 * s2 * s2; \
                 el += eps * (s6 * s6 - 2.0 * s6); \
@@ -464,7 +469,8 @@ template <typename real, bool VIRIAL>
 __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__restrict__ rows, const int *__restrict__ count,
                                                      const int *__restrict__ count_near, int cap,
                                                      const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
-                                                     real beta, const double4 *__restrict__ ljp, double *__restrict__ g_el,
+                                                     real beta, double rce2, double rcv2, const double4 *__restrict__ ljp,
+                                                     double *__restrict__ g_el,
                                                      double *__restrict__ g_lj, double *__restrict__ rowf,
                                                      double *__restrict__ epart, const double *__restrict__ off,
                                                      double *__restrict__ wpart) {
@@ -625,7 +631,8 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 }
 
 // Records of the molecule blocks, once per geometry (the molecule form of k_compact_rows +
-// k_pair_tensors): every candidate J of I with at least one atom pair inside rc is kept, in candidate
+// k_pair_tensors): every candidate J of I with at least one atom pair inside the electrostatics
+// cutoff (r^2 < rce2; pairs beyond it hold zeros, and rce2 = rc2 without a split cutoff) is kept, in candidate
 // order, its G1 / G2 by pair_G from the FP64 minimum-image displacement rounded to real.  One wave
 // per molecule, lanes over candidates, ballot compaction.  Divergence of pair_G's series branches is
 // kept out of the common path: the closed form (both screenings with (a r)^2, (beta r)^2 >= 1) is
@@ -641,7 +648,7 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 template <typename real, bool ROWS>
 __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
                                                      int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
-                                                     double rc2, real beta, int *__restrict__ mj, real *__restrict__ mg,
+                                                     double rc2, double rce2, real beta, int *__restrict__ mj, real *__restrict__ mg,
                                                      int *__restrict__ mcount, int *__restrict__ rows, int acap,
                                                      int *__restrict__ acount, int *__restrict__ acount_near, double beta2) {
     const int lane = threadIdx.x % WAVE;
@@ -681,11 +688,12 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
                     min_image_n(dx, dy, dz, box, n);
                     const double r2d = dx * dx + dy * dy + dz * dz;
                     if (r2d >= rc2) continue;
-                    keep = true;
                     if (ROWS) {
                         akeep |= 1u << (3 * p + k);
                         if (r2d < 1.05 * 2.0 * (a[p].w + b[k].w) || beta2 * r2d < 1.05) anear |= 1u << (3 * p + k);
                     }
+                    if (r2d >= rce2) continue;  // records hold the electrostatic pairs only
+                    keep = true;
                     if (own) continue;  // the own molecule's pairs come after the compaction
                     if (first) {
                         n0[0] = n[0], n0[1] = n[1], n0[2] = n[2];
@@ -776,7 +784,7 @@ __global__ __launch_bounds__(256) void k_mol_tensors(int nmol, const int *__rest
             double dx = ap.x - ak.x, dy = ap.y - ak.y, dz = ap.z - ak.z;
             min_image(dx, dy, dz, box);
             real G[3] = {real(0), real(0), real(0)};
-            if (dx * dx + dy * dy + dz * dz < rc2) {
+            if (dx * dx + dy * dy + dz * dz < rce2) {
                 const real x = (real)dx, y = (real)dy, z = (real)dz;
                 const real aij = rsqrt_(real(2) * (real)(ap.w + ak.w));
                 pair_G<real, 3>(aij, beta, x * x + y * y + z * z, G);

@@ -90,6 +90,7 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
     cs.k_tol = settings.k_tol
     cs.nbead = int(nbead)
     cs.precond = 1 if settings.precond == "molecule" else 0
+    cs.elec_cutoff = 0.0 if settings.elec_cutoff is None else float(settings.elec_cutoff)
     stream = torch.cuda.current_stream()
     ctx = C.c_void_p()
     _lib.check(L.pgm_ctx_create(C.byref(ctx), C.byref(top), C.byref(cs), _ptr(H, C.c_double),

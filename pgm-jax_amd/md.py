@@ -83,7 +83,12 @@ class RigidWaterMD:
             "temperature": rep.temperature, "cg_iterations": rep.cg_iterations_total,
             "cg_iterations_max": rep.cg_iterations_max, "cg_residual_max": rep.cg_residual_max,
             "row_rebuilds": rep.row_rebuilds, "not_converged": st == _lib.PGM_NOT_CONVERGED, "heat": rep.heat,
+            "cg_iter_last": rep.cg_iterations_last,
         }
+
+    def invalidate_forces(self) -> None:
+        """Mark the stored forces stale (positions or box changed outside run): the next run recomputes them first."""
+        self._state.have_forces = 0
 
     def equilibrate(self, nsteps: int, temperature: float, dt_fs: float = 1.0, every: int = 10) -> dict:
         """Run nsteps with the velocities rescaled to `temperature` [K] every `every` steps; return the last report."""

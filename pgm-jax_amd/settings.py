@@ -23,6 +23,9 @@ class MDSettings:
     Beyond the reference's MD settings: recip ("pme" | "ewald", the exact Ewald sum of the
     PeriodicPGM reference model), k_tol (its k-vector truncation) and precond ("jacobi", the
     reference's z = alpha r, or "molecule": the inverse 9x9 block of each 3-site molecule).
+    elec_cutoff [nm] (None: cutoff) is the real-space electrostatics cutoff of the reference's split
+    rows (forcefield.py:44-58): electrostatic pair terms stop there, LJ at cutoff; choose
+    ewald_beta and the PME spacing for it (elec_cutoff_settings).
     """
 
     elec: str = "qpi"
@@ -44,6 +47,7 @@ class MDSettings:
     recip: str = "pme"
     k_tol: float = 1e-12
     precond: str = "jacobi"
+    elec_cutoff: float | None = None
 
     def __post_init__(self) -> None:
         """Validate like PGMForceField.__init__ (forcefield.py:917-1000)."""
@@ -63,6 +67,13 @@ class MDSettings:
             raise ValueError("k_tol must be in (0, 1)")
         if self.precond not in ("jacobi", "molecule"):
             raise ValueError(f"precond must be 'jacobi' or 'molecule', got {self.precond!r}")
+        if self.elec_cutoff is not None and not self.elec_cutoff > 0:
+            raise ValueError(f"elec_cutoff must be positive, got {self.elec_cutoff}")
+
+    @property
+    def elec_rc(self) -> float:
+        """The electrostatics cutoff [nm] (cutoff unless elec_cutoff is set; forcefield.py:475)."""
+        return float(self.cutoff if self.elec_cutoff is None else self.elec_cutoff)
 
     def replace(self, **changes) -> MDSettings:
         """Return a copy with some settings changed (TypeError for an unknown name, as the reference)."""
@@ -94,3 +105,41 @@ def grid_size(H, spacing: float = 0.05, factors=(2, 3, 5)) -> tuple[int, int, in
             n += 1
         out.append(n)
     return tuple(out)
+
+
+# Direct-sum tolerance (Amber's convention, per Angstrom) of the default pair 0.9 nm / 4.0 nm^-1.
+DSUM_TOL = 3.95e-8
+
+
+def ewald_beta_for(elec_cutoff: float, dsum_tol: float = DSUM_TOL) -> float:
+    """Return beta [1/nm] with erfc(beta rc) / rc_A = dsum_tol (rc_A = 10 rc), by bisection as Amber
+    does (pgm_jax/md/forcefield.py:523-567): 4.0 at 0.9 nm, 5.19 at 0.7 nm for the default tolerance.
+    Raises ValueError for a non-positive cutoff or dsum_tol outside (0, 1/rc_A)."""
+    import math
+
+    rc = float(elec_cutoff)
+    if not rc > 0.0:
+        raise ValueError(f"elec_cutoff must be positive, got {elec_cutoff}")
+
+    def f(b: float) -> float:
+        return math.erfc(b * rc) / (10.0 * rc) - dsum_tol
+
+    if not (0.0 < dsum_tol and f(0.0) > 0.0):
+        raise ValueError(f"dsum_tol must be in (0, {1.0 / (10.0 * rc):.3g}), got {dsum_tol}")
+    lo, hi = 0.0, 1.0
+    while f(hi) > 0.0:
+        hi *= 2.0
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0.0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def elec_cutoff_settings(elec_cutoff: float, dsum_tol: float = DSUM_TOL, exponent: float = 1.6) -> dict:
+    """MDSettings arguments for an electrostatics cutoff: beta from ewald_beta_for and the PME spacing
+    0.08 nm (4.0 / beta)^exponent (pgm_jax/md/forcefield.py:570-601)."""
+    b = ewald_beta_for(elec_cutoff, dsum_tol)
+    return {"elec_cutoff": float(elec_cutoff), "ewald_beta": b, "pme_spacing": 0.08 * (4.0 / b) ** exponent}

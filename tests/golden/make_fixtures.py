@@ -13,6 +13,9 @@ Every output is data (inputs and expected outputs), never reference source:
   (pgm_jax/param.py:113-233, :281-307) defines them).
 * ``ff_small_box.npz``, ``ff_pgm3p25_512.npz``, ``periodic_model.npz`` -- copies of the golden
   outputs of the reference's regression harness (tests/regression/golden/), recorded with jax 0.11.2.
+* ``pgm3p25_512.prmtop``, ``pgm3p25_512.rst7`` -- byte copies of the two input files of
+  ``examples/water512`` (the reference's install check, tests/test_water512.py), read by the
+  product's own Amber readers in the tests of the MD runner.
 * ``pmemd_pbc.npz`` -- the independent Fortran (pmemd-pgm) single point of
   data/validation/amber_ref/pmemd_pbc: EELEC / VDWAALS (kcal/mol), forces (kcal/mol/A) and induced
   dipoles (e A).
@@ -24,6 +27,7 @@ from __future__ import annotations
 
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -125,8 +129,15 @@ def pmemd_pbc(ref: str) -> dict:
     return {"eelec_kcal": np.array(eelec), "vdw_kcal": np.array(vdw), "forces_kcal_A": frc, "mu_eA": np.array(mu)}
 
 
+def copy_water512_inputs(ref: str) -> None:
+    """Copy the prmtop and NetCDF restart of examples/water512 into tests/golden/."""
+    for name in ("pgm3p25_512.prmtop", "pgm3p25_512.rst7"):
+        shutil.copyfile(os.path.join(ref, "examples/water512", name), os.path.join(HERE, name))
+
+
 def main(ref: str) -> None:
     """Write every fixture into tests/golden/."""
+    copy_water512_inputs(ref)
     np.savez_compressed(os.path.join(HERE, "water512.npz"), **water512(ref))
     gold = os.path.join(ref, "tests/regression/golden")
     for name in ("ff_small_box", "ff_pgm3p25_512", "periodic_model"):

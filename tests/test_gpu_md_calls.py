@@ -6,7 +6,8 @@
   path hold the positions' address: compute() with positions at a new address must not replay them.
 * A Monte Carlo trial energy (pgm_trial_energy) tests convergence against its own geometry's
   normaliser and leaves the state's in place (pgm_jax/md/forcefield.py:2612-2624): an MD run after
-  a trial equals one without it, normaliser included.
+  a trial equals one without it, normaliser included (the trial's row build at the scaled
+  positions must not leave a displacement reading that triggers an extra rebuild either).
 """
 
 import numpy as np
@@ -55,7 +56,8 @@ def test_graphs_with_new_position_tensors(recip, grid):
     import torch
 
     sys_, pos, vel, H = _water(P, nw=64, edge=1.2416, seed=9)
-    kw = dict(precision="double", predictor="mu4", dipole_tol=1e-9, max_iter=300, cutoff=0.55, ewald_beta=5.0, recip=recip)
+    kw = dict(precision="double", predictor="mu4", dipole_tol=1e-9, max_iter=300, cutoff=0.55, skin=0.05, ewald_beta=5.0,
+              recip=recip)
     if grid:
         kw["pme_grid"] = (grid,) * 3
     pos2 = pos + np.random.default_rng(3).normal(scale=2e-3, size=pos.shape)
