@@ -211,6 +211,15 @@ int pgm_strain_derivative(pgm_ctx *ctx, const double *pos, const double *mu, con
 int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu, double tol,
                                int max_iter, void *stream, int *iterations, double *residual);
 
+/* No counterpart in the reference.  pgm_solve_linear_induction without any allocation, for callers
+ * that cannot allocate device or pinned memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same arguments, kernels and results, with the CG work
+ * vectors in a fixed device array that the library holds once per GPU for the life of the process
+ * (about 1.8 MB).  n may not exceed PGM_DENSE_NOALLOC_MAX_ATOMS (PGM_ERR_ARG beyond it).  Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+#define PGM_DENSE_NOALLOC_MAX_ATOMS 16384
+int pgm_solve_linear_induction_noalloc(int n, const double *T, const double *alpha, const double *F, double *mu,
+                                       double tol, int max_iter, void *stream, int *iterations, double *residual);
 
 /* MD state (caller-owned device buffers, float64 (N,3)) and the report of pgm_md_run. */
 typedef struct {

@@ -44,6 +44,7 @@ EXPORTS = (
     "pgm_energy_forces",
     "pgm_strain_derivative",
     "pgm_solve_linear_induction",
+    "pgm_solve_linear_induction_noalloc",
     "pgm_md_setup",
     "pgm_md_run",
     "pgm_md_set_thermostat",
@@ -196,6 +197,7 @@ def lib() -> C.CDLL:
     L.pgm_energy_forces.argtypes = [P, P, P, C.POINTER(Energy), P]
     L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
+    L.pgm_solve_linear_induction_noalloc.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]
     L.pgm_md_run.argtypes = [P, C.POINTER(MDState), I, D, C.POINTER(MDReport)]
     L.pgm_md_set_thermostat.argtypes = [P, D, D, C.c_ulonglong]

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <climits>
 #include <random>
 #include <cmath>
@@ -2372,13 +2373,64 @@ extern "C" int pgm_kernel_timing(pgm_ctx *c, double *avg_ms, long long *launches
 }
 
 // ---------------------------------------------------------------------------- dense induction solve
-extern "C" int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu,
-                                          double tol, int max_iter, void *stream, int *iterations, double *residual) {
+// The CG of the dense solve on caller-provided device buffers: y, r, z, p (3n doubles each), part
+// (2 max(n, blocks) doubles), out (2 doubles), and a host array h of 4 doubles for the scalars.
+static int dense_solve(int n, const double *T, const double *alpha, const double *F, double *mu, double tol, int max_iter,
+                       hipStream_t st, int *iterations, double *residual, double *y, double *r, double *z, double *p,
+                       double *part, double *out, double *h) {
+    const int n3 = 3 * n, nb = nblk(n3, 256);
+    auto scalars = [&](int np, int stride) -> int {
+        k_dense_reduce<<<1, 256, 0, st>>>(part, np, stride, out);
+        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h, out, stride * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            g_err = "pgm_solve_linear_induction: device error";
+            return PGM_ERR_HIP;
+        }
+        return PGM_OK;
+    };
+    double *x = mu;
+    // x0 = alpha F, r = F - A x0, z = alpha r, p = z
+    k_dense_update<<<nb, 256, 0, st>>>(n3, 3, 0.0, 0.0, F, alpha, nullptr, x, nullptr, nullptr, nullptr, part);
+    PGM_TRY(scalars(nb, 2));
+    const double fnorm = std::sqrt(h[0]);
+    k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, x, y, part);
+    k_dense_update<<<nb, 256, 0, st>>>(n3, 0, 0.0, 0.0, F, alpha, y, x, r, z, p, part);
+    PGM_TRY(scalars(nb, 2));
+    double rz = h[0], rr = h[1];
+    int it = 0;
+    // stopping rule of jax.scipy.sparse.linalg.cg: |r| <= tol |F|
+    while (it < max_iter && std::sqrt(rr) > tol * fnorm) {
+        k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, p, y, part);
+        PGM_TRY(scalars(n, 1));
+        const double al = rz / std::max(h[0], 1e-300);
+        k_dense_update<<<nb, 256, 0, st>>>(n3, 1, al, 0.0, F, alpha, y, x, r, z, p, part);
+        PGM_TRY(scalars(nb, 2));
+        const double be = h[0] / std::max(rz, 1e-300);
+        rz = h[0];
+        rr = h[1];
+        k_dense_update<<<nb, 256, 0, st>>>(n3, 2, 0.0, be, F, alpha, nullptr, x, nullptr, z, p, nullptr);
+        ++it;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        g_err = "pgm_solve_linear_induction: device error";
+        return PGM_ERR_HIP;
+    }
+    if (iterations) *iterations = it;
+    if (residual) *residual = fnorm > 0 ? std::sqrt(rr) / fnorm : 0.0;
+    return std::sqrt(rr) > tol * fnorm ? PGM_NOT_CONVERGED : PGM_OK;
+}
+
+static bool dense_args_ok(int n, const double *T, const double *alpha, const double *F, double *mu, double tol, int max_iter) {
     if (n <= 0 || !T || !alpha || !F || !mu || !(tol > 0) || max_iter < 0) {
         g_err = "pgm_solve_linear_induction: invalid argument";
-        return PGM_ERR_ARG;
+        return false;
     }
-    hipStream_t st = (hipStream_t)stream;
+    return true;
+}
+
+extern "C" int pgm_solve_linear_induction(int n, const double *T, const double *alpha, const double *F, double *mu,
+                                          double tol, int max_iter, void *stream, int *iterations, double *residual) {
+    if (!dense_args_ok(n, T, alpha, F, mu, tol, max_iter)) return PGM_ERR_ARG;
     const int n3 = 3 * n, nb = nblk(n3, 256);
     DevBuf y, r, z, p, part, out;
     double *h = nullptr;
@@ -2396,48 +2448,60 @@ extern "C" int pgm_solve_linear_induction(int n, const double *T, const double *
         g_err = "hipHostMalloc failed";
         return cleanup(PGM_ERR_HIP);
     }
-    auto scalars = [&](int np, int stride) -> int {
-        k_dense_reduce<<<1, 256, 0, st>>>(part.as<double>(), np, stride, out.as<double>());
-        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(h, out.p, stride * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            g_err = "pgm_solve_linear_induction: device error";
+    return cleanup(dense_solve(n, T, alpha, F, mu, tol, max_iter, (hipStream_t)stream, iterations, residual, y.as<double>(),
+                               r.as<double>(), z.as<double>(), p.as<double>(), part.as<double>(), out.as<double>(), h));
+}
+
+// The dense solve without allocations: its work vectors live in `scratch`, a file-scope device
+// array of fixed size that every GPU holds once for the life of the process.  Host code must never
+// pass `scratch` itself to a kernel or a copy (in host code the name is a stand-in, not a device
+// address): its address on each device comes from hipGetSymbolAddress, taken once per device with
+// that device current and kept in scratch_dev[].  Every vector is written before it is read, so
+// nothing needs clearing between calls; calls that overlap on one GPU share the array and race.
+static constexpr size_t align32(size_t v) { return (v + 31) / 32 * 32; }
+static constexpr size_t DENSE_SCRATCH_DOUBLES =
+    4 * align32(3 * (size_t)PGM_DENSE_NOALLOC_MAX_ATOMS) + align32(2 * (size_t)PGM_DENSE_NOALLOC_MAX_ATOMS) + 32;
+static __device__ __align__(256) double scratch[DENSE_SCRATCH_DOUBLES];
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+static std::mutex scratch_mu;
+
+static int scratch_address(double **out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        g_err = "pgm_solve_linear_induction_noalloc: device index beyond 64";
+        return PGM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
             return PGM_ERR_HIP;
         }
-        return PGM_OK;
-    };
-    double *x = mu;
-    // x0 = alpha F, r = F - A x0, z = alpha r, p = z
-    k_dense_update<<<nb, 256, 0, st>>>(n3, 3, 0.0, 0.0, F, alpha, nullptr, x, nullptr, nullptr, nullptr, part.as<double>());
-    if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
-    const double fnorm = std::sqrt(h[0]);
-    k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, x, y.as<double>(), part.as<double>());
-    k_dense_update<<<nb, 256, 0, st>>>(n3, 0, 0.0, 0.0, F, alpha, y.as<double>(), x, r.as<double>(), z.as<double>(), p.as<double>(),
-                                       part.as<double>());
-    if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
-    double rz = h[0], rr = h[1];
-    int it = 0;
-    // stopping rule of jax.scipy.sparse.linalg.cg: |r| <= tol |F|
-    while (it < max_iter && std::sqrt(rr) > tol * fnorm) {
-        k_dense_matvec<<<n, 256, 0, st>>>(n, T, alpha, p.as<double>(), y.as<double>(), part.as<double>());
-        if ((rc = scalars(n, 1)) < 0) return cleanup(rc);
-        const double al = rz / std::max(h[0], 1e-300);
-        k_dense_update<<<nb, 256, 0, st>>>(n3, 1, al, 0.0, F, alpha, y.as<double>(), x, r.as<double>(), z.as<double>(),
-                                           p.as<double>(), part.as<double>());
-        if ((rc = scalars(nb, 2)) < 0) return cleanup(rc);
-        const double be = h[0] / std::max(rz, 1e-300);
-        rz = h[0];
-        rr = h[1];
-        k_dense_update<<<nb, 256, 0, st>>>(n3, 2, 0.0, be, F, alpha, nullptr, x, nullptr, z.as<double>(), p.as<double>(),
-                                           nullptr);
-        ++it;
+        scratch_dev[dev] = (double *)ptr;
     }
-    if (hipStreamSynchronize(st) != hipSuccess) {
-        g_err = "pgm_solve_linear_induction: device error";
-        return cleanup(PGM_ERR_HIP);
+    *out = scratch_dev[dev];
+    return PGM_OK;
+}
+
+extern "C" int pgm_solve_linear_induction_noalloc(int n, const double *T, const double *alpha, const double *F, double *mu,
+                                                  double tol, int max_iter, void *stream, int *iterations, double *residual) {
+    if (!dense_args_ok(n, T, alpha, F, mu, tol, max_iter)) return PGM_ERR_ARG;
+    if (n > PGM_DENSE_NOALLOC_MAX_ATOMS) {
+        g_err = "pgm_solve_linear_induction_noalloc: n = " + std::to_string(n) + " atoms exceeds the scratch limit of " +
+                std::to_string(PGM_DENSE_NOALLOC_MAX_ATOMS);
+        return PGM_ERR_ARG;
     }
-    if (iterations) *iterations = it;
-    if (residual) *residual = fnorm > 0 ? std::sqrt(rr) / fnorm : 0.0;
-    return cleanup(std::sqrt(rr) > tol * fnorm ? PGM_NOT_CONVERGED : PGM_OK);
+    double *base = nullptr;
+    PGM_TRY(scratch_address(&base));
+    const size_t v = align32(3 * (size_t)n);
+    double *y = base, *r = base + v, *z = base + 2 * v, *p = base + 3 * v, *part = base + 4 * v;
+    double *out = part + align32(2 * (size_t)n);  // part holds 2 max(n, blocks) = 2n doubles (blocks <= n)
+    double h[4] = {0.0, 0.0, 0.0, 0.0};
+    return dense_solve(n, T, alpha, F, mu, tol, max_iter, (hipStream_t)stream, iterations, residual, y, r, z, p, part, out, h);
 }
 
 // ---------------------------------------------------------------------------- MD
