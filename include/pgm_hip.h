@@ -204,6 +204,23 @@ int pgm_energy_forces(pgm_ctx *ctx, const double *pos, const double *mu, pgm_ene
 int pgm_strain_derivative(pgm_ctx *ctx, const double *pos, const double *mu, const double *masses, int molecular,
                           pgm_energy *energy, double *forces, double *strain);
 
+/* The backward pass of the differentiable solve (the custom VJP of PGMForceField._solve,
+ * forcefield.py:1853-1915): the adjoint CG A lambda = mu_bar (the forward operator and
+ * preconditioner, x0 = 0, no peek, until max |alpha r| / mean |alpha mu_bar| <= tol, at most
+ * max_iter iterations), then lambda pulled back through the residual b - A mu at the dipoles mu:
+ *   q_bar = -phi(lambda)          (N)    the potential of the dipoles lambda at every atom
+ *   radius_bar                    (N)    through a_ij = (2 (R_i^2 + R_j^2))^(-1/2) of the pair kernels
+ *   alpha_bar = mu . lambda / alpha^2 (N)
+ *   p_bar = field(0, lambda)      (N,3)  the cotangent of the permanent dipoles
+ *   cov_bar                       (n_cov) p_bar projected on each covalent dipole's bond direction
+ * in the units of the solve (field and potential without the Coulomb constant, as `pgm_field`).
+ * pos, mu (the solved dipoles), mu_bar and every output: f64 device; any output may be NULL.  One
+ * system per context, PME reciprocal space.  Returns PGM_NOT_CONVERGED (results valid) when the
+ * adjoint CG stops at max_iter.  Synchronises. */
+int pgm_dipole_vjp(pgm_ctx *ctx, const double *pos, const double *mu, const double *mu_bar, double tol, double *lambda,
+                   double *q_bar, double *radius_bar, double *alpha_bar, double *p_bar, double *cov_bar, int *iterations,
+                   double *residual);
+
 /* solver.solve_linear_induction (pgm_jax/solver.py:29-62): mu (n,3) f64 device solving
  * (T + diag(1/alpha)) mu = F for T (n,n,3,3) f64 device (zero diagonal blocks), alpha (n), F (n,3),
  * by FP64 Jacobi-preconditioned CG from alpha F until |r| <= tol |F| (the reference solves it by

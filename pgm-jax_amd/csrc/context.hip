@@ -26,6 +26,7 @@
 #include "dense.h"
 #include "mlp.h"
 #include "guess.h"
+#include "adjoint.h"
 #include "pgm_hip.h"
 
 using namespace pgm;
@@ -129,6 +130,7 @@ struct pgm_ctx {
     DevBuf q, alpha64, alpha_r, rh, se, mol, R2;
     DevBuf ljp;  // (rmin/2, sqrt eps, molecule, 0) per atom, double4 (the force pass)
     DevBuf cstart, ci, cj, cc, jstart, jlist;
+    DevBuf corder;  // caller index of entry k of the owner-sorted covalent list
     // geometry: the positions of the current evaluation (caller's device buffer) and xr
     const double *pos = nullptr;
     DevBuf xr;
@@ -194,6 +196,8 @@ struct pgm_ctx {
     CGState *h_pin_conv = nullptr; // CG state read-backs of the pipelined CG (4 slots of nbead states)
     double *h_pin_dbl = nullptr;   // 32 + 5 nbead doubles (bead energies from 32, residuals after them)
     int last_iters = 4;
+    int last_adj_iters = 4;  // iterations of the last adjoint solve (read-back hint)
+    DevBuf adj;              // FP64 work arrays of the adjoint: lam, p_bar (3N each), phi, q_bar, R_bar, alpha_bar (N each)
     bool norm_on_device = false;  // CGState.norm holds the normaliser of the last unfused solve
     DevBuf stats;                 // per-solve final residuals (ring of stats_cap)
     int stats_cap = 4096;
@@ -1298,6 +1302,118 @@ static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
 // residual and the normaliser stay on the device (stats slot `slot`).  With beads every bead runs
 // its own CG in the same launches; the host waits for the slowest, and iterations / residual are
 // the largest over the beads (per bead: bead_it, bead_res).  Bead normalisers stay on the device.
+// The CG iterations after the start kernels, on c->stream: iteration k evaluates A p (the field of
+// p) and runs the two update kernels, which return at once after convergence; the converged flags
+// are read back from about the expected count on, so the host rarely waits.  With `graphs`, one
+// captured hipGraph per iteration index.  Returns the iteration count (launched_out: the iterations
+// queued) or a negative status.
+template <typename real>
+static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *launched_out) {
+    const int B = c->nbead, nper = c->nper;
+    hipStream_t st = c->stream;
+    using R4 = real4<real>;
+    const dim3 gB = bead_grid(c, nper);
+    const dim3 gM(nblk(nper, MOLP_BLOCK), c->nbead);
+    const int npb = c->molp ? nblk(nper, MOLP_BLOCK) : nblk(nper, 256);
+    R4 *x = c->vx.as<R4>(), *r = c->vr.as<R4>(), *z = c->vz.as<R4>(), *p = c->vp.as<R4>(), *Ap = c->vAp.as<R4>();
+    CGState *S = c->st.as<CGState>();
+    double *rz = c->rz.as<double>(), *err = c->err.as<double>();
+    const int max_iter = c->s.max_iter;
+    int launched = 0;
+    int conv = 0x7fffffff;  // last conv_it seen on the host (largest over the beads)
+    auto read_conv = [&](int k) -> int {  // queue the read-back of the CG states after iteration k-1 (slot k)
+        HIP_TRY(hipMemcpyAsync(c->h_pin_conv + (size_t)(k & 3) * B, S, sizeof(CGState) * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(c->ev_read[k & 3], st));
+        return PGM_OK;
+    };
+    auto seen_conv = [&](int k) -> int {  // the slowest bead's conv_it in read slot k (after it completed)
+        int m = 0;
+        for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
+        return m;
+    };
+    // iteration k on c->stream: the field of p, then the two update kernels (all early-exit once converged)
+    auto iteration = [&](int k) -> int {
+        hipStream_t cs = c->stream;
+        PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
+        if (c->molp)
+            k_cg_update1_mol<real><<<gM, MOLP_BLOCK, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z,
+                                                               p, Ap, c->alpha_r.as<real>(), c->pinv.as<real>(),
+                                                               c->partB.as<double>(), S);
+        else
+            k_cg_update1<real><<<gB, 256, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
+                                                   c->alpha_r.as<real>(), c->partB.as<double>(), S);
+        k_cg_update2<real><<<gB, 256, 0, cs>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, tol,
+                                               max_iter);
+        LAUNCH_CHECK();
+        return PGM_OK;
+    };
+    hipStream_t gs = st ? st : c->gstream;  // the null stream cannot be captured: a private stream, ordered by events
+    PGM_TRY(read_conv(0));  // converged before any iteration?
+    while (launched < max_iter) {
+        const int k = launched;
+        if (graphs) {
+            if ((int)c->it_graphs.size() <= k) c->it_graphs.resize(k + 1, nullptr);
+            if (gs != st) {
+                HIP_TRY(hipEventRecord(c->ev_g0, st));
+                HIP_TRY(hipStreamWaitEvent(gs, c->ev_g0, 0));
+            }
+            if (!c->it_graphs[k]) {  // capture iteration k once (on gs)
+                if (gs != st) {
+                    c->stream = gs;
+                    FFT_TRY(rocfft_execution_info_set_stream(c->info, gs));
+                }
+                HIP_TRY(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
+                const int rc = iteration(k);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(gs, &g);
+                if (gs != st) {
+                    c->stream = st;
+                    FFT_TRY(rocfft_execution_info_set_stream(c->info, st));
+                }
+                if (rc < 0) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) {
+                    g_err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+                    return PGM_ERR_HIP;
+                }
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    g_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+                    return PGM_ERR_HIP;
+                }
+                c->it_graphs[k] = ex;
+            }
+            HIP_TRY(hipGraphLaunch(c->it_graphs[k], gs));
+            if (gs != st) {
+                HIP_TRY(hipEventRecord(c->ev_g1, gs));
+                HIP_TRY(hipStreamWaitEvent(st, c->ev_g1, 0));
+            }
+        } else {
+            PGM_TRY(iteration(k));
+        }
+        launched = k + 1;
+        // slot `launched` is waited on at the next pass (launched >= expect - 2) or after the loop;
+        // earlier reads would only add a copy and its gap to the queue
+        if (launched >= expect - 2 || launched == max_iter) PGM_TRY(read_conv(launched));
+        if (launched >= expect - 1 || launched == max_iter) {
+            // wait for the state after iteration k-1 (read slot k) while iteration k runs
+            HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
+            conv = seen_conv(k);
+            if (conv <= k) break;  // every bead converged before iteration k: k was a no-op
+        }
+    }
+    if (conv > launched) {  // not yet seen converged: the state after the last launched iteration decides
+        HIP_TRY(hipEventSynchronize(c->ev_read[launched & 3]));
+        conv = seen_conv(launched);
+    }
+    *launched_out = launched;
+    return std::min(conv, launched);
+}
+
 template <typename real>
 // trial: the CG starts from ind->mu (no predictor, the right-hand side and its norm evaluated
 // afresh), and neither the predictor history nor the learned guess's records change
@@ -1380,100 +1496,10 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     k_cg_start_finish<<<B, 256, 0, st>>>(c->partB.as<double>(), npb, S, rz, err, c->rzs, c->s.dipole_tol, c->s.max_iter);
     LAUNCH_CHECK();
     const int max_iter = c->s.max_iter;
-    const int expect = std::max(1, std::min(c->last_iters, max_iter));
     int launched = 0;
-    int conv = 0x7fffffff;  // last conv_it seen on the host (largest over the beads)
-    auto read_conv = [&](int k) -> int {  // queue the read-back of the CG states after iteration k-1 (slot k)
-        HIP_TRY(hipMemcpyAsync(c->h_pin_conv + (size_t)(k & 3) * B, S, sizeof(CGState) * B, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipEventRecord(c->ev_read[k & 3], st));
-        return PGM_OK;
-    };
-    auto seen_conv = [&](int k) -> int {  // the slowest bead's conv_it in read slot k (after it completed)
-        int m = 0;
-        for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
-        return m;
-    };
-    // iteration k on c->stream: the field of p, then the two update kernels (all early-exit once converged)
-    auto iteration = [&](int k) -> int {
-        hipStream_t cs = c->stream;
-        PGM_TRY(field_eval<real>(c, p, false, MODE_AV, nullptr, Ap, c->partA.as<double>(), k));
-        if (c->molp)
-            k_cg_update1_mol<real><<<gM, MOLP_BLOCK, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z,
-                                                               p, Ap, c->alpha_r.as<real>(), c->pinv.as<real>(),
-                                                               c->partB.as<double>(), S);
-        else
-            k_cg_update1<real><<<gB, 256, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
-                                                   c->alpha_r.as<real>(), c->partB.as<double>(), S);
-        k_cg_update2<real><<<gB, 256, 0, cs>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, c->s.dipole_tol,
-                                               max_iter);
-        LAUNCH_CHECK();
-        return PGM_OK;
-    };
-    const bool graphs = c->use_graphs && !c->timing;
-    hipStream_t gs = st ? st : c->gstream;  // the null stream cannot be captured: a private stream, ordered by events
-    PGM_TRY(read_conv(0));  // converged before any iteration?
-    while (launched < max_iter) {
-        const int k = launched;
-        if (graphs) {
-            if ((int)c->it_graphs.size() <= k) c->it_graphs.resize(k + 1, nullptr);
-            if (gs != st) {
-                HIP_TRY(hipEventRecord(c->ev_g0, st));
-                HIP_TRY(hipStreamWaitEvent(gs, c->ev_g0, 0));
-            }
-            if (!c->it_graphs[k]) {  // capture iteration k once (on gs)
-                if (gs != st) {
-                    c->stream = gs;
-                    FFT_TRY(rocfft_execution_info_set_stream(c->info, gs));
-                }
-                HIP_TRY(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed));
-                const int rc = iteration(k);
-                hipGraph_t g = nullptr;
-                const hipError_t ec = hipStreamEndCapture(gs, &g);
-                if (gs != st) {
-                    c->stream = st;
-                    FFT_TRY(rocfft_execution_info_set_stream(c->info, st));
-                }
-                if (rc < 0) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc;
-                }
-                if (ec != hipSuccess) {
-                    g_err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
-                    return PGM_ERR_HIP;
-                }
-                hipGraphExec_t ex = nullptr;
-                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (ei != hipSuccess) {
-                    g_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
-                    return PGM_ERR_HIP;
-                }
-                c->it_graphs[k] = ex;
-            }
-            HIP_TRY(hipGraphLaunch(c->it_graphs[k], gs));
-            if (gs != st) {
-                HIP_TRY(hipEventRecord(c->ev_g1, gs));
-                HIP_TRY(hipStreamWaitEvent(st, c->ev_g1, 0));
-            }
-        } else {
-            PGM_TRY(iteration(k));
-        }
-        launched = k + 1;
-        // slot `launched` is waited on at the next pass (launched >= expect - 2) or after the loop;
-        // earlier reads would only add a copy and its gap to the queue
-        if (launched >= expect - 2 || launched == max_iter) PGM_TRY(read_conv(launched));
-        if (launched >= expect - 1 || launched == max_iter) {
-            // wait for the state after iteration k-1 (read slot k) while iteration k runs
-            HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
-            conv = seen_conv(k);
-            if (conv <= k) break;  // every bead converged before iteration k: k was a no-op
-        }
-    }
-    if (conv > launched) {  // not yet seen converged: the state after the last launched iteration decides
-        HIP_TRY(hipEventSynchronize(c->ev_read[launched & 3]));
-        conv = seen_conv(launched);
-    }
-    const int it = std::min(conv, launched);
+    const int it = cg_iterate<real>(c, c->s.dipole_tol, std::max(1, std::min(c->last_iters, max_iter)),
+                                    c->use_graphs && !c->timing, &launched);
+    if (it < 0) return it;
     c->last_iters = std::max(1, it);
     k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
     if (use_net && ind->count % c->guess.every == 0) PGM_TRY(guess_harvest(c, ind));
@@ -1804,6 +1830,9 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = upload(c->cc, cc.data(), nc, st)) < 0) return fail(r);
     if ((r = upload(c->jstart, js.data(), n + 1, st)) < 0) return fail(r);
     if ((r = upload(c->jlist, jl.data(), nc, st)) < 0) return fail(r);
+    std::vector<int> corder(nc);
+    for (int k = 0; k < nc; ++k) corder[k] = order[k % nc1] + (k / nc1) * nc1;
+    if ((r = upload(c->corder, corder.data(), nc, st)) < 0) return fail(r);
     // buffers
     const size_t r4 = 4 * c->rsize * (size_t)n, d3 = sizeof(double) * 3 * (size_t)n;
     DevBuf *v4[] = {&c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp};
@@ -1923,7 +1952,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (c->bwd) rocfft_plan_destroy(c->bwd);
     if (c->info) rocfft_execution_info_destroy(c->info);
     DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->ljp, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
-                     &c->jstart, &c->jlist, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
+                     &c->jstart, &c->jlist, &c->corder, &c->adj, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
@@ -2179,6 +2208,98 @@ extern "C" int pgm_strain_derivative(pgm_ctx *c, const double *pos, const double
     PGM_TRY(single_system(c, "pgm_strain_derivative"));
     return c->mixed ? ef_api<float>(c, pos, mu, energy, forces, masses, strain, molecular != 0)
                     : ef_api<double>(c, pos, mu, energy, forces, masses, strain, molecular != 0);
+}
+
+// ---------------------------------------------------------------------------- backward pass of the solve
+// The custom VJP of PGMForceField._solve (forcefield.py:1853-1915): the adjoint CG A lam = mu_bar
+// (x0 = 0, no peek, to `tol`, the same operator and preconditioner as the forward solve), then lam
+// pulled back through the residual b - A mu at the dipoles mu (adjoint.h).  Every output pointer
+// may be null.
+template <typename real>
+static int dipole_vjp_impl(pgm_ctx *c, const double *pos, const double *mu, const double *mu_bar, double tol, double *lambda,
+                           double *q_bar, double *radius_bar, double *alpha_bar, double *p_bar, double *cov_bar,
+                           int *iterations, double *residual) {
+    using R4 = real4<real>;
+    const int n = c->n, nper = c->nper;
+    hipStream_t st = c->stream;
+    const int nb = nblk(n, 256);
+    PGM_TRY(prepare(c, pos));
+    k_perm_dipoles<<<nb, 256, 0, st>>>(n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(), c->cc.as<double>(),
+                                       c->pperm.as<double>());
+    LAUNCH_CHECK();
+    PGM_TRY(c->adj.ensure(sizeof(double) * 10 * (size_t)n));
+    double *lam = c->adj.as<double>(), *pb = lam + 3 * (size_t)n, *phi = pb + 3 * (size_t)n;
+    double *qb = q_bar ? q_bar : phi + n, *rb = radius_bar ? radius_bar : phi + 2 * (size_t)n;
+    double *ab = alpha_bar ? alpha_bar : phi + 3 * (size_t)n;
+    // adjoint CG: r = mu_bar, x = 0, normaliser mean |alpha mu_bar|
+    R4 *x = c->vx.as<R4>(), *r = c->vr.as<R4>(), *z = c->vz.as<R4>(), *p = c->vp.as<R4>();
+    CGState *S = c->st.as<CGState>();
+    const int npb = c->molp ? nblk(nper, MOLP_BLOCK) : nblk(nper, 256);
+    k_adjoint_norm<<<1, 256, 0, st>>>(n, mu_bar, c->alpha64.as<double>(), S);
+    k_mu_to_x0<real><<<nb, 256, 0, st>>>(n, mu_bar, r);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemsetAsync(x, 0, sizeof(R4) * n, st));
+    if (c->molp)
+        k_cg_start_mol<real><<<dim3(nblk(nper, MOLP_BLOCK), 1), MOLP_BLOCK, 0, st>>>(nper, r, c->alpha_r.as<real>(),
+                                                                                c->pinv.as<real>(), z, p, c->partB.as<double>());
+    else
+        k_cg_start<real><<<bead_grid(c, nper), 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
+    k_cg_start_finish<<<1, 256, 0, st>>>(c->partB.as<double>(), npb, S, c->rz.as<double>(), c->err.as<double>(), c->rzs, tol,
+                                         c->s.max_iter);
+    LAUNCH_CHECK();
+    c->norm_on_device = false;  // CGState.norm now holds the adjoint's normaliser
+    int launched = 0;
+    const int it = cg_iterate<real>(c, tol, std::max(1, std::min(c->last_adj_iters, c->s.max_iter)), false, &launched);
+    if (it < 0) return it;
+    c->last_adj_iters = std::max(1, it);
+    k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), 0.0, lam);
+    LAUNCH_CHECK();
+    // real-space potential and radius terms (FP64, atom rows)
+    if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));
+    k_adjoint_rows<<<nblk((long long)n * ADJ_RL, 256), 256, 0, st>>>(
+        n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box, c->s.ewald_beta,
+        c->rc_e * c->rc_e, c->q.as<double>(), lam, c->pperm.as<double>(), mu, phi, rb);
+    k_adjoint_alpha<<<nb, 256, 0, st>>>(n, mu, lam, c->alpha64.as<double>(), ab);
+    LAUNCH_CHECK();
+    // p_bar = field(0, lam)
+    R4 *dq = c->vdq.as<R4>();
+    k_pack_dq<real><<<nb, 256, 0, st>>>(n, lam, nullptr, nullptr, dq);
+    LAUNCH_CHECK();
+    PGM_TRY(field_eval<real>(c, dq, false, MODE_FIELD, nullptr, c->vtmp.as<R4>(), nullptr, -1));
+    k_to_double3<real><<<nb, 256, 0, st>>>(n, c->vtmp.as<R4>(), pb);
+    LAUNCH_CHECK();
+    // q_bar: the reciprocal potential of lam at the atoms plus the real-space part
+    PGM_TRY(spread_fft_only<real>(c, dq));
+    ORDER_SWITCH(c->s.order, (k_adjoint_potential<real, PO><<<nb, 256, 0, st>>>(n, pos, c->pg, c->grid.as<real>(), phi, qb)));
+    if (cov_bar && c->ncov > 0)
+        k_adjoint_cov<<<nblk(c->ncov, 256), 256, 0, st>>>(c->ncov, pos, c->box, c->ci.as<int>(), c->cj.as<int>(),
+                                                          c->corder.as<int>(), pb, cov_bar);
+    LAUNCH_CHECK();
+    if (lambda) HIP_TRY(hipMemcpyAsync(lambda, lam, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    if (p_bar) HIP_TRY(hipMemcpyAsync(p_bar, pb, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->h_pin_dbl, c->err.as<double>() + it, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (iterations) *iterations = it;
+    if (residual) *residual = c->h_pin_dbl[0];
+    return (it >= c->s.max_iter && c->h_pin_dbl[0] > tol) ? PGM_NOT_CONVERGED : PGM_OK;
+}
+
+extern "C" int pgm_dipole_vjp(pgm_ctx *c, const double *pos, const double *mu, const double *mu_bar, double tol, double *lambda,
+                              double *q_bar, double *radius_bar, double *alpha_bar, double *p_bar, double *cov_bar,
+                              int *iterations, double *residual) {
+    if (!c || !pos || !mu || !mu_bar || !(tol > 0)) {
+        g_err = "pgm_dipole_vjp: null argument or tol <= 0";
+        return PGM_ERR_ARG;
+    }
+    PGM_TRY(single_system(c, "pgm_dipole_vjp"));
+    if (c->ewald) {
+        g_err = "pgm_dipole_vjp: PME reciprocal space only (this context holds the exact Ewald sum)";
+        return PGM_ERR_STATE;
+    }
+    return c->mixed ? dipole_vjp_impl<float>(c, pos, mu, mu_bar, tol, lambda, q_bar, radius_bar, alpha_bar, p_bar, cov_bar,
+                                             iterations, residual)
+                    : dipole_vjp_impl<double>(c, pos, mu, mu_bar, tol, lambda, q_bar, radius_bar, alpha_bar, p_bar, cov_bar,
+                                              iterations, residual);
 }
 
 // ---------------------------------------------------------------------------- learned guess (ABI)

@@ -278,6 +278,39 @@ class PGMForceField:
                                         C.c_void_p(out.data_ptr())), "pgm_field")
         return out
 
+    def dipole_vjp(self, pos, H, mu, mu_bar, tol: float | None = None) -> dict:
+        """The backward pass of the differentiable solve (the custom VJP of PGMForceField._solve,
+        forcefield.py:1853-1915), through pgm_dipole_vjp: the adjoint CG A lam = mu_bar (the
+        forward operator, x0 = 0, no peek, to `tol`, default settings.adjoint_tol), then lam pulled
+        back through the residual b - A mu at the solved dipoles mu.
+
+        Returns {"lam" (N,3), "q" (N,), "radius" (N,), "alpha" (N,), "p" (N,3), "cov" (n_cov,)}: the
+        cotangents of the charges, Gaussian radii, polarizabilities, permanent dipoles and covalent
+        dipole magnitudes (device float64 tensors; field and potential without the Coulomb constant,
+        as `field`), plus "iterations" and "residual" of the adjoint CG.  Raises on a status < 0;
+        a CG that stops at max_iter returns its results (status 3, as Result.residual reports).
+        """
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        mu = _dev(mu, (self.n, 3))
+        mu_bar = _dev(mu_bar, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        tol = float(self.s.adjoint_tol if tol is None else tol)
+        f64 = dict(dtype=torch.float64, device="cuda")
+        out = {"lam": torch.empty((self.n, 3), **f64), "q": torch.empty(self.n, **f64), "radius": torch.empty(self.n, **f64),
+               "alpha": torch.empty(self.n, **f64), "p": torch.empty((self.n, 3), **f64),
+               "cov": torch.empty(max(len(self.sys.cov_i), 1), **f64)}
+        it, res = C.c_int(), C.c_double()
+        _lib.check(_lib.lib().pgm_dipole_vjp(self._ctx, C.c_void_p(pos.data_ptr()), C.c_void_p(mu.data_ptr()),
+                                             C.c_void_p(mu_bar.data_ptr()), tol,
+                                             *(C.c_void_p(out[k].data_ptr()) for k in ("lam", "q", "radius", "alpha", "p", "cov")),
+                                             C.byref(it), C.byref(res)),
+                   "pgm_dipole_vjp")
+        out["cov"] = out["cov"][:len(self.sys.cov_i)]
+        out["iterations"], out["residual"] = it.value, res.value
+        return out
+
     def pme_energy(self, pos, d, with_charges: bool = True, H=None) -> tuple[float, object]:
         """Return (E_rec [e^2/nm], dE_rec/dd (N,3)) of PME (pgm_jax/md/pme.py:339-411)."""
         torch = _torch()

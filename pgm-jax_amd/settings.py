@@ -26,6 +26,8 @@ class MDSettings:
     elec_cutoff [nm] (None: cutoff) is the real-space electrostatics cutoff of the reference's split
     rows (forcefield.py:44-58): electrostatic pair terms stop there, LJ at cutoff; choose
     ewald_beta and the PME spacing for it (elec_cutoff_settings).
+    adjoint_tol: the CG tolerance of the backward pass of the solve (PGMForceField.dipole_vjp;
+    MDSettings.adjoint_tol, forcefield.py:414-425).
     """
 
     elec: str = "qpi"
@@ -48,6 +50,7 @@ class MDSettings:
     k_tol: float = 1e-12
     precond: str = "jacobi"
     elec_cutoff: float | None = None
+    adjoint_tol: float = 1e-6
 
     def __post_init__(self) -> None:
         """Validate like PGMForceField.__init__ (forcefield.py:917-1000)."""
@@ -67,6 +70,8 @@ class MDSettings:
             raise ValueError("k_tol must be in (0, 1)")
         if self.precond not in ("jacobi", "molecule"):
             raise ValueError(f"precond must be 'jacobi' or 'molecule', got {self.precond!r}")
+        if not self.adjoint_tol > 0:
+            raise ValueError(f"adjoint_tol must be positive, got {self.adjoint_tol}")
         if self.elec_cutoff is not None and not self.elec_cutoff > 0:
             raise ValueError(f"elec_cutoff must be positive, got {self.elec_cutoff}")
 
