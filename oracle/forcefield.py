@@ -130,6 +130,7 @@ class ForceField:
         i, j = g["i"], g["j"]
         a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
         inside = g["r"] < self.rc_e  # split rows (forcefield.py:1197-1238): electrostatics within rc_e only
+        g["inside"] = inside
         g["G"] = tuple(Gn * inside for Gn in pair_kernels(a, self.b0, g["r"], 4))
         # rigid topology: intramolecular vdW weight 0; van der Waals only within rc
         g["wv"] = np.where(g["intra"] | (g["r"] >= self.rc), 0.0, 1.0)
@@ -476,3 +477,117 @@ class ForceField:
         cov_bar = np.sum(p_bar[t.cov_i] * u, -1)
         return {"lam": lam, "iterations": it, "q": q_bar, "alpha": alpha_bar, "p": p_bar, "radius": R_bar, "cov": cov_bar}
 
+    # ------------------------------------------------------------------ parameter derivatives
+    def geometry_frozen(self, pos: np.ndarray, H: np.ndarray, g0: dict, pos0: np.ndarray) -> dict:
+        """The pairs of g0 (built at pos0) at positions pos: the same pairs, lattice shifts and cutoff
+        masks, the kernels evaluated at the new distances.  Autodiff of the reference's masked pair
+        sums (forcefield.py:2232-2292) differentiates exactly this: the masks carry no derivative."""
+        t = self.top
+        i, j = g0["i"], g0["j"]
+        x = pos[i] - pos[j] + (g0["x"] - (pos0[i] - pos0[j]))
+        r = np.sqrt(np.sum(x * x, -1))
+        a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
+        G = tuple(Gn * g0["inside"] for Gn in pair_kernels(a, self.b0, r, 4))
+        return dict(g0, x=x, r=r, G=G)
+
+    def param_grad(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, g: dict) -> dict:
+        """Return dE/dtheta at fixed mu [kJ/mol per unit] for every per-atom parameter and covalent
+        dipole strength (the Hellmann-Feynman derivative: E is stationary in mu), and the total field
+        -dU/dd (N, 3) of q and d = p + mu.  E = KE [pairs + U_rec + U_self + U_bg + U_pol] + LJ + tail
+        (forcefield.py:2011-2045, 2294-2430; lj.py:87-114):
+          q_i:      KE [sum_k (q_k G0 + d_k.x G1) + phi_rec_i - 2 beta q_i / sqrt(pi) - pi Q / (V beta^2)]
+          R_i:      KE sum_k de_ik/da da/dR_i,  da/dR_i = -2 R_i a^3,  dG_n/da = (2/sqrt(pi)) (2a^2)^n exp(-a^2 r^2)
+          alpha_i:  -KE |mu_i|^2 / (2 alpha_i^2)
+          c_m:      dE/dp_i . u_m = -KE field_i . u_m
+          R*_i/2:   sum_k 12 eps (s^12 - s^6) / rmin (+ tail),  sqrt(eps_i): sum_k sqrt(eps_k) (s^12 - 2 s^6) (+ tail)
+        with x = r_i - r_k, s = rmin / r and the pairs of g."""
+        t = self.top
+        n = t.n
+        q = t.q
+        p = self.perm_dipoles(pos, H)
+        d = p + mu
+        S = self.pme.setup(pos, H)
+        Gk = self.pme.influence(H)
+        i, j, x, r = g["i"], g["j"], g["x"], g["r"]
+        G0, G1, G2, _ = g["G"]
+        dix, djx = np.sum(d[i] * x, -1), np.sum(d[j] * x, -1)
+        didj = np.sum(d[i] * d[j], -1)
+        gq = np.zeros(n)
+        np.add.at(gq, i, q[j] * G0 + djx * G1)
+        np.add.at(gq, j, q[i] * G0 - dix * G1)
+        phi = self.pme.potential(S, Gk, q, d)
+        P = phi.reshape(-1)[self.pme._flat(S)].reshape(n, *(self.pme.order,) * 3)
+        th = [S["th"][:, a] for a in range(3)]
+        rec = np.einsum("nabc,na,nb,nc->n", P, th[0], th[1], th[2], optimize=True)
+        V = abs(np.linalg.det(H))
+        gq = KE * (gq + rec - 2.0 * self.b0 * q / SQRT_PI - math.pi * np.sum(q) / (V * self.b0**2))
+        a = 1.0 / np.sqrt(2.0 * (t.radius[i] ** 2 + t.radius[j] ** 2))
+        H0 = (2.0 / SQRT_PI) * np.exp(-((a * r) ** 2)) * g["inside"]
+        H1 = H0 * 2.0 * a * a
+        H2 = H1 * 2.0 * a * a
+        dea = q[i] * q[j] * H0 + (q[i] * djx - q[j] * dix) * H1 - H2 * dix * djx + H1 * didj
+        gR = np.zeros(n)
+        np.add.at(gR, i, -2.0 * t.radius[i] * a**3 * dea)
+        np.add.at(gR, j, -2.0 * t.radius[j] * a**3 * dea)
+        galpha = -KE * div_alpha(np.sum(mu * mu, -1), 2.0 * t.alpha**2)
+        field = self.field(g, S, Gk, q, d)
+        v = min_image(pos[t.cov_j] - pos[t.cov_i], H)
+        u = v / np.sqrt(np.sum(v * v, -1, keepdims=True))
+        gcov = -KE * np.sum(field[t.cov_i] * u, -1)
+        on = g["wv"] != 0
+        rmin, eps = g["rmin"], g["eps"]
+        s6 = np.where(on, (rmin / r) ** 6, 0.0)
+        drmin = np.where(on & (eps != 0), 12.0 * eps * (s6 * s6 - s6) / np.where(rmin > 0, rmin, 1.0), 0.0)
+        grh, gse = np.zeros(n), np.zeros(n)
+        np.add.at(grh, i, drmin)
+        np.add.at(grh, j, drmin)
+        ese = s6 * s6 - 2.0 * s6
+        np.add.at(gse, i, t.lj_sqrt_eps[j] * ese)
+        np.add.at(gse, j, t.lj_sqrt_eps[i] * ese)
+        if self.s.lj_lrc:  # E_lrc = -2 pi SB / (3 V rc^3), SB = 2 sum_k C(6,k) M_k M_(6-k), M_k = sum se rh^k
+            rh, se = t.lj_rmin_half, t.lj_sqrt_eps
+            M = [np.sum(se * rh**k) for k in range(7)]
+            c0 = -2.0 * math.pi / (3.0 * V * self.rc**3)
+            for k in range(7):
+                dSB = 4.0 * math.comb(6, k) * M[6 - k]
+                gse += c0 * dSB * rh**k
+                if k:
+                    grh += c0 * dSB * k * se * rh ** (k - 1)
+        return {"q": gq, "radius": KE * gR, "alpha": galpha, "lj_rmin_half": grh, "lj_sqrt_eps": gse, "cov": gcov,
+                "field": field}
+
+    def compute_vjp(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, e_bar: float = 0.0, f_bar=None, mu_bar=None,
+                    tol: float = 1e-12, step: float = 1e-4) -> dict:
+        """Return the parameter cotangents of (E, F, mu) of the differentiable force field
+        (forcefield.py:1853-1915, 2489-2560 with differentiable=True): for a loss with dL/dE = e_bar,
+        dL/dF = f_bar (N, 3), dL/dmu = mu_bar (N, 3) at the solved dipoles mu,
+
+          theta_bar = e_bar dE/dtheta + f_bar . dF/dtheta|_mu + mu_bar_eff . dmu/dtheta,
+          f_bar . dF/dtheta|_mu = -D_v (dE/dtheta),   mu_bar_eff = mu_bar + KE D_v field,   v = f_bar,
+
+        with D_v the derivative along r -> r + eps v at fixed mu and fixed pair set (F = -dE/dr at
+        fixed mu, dF/dmu = KE dfield/dr), and the last term the adjoint solve (dipole_vjp).  D_v is
+        taken here by fourth-order central differences (largest displacement `step` nm, the pairs and
+        masks of the unperturbed geometry).  Returns per-atom arrays and "cov" (per covalent dipole)."""
+        pos, H, mu = np.asarray(pos, float), np.asarray(H, float), np.asarray(mu, float)
+        keys = ("q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps", "cov")
+        g0 = self.geometry(pos, H)
+        base = self.param_grad(pos, H, mu, g0)
+        out = {k: e_bar * base[k] for k in keys}
+        mub = np.zeros_like(mu) if mu_bar is None else np.array(mu_bar, float)
+        if f_bar is not None and np.any(f_bar):
+            v = np.asarray(f_bar, float)
+            h = step / np.abs(v).max()
+            f = {}
+            for sgn in (1, -1, 2, -2):
+                x = pos + sgn * h * v
+                f[sgn] = self.param_grad(x, H, mu, self.geometry_frozen(x, H, g0, pos))
+            D = {k: (8.0 * (f[1][k] - f[-1][k]) - (f[2][k] - f[-2][k])) / (12.0 * h) for k in keys + ("field",)}
+            for k in keys:
+                out[k] = out[k] - D[k]
+            mub = mub + KE * D["field"]
+        if np.any(mub):
+            adj = self.dipole_vjp(pos, H, mu, mub, tol=tol)
+            for k in ("q", "radius", "alpha", "cov"):
+                out[k] = out[k] + adj[k]
+        return out
