@@ -221,6 +221,22 @@ int pgm_dipole_vjp(pgm_ctx *ctx, const double *pos, const double *mu, const doub
                    double *q_bar, double *radius_bar, double *alpha_bar, double *p_bar, double *cov_bar, int *iterations,
                    double *residual);
 
+/* Parameter cotangents of the differentiable force field's outputs (PGMForceField.compute with
+ * differentiable=True and a parameter pytree, forcefield.py:1853-1915, 2489-2560): for a loss with
+ * dL/dE = e_bar (the total energy, host scalar), dL/dF = f_bar (N,3) and dL/dmu = mu_bar (N,3) at
+ * the induced dipoles mu solved at pos, the per-atom cotangents
+ *   theta_bar = e_bar dE/dtheta + f_bar . dF/dtheta + mu_bar . dmu/dtheta
+ * of the charges, Gaussian radii, polarizabilities, LJ rmin/2 and sqrt(eps) (N each) and of the
+ * covalent-dipole strengths (n_cov).  The forces' part takes the derivatives of dE/dtheta and of
+ * the field along f_bar (forward mode, the pair set and cutoff masks held fixed, as autodiff of the
+ * reference's masked sums); the dipoles' part is the adjoint solve of mu_bar + KE D_f_bar field to
+ * `tol` (pgm_dipole_vjp).  f_bar and mu_bar may be NULL (zero); any output may be NULL.  Summing
+ * the cotangents over the atoms of each tying key gives the gradient of the reference's parameter
+ * table.  Device f64 arrays; one system per context, PME reciprocal space.  Synchronises. */
+int pgm_compute_vjp(pgm_ctx *ctx, const double *pos, const double *mu, double e_bar, const double *f_bar,
+                    const double *mu_bar, double tol, double *q_bar, double *radius_bar, double *alpha_bar,
+                    double *rmin_half_bar, double *sqrt_eps_bar, double *cov_bar, int *iterations, double *residual);
+
 /* solver.solve_linear_induction (pgm_jax/solver.py:29-62): mu (n,3) f64 device solving
  * (T + diag(1/alpha)) mu = F for T (n,n,3,3) f64 device (zero diagonal blocks), alpha (n), F (n,3),
  * by FP64 Jacobi-preconditioned CG from alpha F until |r| <= tol |F| (the reference solves it by

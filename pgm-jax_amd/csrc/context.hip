@@ -198,6 +198,7 @@ struct pgm_ctx {
     int last_iters = 4;
     int last_adj_iters = 4;  // iterations of the last adjoint solve (read-back hint)
     DevBuf adj;              // FP64 work arrays of the adjoint: lam, p_bar (3N each), phi, q_bar, R_bar, alpha_bar (N each)
+    DevBuf adj2;             // FP64 work arrays of the parameter cotangents (compute_vjp_impl)
     bool norm_on_device = false;  // CGState.norm holds the normaliser of the last unfused solve
     DevBuf stats;                 // per-solve final residuals (ring of stats_cap)
     int stats_cap = 4096;
@@ -1952,7 +1953,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (c->bwd) rocfft_plan_destroy(c->bwd);
     if (c->info) rocfft_execution_info_destroy(c->info);
     DevBuf *all[] = {&c->q, &c->alpha64, &c->alpha_r, &c->rh, &c->se, &c->mol, &c->ljp, &c->R2, &c->cstart, &c->ci, &c->cj, &c->cc,
-                     &c->jstart, &c->jlist, &c->corder, &c->adj, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
+                     &c->jstart, &c->jlist, &c->corder, &c->adj, &c->adj2, &c->xr, &c->rows_full, &c->count_full, &c->rows, &c->count, &c->count_near, &c->maxcount, &c->jell, &c->G1, &c->G2, &c->key,
                      &c->atom, &c->skey, &c->satom, &c->cell_start, &c->cell_end, &c->sort_tmp, &c->grid, &c->cgrid, &c->Gc,
                      &c->fft_work, &c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp,
                      &c->pperm, &c->dd, &c->g_el, &c->g_lj, &c->rowf, &c->rec_dr, &c->gp, &c->partA, &c->partB, &c->partC,
@@ -2300,6 +2301,156 @@ extern "C" int pgm_dipole_vjp(pgm_ctx *c, const double *pos, const double *mu, c
                                              iterations, residual)
                     : dipole_vjp_impl<double>(c, pos, mu, mu_bar, tol, lambda, q_bar, radius_bar, alpha_bar, p_bar, cov_bar,
                                               iterations, residual);
+}
+
+// The library transform of the grid in place (rocFFT r2c, x Gc, c2r): phi of the grid's Q, one bead.
+template <typename real> static int library_transform(pgm_ctx *c) {
+    hipStream_t st = c->stream;
+    void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
+    FFT_TRY(rocfft_execute(c->fwd, in, out, c->info));
+    k_convolve<real, false><<<dim3(nblk((long long)c->ncplx, 256), 1), 256, 0, st>>>(
+        c->ncplx, c->s.grid[2] / 2 + 1, c->s.grid[2] % 2 == 0, c->cgrid.as<real>(), c->Gc.as<real>(), nullptr, nullptr, -1);
+    LAUNCH_CHECK();
+    void *in2[1] = {c->cgrid.p}, *out2[1] = {c->grid.p};
+    FFT_TRY(rocfft_execute(c->bwd, in2, out2, c->info));
+    return PGM_OK;
+}
+
+// Parameter cotangents of (E, F, mu) of the differentiable force field (forcefield.py:1853-1915,
+// 2489-2560 with differentiable=True): ebar dE/dtheta + f_bar . dF/dtheta + mu_bar . dmu/dtheta with
+// f_bar . dF/dtheta = -D_v dE/dtheta + (KE D_v field) . dmu/dtheta (v = f_bar; adjoint.h), the
+// dmu/dtheta terms by the adjoint solve of mu_bar + KE D_v field.
+template <typename real>
+static int compute_vjp_impl(pgm_ctx *c, const double *pos, const double *mu, double ebar, const double *f_bar,
+                            const double *mu_bar, double tol, double *q_bar, double *radius_bar, double *alpha_bar,
+                            double *rh_bar, double *se_bar, double *cov_bar, int *iterations, double *residual) {
+    using R4 = real4<real>;
+    const int n = c->n, ncov = c->ncov;
+    const size_t N = (size_t)n;
+    hipStream_t st = c->stream;
+    const int nb = nblk(n, 256);
+    PGM_TRY(prepare(c, pos));
+    k_perm_dipoles<<<nb, 256, 0, st>>>(n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(), c->cc.as<double>(),
+                                       c->pperm.as<double>());
+    LAUNCH_CHECK();
+    // work arrays: d, D_v p (3N each), row sums (16N), reciprocal sums (8N), field and D_v field (6N),
+    // mu_eff (3N), tail (2N), 5 N-vectors of cotangents without the solve part, the solve's 4 + n_cov,
+    // a copy of the potential grid
+    const size_t ngrid_d = (c->ngrid * sizeof(real) + sizeof(double) - 1) / sizeof(double);
+    PGM_TRY(c->adj2.ensure(sizeof(double) * (50 * N + 2 * (size_t)ncov + ngrid_d + 16)));
+    double *d = c->adj2.as<double>(), *pdot = d + 3 * N, *rowsum = pdot + 3 * N, *rec = rowsum + 16 * N;
+    double *fld = rec + 8 * N, *mu_eff = fld + 6 * N, *tail = mu_eff + 3 * N, *own = tail + 2 * N;
+    double *sol = own + 5 * N, *cov_tmp = sol + 4 * N, *cov_sol = cov_tmp + ncov;
+    real *gridP = reinterpret_cast<real *>(cov_sol + ncov);
+    if (reinterpret_cast<char *>(gridP + c->ngrid) > static_cast<char *>(c->adj2.p) + c->adj2.bytes) {
+        g_err = "pgm_compute_vjp: work array layout exceeds its allocation";
+        return PGM_ERR_STATE;
+    }
+    double *qb = q_bar ? q_bar : own, *rb = radius_bar ? radius_bar : own + N, *ab = alpha_bar ? alpha_bar : own + 2 * N;
+    double *rhb = rh_bar ? rh_bar : own + 3 * N, *seb = se_bar ? se_bar : own + 4 * N;
+    const double *v = f_bar;
+    k_add_vec<<<nblk(3 * (long long)n, 256), 256, 0, st>>>(3 * n, c->pperm.as<double>(), mu, d);
+    if (v) k_perm_tangent<<<nb, 256, 0, st>>>(n, pos, v, c->box, c->cstart.as<int>(), c->cj.as<int>(), c->cc.as<double>(), pdot);
+    LAUNCH_CHECK();
+    // real-space sums over the atom rows
+    if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));
+    k_param_rows<<<nblk((long long)n * ADJ_RL, 256), 256, 0, st>>>(
+        n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box, c->s.ewald_beta,
+        c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->q.as<double>(), c->ljp.as<double4>(), d, pdot, v, rowsum);
+    LAUNCH_CHECK();
+    // reciprocal potential of (q, d) and its D_v
+    R4 *dq = c->vdq.as<R4>();
+    k_pack_dq<real><<<nb, 256, 0, st>>>(n, d, nullptr, c->q.as<double>(), dq);
+    LAUNCH_CHECK();
+    {
+        const bool timing = c->timing;
+        c->timing = false;
+        const int rs = spread_fft<real>(c, dq, true, false, nullptr, -1);
+        c->timing = timing;
+        if (rs < 0) return rs;
+    }
+    HIP_TRY(hipMemcpyAsync(gridP, c->grid.p, c->ngrid * sizeof(real), hipMemcpyDeviceToDevice, st));
+    if (v) {
+        k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
+        ORDER_SWITCH(c->s.order, (k_spread_tangent<real, PO><<<nb, 256, 0, st>>>(n, pos, v, c->pg, c->q.as<double>(), d, pdot,
+                                                                                 c->grid.as<real>())));
+        LAUNCH_CHECK();
+        PGM_TRY(library_transform<real>(c));
+    }
+    ORDER_SWITCH(c->s.order, (k_gather_tangent<real, PO><<<nb, 256, 0, st>>>(n, pos, v, c->pg, gridP,
+                                                                             v ? c->grid.as<real>() : nullptr, rec)));
+    LAUNCH_CHECK();
+    // LJ tail: dE_lrc/d(R*/2), dE_lrc/d sqrt(eps) per atom (E_lrc = -2 pi SB / (3 V rc^3), SB = 2 sum_k
+    // C(6,k) M_k M_(6-k), M_k = sum sqrt(eps) (R*/2)^k)
+    std::vector<double> ht;
+    if (ebar != 0.0 && c->s.lj_lrc) {
+        double M[7] = {0};
+        for (int i = 0; i < c->nper; ++i) {
+            double pw = c->h_se[i];
+            for (int k = 0; k < 7; ++k) {
+                M[k] += pw;
+                pw *= c->h_rh[i];
+            }
+        }
+        static const double C6[7] = {1, 6, 15, 20, 15, 6, 1};
+        const double c0 = -2.0 * M_PI / (3.0 * c->volume * c->s.cutoff * c->s.cutoff * c->s.cutoff);
+        ht.assign(2 * N, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const double rh = c->h_rh[i], se = c->h_se[i];
+            double pk = 1.0, pk1 = 0.0;  // rh^k, rh^(k-1)
+            for (int k = 0; k < 7; ++k) {
+                const double dSB = 4.0 * C6[k] * M[6 - k];
+                ht[2 * i + 1] += c0 * dSB * pk;
+                if (k) ht[2 * i] += c0 * dSB * k * se * pk1;
+                pk1 = pk;
+                pk *= rh;
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(tail, ht.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, st));
+    }
+    const double b0 = c->s.ewald_beta;
+    k_vjp_atoms<<<nb, 256, 0, st>>>(n, rowsum, rec, c->q.as<double>(), c->xr.as<double4>(), c->alpha64.as<double>(), mu, d,
+                                    v ? pdot : nullptr, mu_bar, ht.empty() ? nullptr : tail, ebar,
+                                    -M_PI * c->sum_q / (c->volume * b0 * b0), -2.0 * b0 / SQRT_PI, c_self(c), qb, rb, ab, rhb,
+                                    seb, fld, mu_eff);
+    if (ncov > 0)
+        k_vjp_cov<<<nblk(ncov, 256), 256, 0, st>>>(ncov, pos, v, c->box, c->ci.as<int>(), c->cj.as<int>(), c->corder.as<int>(),
+                                                    fld, ebar, cov_bar ? cov_bar : cov_tmp);
+    LAUNCH_CHECK();
+    int it = 0;
+    double res = 0.0;
+    int rc = PGM_OK;
+    if (v || mu_bar) {  // the induced dipoles' part: the adjoint solve of mu_eff
+        rc = dipole_vjp_impl<real>(c, pos, mu, mu_eff, tol, nullptr, sol, sol + N, sol + 2 * N, nullptr, cov_sol, &it, &res);
+        if (rc < 0) return rc;
+        k_add<<<nb, 256, 0, st>>>(n, sol, qb);
+        k_add<<<nb, 256, 0, st>>>(n, sol + N, rb);
+        k_add<<<nb, 256, 0, st>>>(n, sol + 2 * N, ab);
+        if (cov_bar && ncov > 0) k_add<<<nblk(ncov, 256), 256, 0, st>>>(ncov, cov_sol, cov_bar);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (iterations) *iterations = it;
+    if (residual) *residual = res;
+    return rc;
+}
+
+extern "C" int pgm_compute_vjp(pgm_ctx *c, const double *pos, const double *mu, double e_bar, const double *f_bar,
+                               const double *mu_bar, double tol, double *q_bar, double *radius_bar, double *alpha_bar,
+                               double *rmin_half_bar, double *sqrt_eps_bar, double *cov_bar, int *iterations, double *residual) {
+    if (!c || !pos || !mu || !(tol > 0) || !std::isfinite(e_bar)) {
+        g_err = "pgm_compute_vjp: null argument, tol <= 0 or e_bar not finite";
+        return PGM_ERR_ARG;
+    }
+    PGM_TRY(single_system(c, "pgm_compute_vjp"));
+    if (c->ewald) {
+        g_err = "pgm_compute_vjp: PME reciprocal space only (this context holds the exact Ewald sum)";
+        return PGM_ERR_STATE;
+    }
+    return c->mixed ? compute_vjp_impl<float>(c, pos, mu, e_bar, f_bar, mu_bar, tol, q_bar, radius_bar, alpha_bar, rmin_half_bar,
+                                              sqrt_eps_bar, cov_bar, iterations, residual)
+                    : compute_vjp_impl<double>(c, pos, mu, e_bar, f_bar, mu_bar, tol, q_bar, radius_bar, alpha_bar, rmin_half_bar,
+                                               sqrt_eps_bar, cov_bar, iterations, residual);
 }
 
 // ---------------------------------------------------------------------------- learned guess (ABI)

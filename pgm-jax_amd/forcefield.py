@@ -311,6 +311,35 @@ class PGMForceField:
         out["iterations"], out["residual"] = it.value, res.value
         return out
 
+    def compute_vjp(self, pos, H, mu, e_bar: float = 0.0, f_bar=None, mu_bar=None, tol: float | None = None) -> dict:
+        """Parameter cotangents of compute's outputs (differentiable=True with a parameter pytree,
+        forcefield.py:1853-1915, 2489-2560), through pgm_compute_vjp: for a loss with dL/dE = e_bar,
+        dL/dF = f_bar (N,3), dL/dmu = mu_bar (N,3) at the dipoles mu solved at pos, returns
+        {"q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps"} (N,) and "cov" (n_cov,) device float64
+        tensors, plus "iterations" and "residual" of the adjoint CG (tol: settings.adjoint_tol by
+        default).  System.reduce_params sums them into the tied parameter table's gradient.
+        """
+        torch = _torch()
+        pos = _dev(pos, (self.n, 3))
+        mu = _dev(mu, (self.n, 3))
+        fb = None if f_bar is None else _dev(f_bar, (self.n, 3))
+        mb = None if mu_bar is None else _dev(mu_bar, (self.n, 3))
+        self._box(H)
+        self._ensure_rows(pos)
+        tol = float(self.s.adjoint_tol if tol is None else tol)
+        f64 = dict(dtype=torch.float64, device="cuda")
+        keys = ("q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps")
+        out = {k: torch.empty(self.n, **f64) for k in keys}
+        out["cov"] = torch.empty(max(len(self.sys.cov_i), 1), **f64)
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+        it, res = C.c_int(), C.c_double()
+        _lib.check(_lib.lib().pgm_compute_vjp(self._ctx, ptr(pos), ptr(mu), float(e_bar), ptr(fb), ptr(mb), tol,
+                                              *(ptr(out[k]) for k in keys + ("cov",)), C.byref(it), C.byref(res)),
+                   "pgm_compute_vjp")
+        out["cov"] = out["cov"][:len(self.sys.cov_i)]
+        out["iterations"], out["residual"] = it.value, res.value
+        return out
+
     def pme_energy(self, pos, d, with_charges: bool = True, H=None) -> tuple[float, object]:
         """Return (E_rec [e^2/nm], dE_rec/dd (N,3)) of PME (pgm_jax/md/pme.py:339-411)."""
         torch = _torch()
