@@ -113,6 +113,14 @@ long long pgm_row_entries(pgm_ctx *ctx);
  * the last evaluation; 0 with atom rows, -1 before the first. */
 long long pgm_row_blocks(pgm_ctx *ctx);
 
+/* New parameters for the next evaluations: the parameter pytree of PGMForceField.compute(...,
+ * params) (forcefield.py:2489-2560) expanded per atom (System.expand, pgm_jax/system.py:561-577).
+ * Host f64 arrays of one bead (n_atoms; cov_c: n_cov in the topology's order), NULL: unchanged.
+ * Radii must be > 0, polarizabilities >= 0, everything finite (PGM_ERR_ARG otherwise, nothing
+ * changed).  Synchronises. */
+int pgm_set_params(pgm_ctx *ctx, const double *q, const double *radius, const double *alpha, const double *lj_rmin_half,
+                   const double *lj_sqrt_eps, const double *cov_c);
+
 /* PGMForceField.perm_dipoles (forcefield.py:1032-1042): p (N,3) f64 device. */
 int pgm_perm_dipoles(pgm_ctx *ctx, const double *pos, double *p);
 

@@ -45,6 +45,7 @@ EXPORTS = (
     "pgm_strain_derivative",
     "pgm_dipole_vjp",
     "pgm_compute_vjp",
+    "pgm_set_params",
     "pgm_solve_linear_induction",
     "pgm_solve_linear_induction_noalloc",
     "pgm_md_setup",
@@ -200,6 +201,7 @@ def lib() -> C.CDLL:
     L.pgm_strain_derivative.argtypes = [P, P, P, P, I, C.POINTER(Energy), P, C.POINTER(D)]
     L.pgm_dipole_vjp.argtypes = [P, P, P, P, D, P, P, P, P, P, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_compute_vjp.argtypes = [P, P, P, D, P, P, D, P, P, P, P, P, P, C.POINTER(I), C.POINTER(D)]
+    L.pgm_set_params.argtypes = [P, P, P, P, P, P, P]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_solve_linear_induction_noalloc.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]

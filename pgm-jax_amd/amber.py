@@ -104,6 +104,9 @@ def read_prmtop_pgm(path: str, first_residue_only: bool = True) -> list:
     rh, se = _lj_per_type(s)
     z = s.get("ATOMIC_NUMBER")
     zel = {1: "H", 6: "C", 7: "N", 8: "O", 9: "F", 15: "P", 16: "S", 17: "Cl"}
+    # bonds (the symmetry classes of the parameter tying): atom pointers are 3 (i - 1) in the prmtop
+    bl = [int(v) for k in ("BONDS_INC_HYDROGEN", "BONDS_WITHOUT_HYDROGEN") for v in s.get(k, [])]
+    bond_pairs = [(bl[t] // 3, bl[t + 1] // 3) for t in range(0, len(bl) - 2, 3)]
     mols = []
     for r in range(1 if first_residue_only else len(res_lab)):
         a0, a1 = res_ptr[r], res_ptr[r + 1]
@@ -117,8 +120,10 @@ def read_prmtop_pgm(path: str, first_residue_only: bool = True) -> list:
             el = [zel.get(int(v), "X") for v in z[a0:a1]]
         else:
             el = [re.sub(r"\d+", "", n)[:1].upper() for n in names[a0:a1]]
+        types = [str(t) for t in s["AMBER_ATOM_TYPE"][a0:a1]] if "AMBER_ATOM_TYPE" in s else None
+        bonds = [(i - a0, j - a0) for i, j in bond_pairs if a0 <= i < a1 and a0 <= j < a1]
         mols.append(Molecule(res_lab[r], el, q[a0:a1].copy(), rad[a0:a1].copy(), alp[a0:a1].copy(), cov,
-                             rh[a0:a1].copy(), se[a0:a1].copy(), masses=mass[a0:a1].copy()))
+                             rh[a0:a1].copy(), se[a0:a1].copy(), masses=mass[a0:a1].copy(), types=types, bonds=bonds))
     return mols
 
 

@@ -131,6 +131,7 @@ struct pgm_ctx {
     DevBuf ljp;  // (rmin/2, sqrt eps, molecule, 0) per atom, double4 (the force pass)
     DevBuf cstart, ci, cj, cc, jstart, jlist;
     DevBuf corder;  // caller index of entry k of the owner-sorted covalent list
+    std::vector<int> h_corder;
     // geometry: the positions of the current evaluation (caller's device buffer) and xr
     const double *pos = nullptr;
     DevBuf xr;
@@ -1831,9 +1832,9 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if ((r = upload(c->cc, cc.data(), nc, st)) < 0) return fail(r);
     if ((r = upload(c->jstart, js.data(), n + 1, st)) < 0) return fail(r);
     if ((r = upload(c->jlist, jl.data(), nc, st)) < 0) return fail(r);
-    std::vector<int> corder(nc);
-    for (int k = 0; k < nc; ++k) corder[k] = order[k % nc1] + (k / nc1) * nc1;
-    if ((r = upload(c->corder, corder.data(), nc, st)) < 0) return fail(r);
+    c->h_corder.resize(nc);
+    for (int k = 0; k < nc; ++k) c->h_corder[k] = order[k % nc1] + (k / nc1) * nc1;
+    if ((r = upload(c->corder, c->h_corder.data(), nc, st)) < 0) return fail(r);
     // buffers
     const size_t r4 = 4 * c->rsize * (size_t)n, d3 = sizeof(double) * 3 * (size_t)n;
     DevBuf *v4[] = {&c->vx, &c->vr, &c->vz, &c->vp, &c->vAp, &c->vb, &c->vx0, &c->vrow, &c->vdq, &c->vtmp};
@@ -2451,6 +2452,89 @@ extern "C" int pgm_compute_vjp(pgm_ctx *c, const double *pos, const double *mu, 
                                               sqrt_eps_bar, cov_bar, iterations, residual)
                     : compute_vjp_impl<double>(c, pos, mu, e_bar, f_bar, mu_bar, tol, q_bar, radius_bar, alpha_bar, rmin_half_bar,
                                                sqrt_eps_bar, cov_bar, iterations, residual);
+}
+
+// ---------------------------------------------------------------------------- parameters
+// New per-atom parameters and covalent-dipole strengths (host arrays of one bead; NULL: unchanged):
+// the parameter pytree of PGMForceField.compute(..., params) (forcefield.py:2489-2560, System.expand
+// of pgm_jax/system.py:561-577).  Every quantity derived from them is refreshed: device copies, the
+// charge sums of the self and background terms, the LJ tail's moment sum, the packed LJ records, the
+// molecule-block preconditioner (rebuilt at the next evaluation).  Synchronises.
+extern "C" int pgm_set_params(pgm_ctx *c, const double *q, const double *radius, const double *alpha,
+                              const double *lj_rmin_half, const double *lj_sqrt_eps, const double *cov_c) {
+    if (!c) return PGM_ERR_ARG;
+    const int n = c->n, nper = c->nper;
+    for (int i = 0; i < nper; ++i) {
+        if ((q && !std::isfinite(q[i])) || (radius && !(radius[i] > 0 && std::isfinite(radius[i]))) ||
+            (alpha && !(alpha[i] >= 0 && std::isfinite(alpha[i]))) || (lj_rmin_half && !std::isfinite(lj_rmin_half[i])) ||
+            (lj_sqrt_eps && !std::isfinite(lj_sqrt_eps[i]))) {
+            g_err = "pgm_set_params: atom " + std::to_string(i) +
+                    ": parameters must be finite, radii > 0 and polarizabilities >= 0";
+            return PGM_ERR_ARG;
+        }
+    }
+    const int nc1 = c->nbead ? c->ncov / c->nbead : 0;
+    if (cov_c)
+        for (int m = 0; m < nc1; ++m)
+            if (!std::isfinite(cov_c[m])) {
+                g_err = "pgm_set_params: covalent dipole strengths must be finite";
+                return PGM_ERR_ARG;
+            }
+    hipStream_t st = c->stream;
+    HIP_TRY(hipStreamSynchronize(st));  // earlier uploads from the host copies have completed
+    if (q) {
+        c->sum_q = c->sum_q2 = 0.0;
+        for (int i = 0; i < n; ++i) c->h_q[i] = q[i % nper];
+        for (int i = 0; i < nper; ++i) {
+            c->sum_q += q[i];
+            c->sum_q2 += q[i] * q[i];
+        }
+        PGM_TRY(upload(c->q, c->h_q.data(), n, st));
+    }
+    if (radius) {
+        for (int i = 0; i < n; ++i) c->h_R2[i] = radius[i % nper] * radius[i % nper];
+        PGM_TRY(upload(c->R2, c->h_R2.data(), n, st));
+    }
+    std::vector<float> af;
+    if (alpha) {
+        c->alpha0 = false;
+        for (int i = 0; i < n; ++i) {
+            c->h_alpha[i] = alpha[i % nper];
+            if (c->h_alpha[i] == 0) c->alpha0 = true;
+        }
+        PGM_TRY(upload(c->alpha64, c->h_alpha.data(), n, st));
+        if (c->mixed) {
+            af.assign(c->h_alpha.begin(), c->h_alpha.end());
+            PGM_TRY(upload(c->alpha_r, af.data(), n, st));
+        } else {
+            PGM_TRY(upload(c->alpha_r, c->h_alpha.data(), n, st));
+        }
+        c->pinv_stale = true;
+    }
+    std::vector<double> lj;
+    if (lj_rmin_half || lj_sqrt_eps) {
+        lj.resize(4 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            if (lj_rmin_half) c->h_rh[i] = lj_rmin_half[i % nper];
+            if (lj_sqrt_eps) c->h_se[i] = lj_sqrt_eps[i % nper];
+            lj[4 * (size_t)i] = c->h_rh[i];
+            lj[4 * (size_t)i + 1] = c->h_se[i];
+            lj[4 * (size_t)i + 2] = (double)c->h_mol[i];
+            lj[4 * (size_t)i + 3] = 0.0;
+        }
+        c->lj_sb = lj_sum_b(c);
+        PGM_TRY(upload(c->rh, c->h_rh.data(), n, st));
+        PGM_TRY(upload(c->se, c->h_se.data(), n, st));
+        PGM_TRY(upload(c->ljp, lj.data(), lj.size(), st));
+    }
+    std::vector<double> cc;
+    if (cov_c && c->ncov > 0) {
+        cc.resize(c->ncov);
+        for (int k = 0; k < c->ncov; ++k) cc[k] = cov_c[c->h_corder[k] % nc1];
+        PGM_TRY(upload(c->cc, cc.data(), c->ncov, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- learned guess (ABI)

@@ -177,9 +177,13 @@ class PGMForceField:
 
     # ------------------------------------------------------------------ entry points
     def compute(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None) -> Result:
-        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560)."""
+        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560).
+
+        params: a parameter pytree {quantity: (K,) array} of the system's tied table (System.expand;
+        None: the parameters in use), set on the device before the evaluation (set_params).  Its
+        gradient for a loss of the outputs is grad_params."""
         if params is not None:
-            raise NotImplementedError("parameter pytrees are not part of the HIP path")
+            self.set_params(params)
         pos = _dev(pos, (self.n, 3))
         self._box(H)
         self._ensure_rows(pos)
@@ -201,7 +205,7 @@ class PGMForceField:
         (forcefield.py:2562-2626): the dipoles are solved from ind.mu (no predictor) into a new mu;
         the state's history, count and norm are those passed in."""
         if params is not None:
-            raise NotImplementedError("parameter pytrees are not part of the HIP path")
+            self.set_params(params)
         pos = _dev(pos, (self.n, 3))
         self._box(H)
         self._ensure_rows(pos)
@@ -310,6 +314,27 @@ class PGMForceField:
         out["cov"] = out["cov"][:len(self.sys.cov_i)]
         out["iterations"], out["residual"] = it.value, res.value
         return out
+
+    def set_params(self, params: dict) -> None:
+        """Use the parameter pytree `params` ({quantity: (K,) array} of the system's table, or per-atom
+        arrays for a system given as arrays; missing quantities keep their values) from now on:
+        pgm_set_params with the expanded per-atom arrays, skipped when they are the ones in use."""
+        arrays = self.sys.expand(params)
+        cur = getattr(self, "_params_now", None)
+        if cur is not None and all(np.array_equal(arrays[k], cur[k]) for k in arrays):
+            return
+        keep = [np.ascontiguousarray(arrays[k], float) for k in ("q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps", "cov")]
+        if keep[5].size == 0:
+            keep[5] = np.zeros(1)
+        _lib.check(_lib.lib().pgm_set_params(self._ctx, *(_ptr(a, C.c_double) for a in keep)), "pgm_set_params")
+        self._params_now = {k: np.array(v, float) for k, v in arrays.items()}
+
+    def grad_params(self, pos, H, mu, e_bar: float = 0.0, f_bar=None, mu_bar=None, tol: float | None = None) -> dict:
+        """The gradient over the system's parameter table of a loss with dL/dE = e_bar, dL/dF = f_bar,
+        dL/dmu = mu_bar at the dipoles mu solved at pos with the parameters in use (what jax.grad of
+        such a loss gives with differentiable=True; regression_cases.py:282-287): compute_vjp
+        summed per tying key (System.reduce_params).  {quantity: (K,) numpy array}."""
+        return self.sys.reduce_params(self.compute_vjp(pos, H, mu, e_bar, f_bar, mu_bar, tol))
 
     def compute_vjp(self, pos, H, mu, e_bar: float = 0.0, f_bar=None, mu_bar=None, tol: float | None = None) -> dict:
         """Parameter cotangents of compute's outputs (differentiable=True with a parameter pytree,

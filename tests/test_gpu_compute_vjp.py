@@ -129,3 +129,59 @@ def test_energy_cotangent_alone():
     for k in KEYS:
         g, rk = got[k].cpu().numpy(), np.asarray(ref[k])
         assert np.abs(g - rk).max() <= 1e-10 * np.abs(rk).max(), k
+
+
+def _toy_system(P):
+    """The toy box's System from its molecule templates (types and bonds of pgm_jax/models/toy.py:26-78),
+    in the order of systems.small_box: 4 methanols, then 30 waters."""
+    from test_param_table import toy_molecules
+
+    M, W = toy_molecules(P)
+    return P.System([M] * 4 + [W] * 30)
+
+
+def test_grad_params_through_the_tied_table():
+    """The product's own tying (System from templates) and grad_params give the golden dloss.*."""
+    import pgm_jax_amd as P
+
+    _, pos, H = systems.small_box(0)
+    sys_ = _toy_system(P)
+    ff = P.PGMForceField(sys_, H, P.MDSettings(precision="double", predictor="none", skin=0.05, **SMALL))
+    res = ff.compute(pos, H, params=sys_.initial_params())
+    grad = ff.grad_params(pos, H, res.induction.mu, 0.0, 2e-6 * res.forces, 2e2 * res.induction.mu, tol=1e-6)
+    with np.load(GOLD) as gold:
+        for qn in KEYS:
+            ref = gold["dloss." + qn]
+            err = np.abs(grad[qn] - ref).max() / np.abs(ref).max()
+            print(f"  grad_params {qn}: {err:.2e}", flush=True)
+            assert grad[qn].shape == ref.shape and err <= 1e-8, qn
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_params_pytree_in_compute(precision):
+    """compute(params=p) equals a context built with p's per-atom values; going back to the initial
+    table restores the first result exactly."""
+    import pgm_jax_amd as P
+
+    _, pos, H = systems.small_box(0)
+    sys_ = _toy_system(P)
+    st = P.MDSettings(precision=precision, predictor="none", skin=0.05, **SMALL)
+    ff = P.PGMForceField(sys_, H, st)
+    p0 = sys_.initial_params()
+    r0 = ff.compute(pos, H, params=p0)
+    p = {"q": p0["q"] * 1.01, "radius": p0["radius"] * 1.02, "alpha": p0["alpha"] * 0.97,
+         "lj_rmin_half": p0["lj_rmin_half"] * 1.01, "lj_sqrt_eps": p0["lj_sqrt_eps"] * 0.99, "cov": p0["cov"] * 1.05}
+    r1 = ff.compute(pos, H, params=p)
+    ex = sys_.expand(p)
+    fresh = P.PGMForceField(P.System(q=ex["q"], radius=ex["radius"], alpha=ex["alpha"], lj_rmin_half=ex["lj_rmin_half"],
+                                     lj_sqrt_eps=ex["lj_sqrt_eps"], masses=sys_.masses, mol=sys_.mol, cov_i=sys_.cov_i,
+                                     cov_j=sys_.cov_j, cov_c=ex["cov"]), H, st)
+    rf = fresh.compute(pos, H)
+    print(f"  {precision}: E {r0.energy['total']:.10f} -> {r1.energy['total']:.10f} (fresh {rf.energy['total']:.10f})", flush=True)
+    assert abs(r1.energy["total"] - r0.energy["total"]) > 1e-3 * abs(r0.energy["total"])
+    for k in ("elec", "vdw", "total"):
+        assert abs(r1.energy[k] - rf.energy[k]) <= 1e-10 * abs(rf.energy[k]), k
+    assert (r1.forces - rf.forces).abs().max().item() <= 1e-9 * rf.forces.abs().max().item()
+    assert r1.iterations == rf.iterations
+    r2 = ff.compute(pos, H, params=p0)
+    assert r2.energy == r0.energy and bool((r2.forces == r0.forces).all())
