@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpgm_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "context.hip")]
-HEADERS = ["common.h", "pairs.h", "pme.h", "fft.h", "cg.h", "md.h", "virial.h", "ewald.h", "dense.h", "mlp.h", "guess.h"]
+HEADERS = ["common.h", "pairs.h", "pme.h", "fft.h", "cg.h", "md.h", "virial.h", "ewald.h", "dense.h", "mlp.h", "guess.h", "adjoint.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -768,6 +768,12 @@ static int prepare(pgm_ctx *c, const double *pos) {
                                                                    c->pinv.as<double>());
         LAUNCH_CHECK();
     }
+    auto bin = [&]() -> int { return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos); };
+    // the PME binning (key, sort, tile bounds, splines) depends on the positions only: with molecule
+    // blocks it runs on the side stream beside the record pass (enqueued after it, so that the record
+    // pass starts while the host still enqueues the binning's dozen launches), joined before return
+    const bool fork_bin = c->layout == 1 && c->tiled && !c->ewald;
+    if (fork_bin) HIP_TRY(hipEventRecord(c->ev_fork, st));
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->rc_pair * c->rc_pair, rce2 = c->rc_e * c->rc_e;
@@ -794,8 +800,18 @@ static int prepare(pgm_ctx *c, const double *pos) {
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
             c->s.ewald_beta, c->rc_e * c->rc_e, c->jell.as<int>(), c->G1.as<double>(), c->G2.as<double>());
     LAUNCH_CHECK();
+    if (fork_bin) {
+        HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        c->stream = c->side;
+        const int rb = bin();
+        c->stream = st;
+        PGM_TRY(rb);
+        HIP_TRY(hipEventRecord(c->ev_join, c->side));
+        HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+        return PGM_OK;
+    }
     if (c->ewald) return PGM_OK;
-    return c->mixed ? pme_bin<float>(c, pos) : pme_bin<double>(c, pos);
+    return bin();
 }
 
 // Live timing: fold the recorded event pairs into the sums (waits for them), then free the slots.
@@ -1588,10 +1604,19 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), c->off.as<double>(),
             c->vir_pair.as<double>());
     } else {
-        k_pair_forces<real, false><<<gpair, 256, 0, st>>>(
+        // without the strain derivative the pair pass runs on the side stream beside the PME chain,
+        // joined before the force gather (which reads its row field)
+        hipStream_t ps = st;
+        if (!c->ewald) {
+            HIP_TRY(hipEventRecord(c->ev_fork, st));
+            HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            ps = c->side;
+        }
+        k_pair_forces<real, false><<<gpair, 256, 0, ps>>>(
             nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
             (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
             c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
+        if (!c->ewald) HIP_TRY(hipEventRecord(c->ev_join, c->side));
     }
     LAUNCH_CHECK();
     if (c->ewald) {
@@ -1608,6 +1633,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
             c_self(c), c->rec_dr.as<double>(), c->gp.as<double>(), vir ? c->rec_dd.as<double>() : nullptr);
     } else {
         PGM_TRY(spread_fft<real>(c, dq, true, true, nullptr, -1, vir ? c->vir_conv.as<double>() : nullptr));
+        if (!vir) HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));  // the pair pass's row field
         ORDER_SWITCH(c->s.order, (k_gather_forces<real, PO><<<nb, 256, 0, st>>>(
                                      n, nper, c->pos, c->pg, c->grid.as<real>(), dq, c->dd.as<double>(), c->rowf.as<double>(),
                                      c_self(c), c->rec_dr.as<double>(), c->gp.as<double>(),
