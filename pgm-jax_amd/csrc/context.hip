@@ -113,6 +113,8 @@ struct pgm_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;           // second stream: the pair rows run beside the PME chain
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_rec = nullptr;  // the molecule records of the last evaluation, when they ran on the side stream
+    bool rec_pending = false;     // ev_rec not yet waited for on the context stream
     hipEvent_t ev_read[4] = {nullptr, nullptr, nullptr, nullptr};
     bool mixed = false;
     size_t rsize = 8;  // sizeof(real)
@@ -745,7 +747,15 @@ static int compact_atom_rows(pgm_ctx *c) {
 
 // Pack positions and compact the rows to the cutoff (every force evaluation).  With molecule
 // blocks the records are the compacted form; the atom rows are compacted only when asked for.
-static int prepare(pgm_ctx *c, const double *pos) {
+// defer_records (the solve path): with molecule blocks the record pass runs on the side stream, where
+// the first field evaluation's row field follows it in stream order, while the context stream
+// goes on with the binning, the permanent dipoles and the PME chain; later consumers wait for ev_rec
+// (rec_pending).
+static int prepare(pgm_ctx *c, const double *pos, bool defer_records = false) {
+    if (c->rec_pending) {  // records of an earlier evaluation still pending on the side stream
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_rec, 0));
+        c->rec_pending = false;
+    }
     if (!c->rows_valid) PGM_TRY(build_rows(c, pos));
     // Captured CG iterations of the per-atom PME and Ewald paths hold the position pointer as a
     // kernel argument: positions at another address need new captures.
@@ -773,24 +783,32 @@ static int prepare(pgm_ctx *c, const double *pos) {
     // blocks it runs on the side stream beside the record pass (enqueued after it, so that the record
     // pass starts while the host still enqueues the binning's dozen launches), joined before return
     const bool fork_bin = c->layout == 1 && c->tiled && !c->ewald;
+    const bool side_records = fork_bin && defer_records;
     if (fork_bin) HIP_TRY(hipEventRecord(c->ev_fork, st));
+    if (side_records) HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->rc_pair * c->rc_pair, rce2 = c->rc_e * c->rc_e;
         // FP32: the same pass compacts the atom rows of the force pass (k_compact_rows not needed)
         const double beta2 = c->s.ewald_beta * c->s.ewald_beta;
+        hipStream_t rs = side_records ? c->side : st;
         if (c->mixed)
-            k_mol_tensors<float, true><<<nblk(nmol, 4), 256, 0, st>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
+            k_mol_tensors<float, true><<<nblk(nmol, 4), 256, 0, rs>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
                                                                c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2, rce2,
                                                                (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(),
                                                                c->mcount.as<int>(), c->rows.as<int>(), c->cap,
                                                                c->count.as<int>(), c->count_near.as<int>(), beta2);
         else
-            k_mol_tensors<double, false><<<nblk(nmol, 4), 256, 0, st>>>(
+            k_mol_tensors<double, false><<<nblk(nmol, 4), 256, 0, rs>>>(
                 nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full, c->mcap, c->rlm, c->xr.as<double4>(), c->box,
                 rc2, rce2, c->s.ewald_beta, c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(), nullptr, 0, nullptr, nullptr,
                 beta2);
         c->atom_rows_fresh = c->mixed;
+        if (side_records) {
+            LAUNCH_CHECK();
+            HIP_TRY(hipEventRecord(c->ev_rec, c->side));
+            c->rec_pending = true;
+        }
     } else if (c->mixed)
         k_pair_tensors<float><<<nblk((long long)n * RL, 256), 256, 0, st>>>(
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
@@ -800,6 +818,7 @@ static int prepare(pgm_ctx *c, const double *pos) {
             n, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), c->box,
             c->s.ewald_beta, c->rc_e * c->rc_e, c->jell.as<int>(), c->G1.as<double>(), c->G2.as<double>());
     LAUNCH_CHECK();
+    if (side_records) return bin();  // on the context stream, beside the records
     if (fork_bin) {
         HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         c->stream = c->side;
@@ -1571,6 +1590,10 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     const int nb = nblk(n, 256);
     const int npb = nblk(nper, 256);  // pack partials per bead
     R4 *dq = c->vdq.as<R4>();
+    if (c->rec_pending) {  // the force pass reads the atom rows the record pass wrote
+        HIP_TRY(hipStreamWaitEvent(st, c->ev_rec, 0));
+        c->rec_pending = false;
+    }
     k_energy_pack<real><<<bead_grid(c, nper), 256, 0, st>>>(nper, c->pperm.as<double>(), mu, c->q.as<double>(),
                                                             c->alpha64.as<double>(), c->dd.as<double>(), dq,
                                                             c->pack_part.as<double>());
@@ -1888,7 +1911,8 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         if ((r = b->ensure(sizeof(int) * n)) < 0) return fail(r);
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rec, hipEventDisableTiming) != hipSuccess) {
         g_err = "stream/event creation failed";
         return fail(PGM_ERR_HIP);
     }
@@ -1963,6 +1987,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_rec) (void)hipEventDestroy(c->ev_rec);
     for (int k = 0; k < 4; ++k)
         if (c->ev_read[k]) (void)hipEventDestroy(c->ev_read[k]);
     for (auto &t : c->timed)
@@ -2112,7 +2137,7 @@ extern "C" int pgm_pme_energy(pgm_ctx *c, const double *pos, int with_charges, c
 template <typename real>
 static int compute_api(pgm_ctx *c, const double *pos, pgm_induction *ind, pgm_energy *energy, double *forces, int *iterations,
                        double *residual, bool quiet = false) {
-    PGM_TRY(prepare(c, pos));
+    PGM_TRY(prepare(c, pos, true));
     k_perm_dipoles<<<nblk(c->n, 256), 256, 0, c->stream>>>(c->n, pos, c->box, c->cstart.as<int>(), c->cj.as<int>(),
                                                             c->cc.as<double>(), c->pperm.as<double>());
     LAUNCH_CHECK();
