@@ -645,10 +645,11 @@ __global__ __launch_bounds__(256) void k_mol_rows(int nmol, int nmol_bead, const
 // row and the others from the back, in (k, candidate) order per wave chunk; count / count_near
 // as k_compact_rows gives them.  (Used for FP32; the FP64 variant keeps its registers for the
 // tensors and leaves the atom rows to k_compact_rows.)
-// Four waves per SIMD (at most 128 VGPRs, no spills): 11 % faster than the three that the unbounded
-// register allocation gives (319 against 360 us at 32k waters); five spill.
+// FP32: four waves per SIMD (at most 128 VGPRs, no spills), 11 % faster than the three that the
+// unbounded register allocation gives (319 against 360 us at 32k waters); five spill, and so does
+// the FP64 variant at four.
 template <typename real, bool ROWS>
-__global__ __launch_bounds__(256, 4) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
+__global__ __launch_bounds__(256, sizeof(real) == 4 ? 4 : 1) void k_mol_tensors(int nmol, const int *__restrict__ mrows, const int *__restrict__ mcount_full,
                                                      int mcap_full, int mcap, int rlm, const double4 *__restrict__ xr, Box box,
                                                      double rc2, double rce2, real beta, int *__restrict__ mj, real *__restrict__ mg,
                                                      int *__restrict__ mcount, int *__restrict__ rows, int acap,
