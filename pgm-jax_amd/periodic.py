@@ -18,8 +18,9 @@ preconditioned CG (converged when max|alpha r| / mean|alpha b| <= dipole_tol) in
 jax.scipy's unpreconditioned CG; at the default 1e-12 both reach the solution to rounding.  The
 neighbour rows are rebuilt from the current positions (cutoff + skin) instead of being frozen at
 positions_ref, which gives the same result whenever the reference's frozen list is complete.
-Parameter pytrees are not taken (params must be None: build the model from a System with the
-parameters), and the GVDW / DE van der Waals forms are not provided (NotImplementedError).
+Parameter pytrees are taken as the reference takes them (the System's tied table, or per-atom
+arrays for a System given as arrays; None: the System's own values) and set on the device context
+(pgm_set_params); the GVDW / DE van der Waals forms are not provided (NotImplementedError).
 """
 
 from __future__ import annotations
@@ -35,12 +36,6 @@ from .system import System
 BAR_PER_KJMOL_NM3 = 16.605390671738466  # 1 kJ/mol/nm^3 in bar (pgm_jax/units.py:17)
 KE = 138.935458
 ELEC_LEVELS = {"q": (False, False), "qp": (True, False), "qi": (False, True), "qpi": (True, True)}
-
-
-def _no_params(params) -> None:
-    if params is not None:
-        raise NotImplementedError("parameter pytrees are not part of the HIP path: build the model from a System "
-                                  "carrying those parameters")
 
 
 def _variant(sys: System, perm: bool, lj: bool) -> System:
@@ -59,8 +54,9 @@ class _EwaldField:
         if elec not in ELEC_LEVELS:
             raise ValueError(f"unknown electrostatics level {elec!r} (expected q, qp, qi or qpi)")
         self.pd, self.ind = ELEC_LEVELS[elec]
-        self.sys = sys
+        self.sys, self.lj = sys, bool(lj)
         self.n = sys.n
+        self.custom = False  # parameters other than the System's own are set on the context
         s = MDSettings(cutoff=cutoff, skin=skin, ewald_beta=ewald_beta, pme_grid=(8, 8, 8), pme_order=4,
                        lj_lrc=bool(lj and lj_lrc), dipole_tol=dipole_tol, max_iter=2000, predictor="none", fused=False,
                        peek=0.0, precision="double", recip="ewald", k_tol=k_tol)
@@ -69,6 +65,21 @@ class _EwaldField:
 
     def box(self, H):
         return self.H if H is None else np.asarray(H, float)
+
+    def set_params(self, params) -> dict:
+        """Put the parameters of a pytree (None: the System's own) on the context (with this level's
+        covalent dipoles and LJ switched off where the context has none); return them per atom."""
+        ex = self.sys.expand(params)
+        if params is None and not self.custom:
+            return ex
+        arrays = dict(ex)
+        if not self.lj:
+            arrays["lj_sqrt_eps"] = np.zeros_like(ex["lj_sqrt_eps"])
+        if not self.pd:
+            arrays["cov"] = np.zeros(0)
+        self.ff.set_params(arrays)
+        self.custom = params is not None
+        return ex
 
     def mu(self, pos, H):
         """Induced dipoles at pos (zeros without induction)."""
@@ -101,7 +112,7 @@ class PeriodicPGM:
 
     def induced_dipoles(self, pos, params=None, H=None):
         """Return the induced dipoles mu* (N, 3) [e nm] (ewald.py:421-430), computed even without induction."""
-        _no_params(params)
+        self._f.set_params(params)
         pos = _dev(pos, (self.sys.n, 3))
         return self._f.ff.compute(pos, self._f.box(H)).induction.mu
 
@@ -120,7 +131,7 @@ class PeriodicPGM:
 
     def energy(self, pos, params=None, H=None) -> tuple[dict, dict]:
         """Return ({"perm", "ind", "total"} [kJ/mol], {"p": (N, 3)}) (ewald.py:432-464)."""
-        _no_params(params)
+        self._f.set_params(params)
         torch = _torch()
         pos = _dev(pos, (self.sys.n, 3))
         zero = torch.zeros((self.sys.n, 3), dtype=torch.float64, device="cuda")
@@ -130,7 +141,7 @@ class PeriodicPGM:
 
     def forces(self, pos, params=None, H=None):
         """Return -dE_total/dpos (N, 3) [kJ/mol/nm] at mu* (ewald.py:466-470)."""
-        _no_params(params)
+        self._f.set_params(params)
         pos = _dev(pos, (self.sys.n, 3))
         return self._f.energy_forces(pos, H, self._f.mu(pos, H))[1]
 
@@ -169,7 +180,7 @@ class PeriodicModel:
         return self._elec
 
     def _state(self, pos, params, H):
-        _no_params(params)
+        self._lj_now = self._full.set_params(params)
         pos = _dev(pos, (self.sys.n, 3))
         return pos, self._full.mu(pos, H)
 
@@ -190,12 +201,12 @@ class PeriodicModel:
     def _tail(self, H) -> float:
         if not self.lj_lrc:
             return 0.0
-        s = self.sys
+        ex = getattr(self, "_lj_now", None) or self.sys.expand(None)
         V = abs(np.linalg.det(self._full.box(H)))
         rc = self._args[1]
         # E_lrc = -2 pi / (3 V rc^3) sum_{i,j} 2 eps_ij rmin_ij^6 (pgm_jax/lj.py:87-114), with
         # eps_ij rmin_ij^6 = sqrt(e_i) sqrt(e_j) (r_i + r_j)^6 summed by the binomial moments of r
-        e, r = s.lj_sqrt_eps, s.lj_rmin_half
+        e, r = ex["lj_sqrt_eps"], ex["lj_rmin_half"]
         mom = [np.sum(e * r**k) for k in range(7)]
         total = 2.0 * sum(comb(6, k) * mom[k] * mom[6 - k] for k in range(7))
         return -2.0 * np.pi / (3.0 * V * rc**3) * total
