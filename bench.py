@@ -39,6 +39,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "ns/day + CG iters/step, 32k-water pGM qpi; learned-guess vs plain FP64 path"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s (AMD Instinct MI355X data sheet)
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (packed), the peak the pair kernels are held against
 
 
 def _package():
@@ -165,6 +166,11 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
       transform: (fused kernels, isolated timings) forward planes: the grid read and the half
                  spectrum (K^3/2 complex x 8 B) written; x transform: the spectrum read and written
                  and Gc read (K^3/2 x 4 B); inverse planes: the spectrum read and the grid written.
+
+    Also the FP32 view of the row field ("fp32"): 66 flops per unique pair per matvec (the kernel
+    evaluation, ~30, plus the matvec's 36; FMA = 2 flops), unique pairs = row entries / 2, against the
+    vector FP32 peak.  The kernel evaluation runs once per geometry in the record pass, so the row
+    field itself does the matvec's 36 ("matvec_part").
     """
     if layout == "molecule":
         row_bytes = blocks * 76 + n * 64
@@ -201,6 +207,13 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                    "algorithmic_bytes": gather_bytes, "ms_per_launch": round(gather_ms, 5)},
         "isolated_ms": {k: round(v, 5) for k, v in kern.items()},
     }
+    pairs = entries // 2
+    tf = 66 * pairs / (row_ms * 1e-3) / 1e12
+    tm = 36 * pairs / (row_ms * 1e-3) / 1e12
+    out["fp32"] = {"flop_per_unique_pair": 66, "unique_pairs": pairs, "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS,
+                   "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 5),
+                   "matvec_part": {"flop_per_unique_pair": 36, "achieved": round(tm, 3),
+                                   "frac": round(tm / FP32_PEAK_TFLOPS, 5)}}
     if floor_us is not None:  # the spread / gather bytes at the launch floor: their HBM fraction's ceiling
         out["launch_floor_us"] = round(floor_us, 3)
         out["spread"]["ceiling_frac_at_launch_floor"] = round(gbs(spread_bytes, floor_us * 1e-3) / HBM_PEAK_GBS, 4)

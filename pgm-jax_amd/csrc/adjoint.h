@@ -212,12 +212,9 @@ __global__ __launch_bounds__(256) void k_param_rows(int n, const int *__restrict
     const double4 a = xr[i];
     const double qi = q[i];
     const double4 lji = ljp[i];
-    dual di[3], vi[3];
+    dual di[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        di[c] = {d[3 * i + c], v ? ddot[3 * i + c] : 0.0};
-        vi[c] = {0.0, v ? v[3 * i + c] : 0.0};
-    }
+    for (int c = 0; c < 3; ++c) di[c] = {d[3 * i + c], v ? ddot[3 * i + c] : 0.0};
     const int cnt = count[i], cn = count_near[i];
     const int *row = rows + (size_t)i * cap;
     dual sq{0, 0}, sR{0, 0}, srh{0, 0}, sse{0, 0}, fx{0, 0}, fy{0, 0}, fz{0, 0};
