@@ -15,8 +15,10 @@ with the learned CG initial guess and the molecule-block preconditioner (--preco
 reference's); beside it: the same mixed path from the previous-step guess, the plain FP64 path
 (Jacobi, previous-step guess: the reference's algorithm; "fp64", the baseline of the metric), the energy /
 force / dipole errors of mixed vs FP64 at the final configuration, the HBM roofline of the
-dominant kernel (the CG matvec's row field; HIP events around every launch of the timed steps)
-with the PME spread / gather beside it, and the float64 C++/OpenMP restatement of the same algorithm on
+dominant kernel (the CG matvec's row field) with the PME spread / gather beside it, from HIP events
+attached to every launch of these kernels (hipExtLaunchKernel) in a second block of timed steps right
+after the measured one (--timing-steps; the events cost about a tenth of the step time, so the block
+that gives `value` runs without them), and the float64 C++/OpenMP restatement of the same algorithm on
 every host core (cpu_baseline, oracle/cpu_ref.cpp) from the same equilibrated state.
 
 Beads (--bead-evals > 0): 32 beads of 1,024 waters (3.94 nm box, PME 48^3, mixed, previous-step
@@ -149,8 +151,8 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                 lanes: int = 16, floor_us: float | None = None) -> dict:
     """HBM roofline of the dominant kernel (the CG matvec's row field) with the PME spread / gather beside it.
 
-    Launch durations are the averages of HIP events recorded around every CG-matvec launch of the
-    timed steps (row field on its side stream; spread + merge and gather on the main stream).
+    Launch durations are the averages of the HIP event pairs attached to every launch of the row
+    field, the spread kernel and the gather in the instrumented block of timed MD steps.
 
     Algorithmic bytes per launch (each logical read or write once):
       row field, molecule blocks (3-site molecules, the default for water): per record (molecule
@@ -159,9 +161,9 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
                  (16 B);
       row field, atom rows: per row entry j (4 B) + G1, G2 (FP32, 8 B); per atom the dipole read
                  (16 B) and the field written (16 B);
-      spread:    per atom positions 12 + q 4 + d 12 B, the FP32 grid written once (K^3 x 4 B) -- the
-                 tiled spread plus its block merge (with the fused transform the merge belongs to
-                 the forward planes and the spread entry is the tiled spread alone);
+      spread:    per atom positions 12 + q 4 + d 12 B, the FP32 grid written once (K^3 x 4 B), for the
+                 tiled spread kernel (its blocks are merged by the forward planes of the fused
+                 transform or by k_merge_table before the library transform);
       gather:    the FP32 grid read once (K^3 x 4 B), per atom d in 12 B and the field out 12 B;
       transform: (fused kernels, isolated timings) forward planes: the grid read and the half
                  spectrum (K^3/2 complex x 8 B) written; x transform: the spectrum read and written
@@ -198,7 +200,7 @@ def roofline_of(live: dict, kern: dict, entries: int, blocks: int, layout: str, 
         "launches": live["row_field"]["launches"],
         "row_entries": entries,
         "row_blocks": blocks,
-        "spread": {"kernels": "k_spread_tiles" if fused else "k_spread_tiles + k_merge_table",
+        "spread": {"kernels": "k_spread_tiles",
                    "achieved": round(gbs(spread_bytes, spread_ms), 2),
                    "frac": round(gbs(spread_bytes, spread_ms) / HBM_PEAK_GBS, 5), "algorithmic_bytes": spread_bytes,
                    "ms_per_launch": round(spread_ms, 5)},
@@ -331,6 +333,8 @@ def main() -> int:  # noqa: C901
     ap.add_argument("--guess", default="learned", choices=["learned", "mu4", "previous"],
                     help="CG initial guess of the measured mixed path")
     ap.add_argument("--train-steps", type=int, default=300, help="Adam steps per 100 equilibration steps (learned)")
+    ap.add_argument("--timing-steps", type=int, default=100,
+                    help="MD steps after the timed ones with HIP events on the row field, spread and gather (roofline)")
     ap.add_argument("--prev-steps", type=int, default=30, help="timed steps of the mixed path from the previous-step guess")
     ap.add_argument("--bead-evals", type=int, default=10, help="timed force evaluations of the 32-bead leg (0: off)")
     ap.add_argument("--graph-steps", type=int, default=50,
@@ -390,7 +394,7 @@ def main() -> int:  # noqa: C901
     pos = mde.pos.cpu().numpy()
     vel = mde.vel.cpu().numpy()
 
-    def run(precision: str, steps: int, warmup: int, live_timing: bool = False, predictor: str = "previous",
+    def run(precision: str, steps: int, warmup: int, timing_steps: int = 0, predictor: str = "previous",
             ff=None, md=None, precond: str = "jacobi"):
         if ff is None:
             ff = P.PGMForceField(sys_, H, P.MDSettings(precision=precision, predictor=predictor, precond=precond, **base))
@@ -400,23 +404,29 @@ def main() -> int:  # noqa: C901
         rep0 = md.run(warmup, a.dt_fs)
         torch.cuda.synchronize()
         log(precision, predictor, "timed", steps)
-        if live_timing:
-            ff.set_timing(True)
         t0 = time.perf_counter()
         rep = md.run(steps, a.dt_fs)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
-        if live_timing:
-            rep["kernel_timing"] = ff.kernel_timing()
-            ff.set_timing(False)
         log(precision, f"{ms:.3f} ms/step, {rep['cg_iterations'] / steps:.2f} CG it/step, T {rep['temperature']:.1f} K, "
             f"Etot {rep['etot']:.3f}, rebuilds {rep['row_rebuilds']}")
+        if timing_steps > 0:  # the same trajectory on, with the kernels' event pairs
+            ff.set_timing(True)
+            t0 = time.perf_counter()
+            rt = md.run(timing_steps, a.dt_fs)
+            torch.cuda.synchronize()
+            ms_t = (time.perf_counter() - t0) * 1e3 / timing_steps
+            rep["kernel_timing"] = ff.kernel_timing()
+            ff.set_timing(False)
+            rep["instrumented"] = {"steps": timing_steps, "ms_per_step": round(ms_t, 4),
+                                   "cg_iters_per_step": round(rt["cg_iterations"] / timing_steps, 3)}
+            log(precision, f"instrumented: {ms_t:.3f} ms/step, {rt['cg_iterations'] / timing_steps:.2f} CG it/step")
         rep["autotune"] = tune
         rep["etot_start"] = rep0["etot"]
         return ff, md, rep, ms
 
     # the measured path: mixed precision with the guess (the equilibrated context keeps the trained network)
-    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, live_timing=True, predictor=a.guess, ff=ffe, md=mde)
+    ffm, mdm, repm, ms_m = run("mixed", a.steps, a.warmup, max(1, a.timing_steps), predictor=a.guess, ff=ffe, md=mde)
     guess_status = ffm.guess_status() if a.guess == "learned" else None
     guess_label = a.guess
     if guess_status:
@@ -436,7 +446,7 @@ def main() -> int:  # noqa: C901
                   "cg_iters_per_step": round(repg["cg_iterations"] / a.graph_steps, 3)}
         log(f"mixed with CG hipGraphs: {ms_g:.3f} ms/step, {repg['cg_iterations'] / a.graph_steps:.2f} CG it/step")
     live = repm["kernel_timing"]
-    log("live kernel timing over the timed steps:", json.dumps(live))
+    log("live kernel timing over the instrumented steps:", json.dumps(live))
     kern = mdm.time_kernels(reps=50)
     entries = ffm.row_entries()
     layout, lanes = ffm.row_layout()
@@ -474,6 +484,7 @@ def main() -> int:  # noqa: C901
     n = 3 * a.waters
     K3 = a.grid**3
     roofline = roofline_of(live, kern, entries, blocks, layout, n, K3, ffm.pme_transform()[0] == "fused", lanes, floor_us)
+    roofline["events_block"] = repm["instrumented"]  # the MD steps the launch durations come from
     value = ns_per_day(ms_m, a.dt_fs)
     out = {
         "metric": METRIC,

@@ -185,17 +185,20 @@ int pgm_pme_transform(const pgm_ctx *ctx, int *fused, int *available);
 /* Where a CG field evaluation starts the row field on its side stream (no counterpart in the
  * reference, whose field is one XLA program): 0 before the PME spread, 1 after it, 2 after the
  * forward planes of the fused transform, 3 before the spread with the forward planes waiting for
- * the row field.  Only the overlap changes, not the results; the autotuner picks the fastest. */
+ * the row field; 4: no side stream (the row field after the spread on the context's stream, no
+ * fork or join events).  Only the overlap changes, not the results; the autotuner picks the
+ * fastest. */
 int pgm_set_row_fork(pgm_ctx *ctx, int stage);
 
 /* Autotuner (no counterpart in the reference): times, on this context's system at positions pos
  * (device), every applicable row-field layout (molecule blocks with 4, 8, 16 lanes; atom rows),
  * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that),
  * both reciprocal transforms and the point where a CG field evaluation starts the row field on its
- * side stream (before the spread, after it, after the forward planes of the fused transform),
+ * side stream (before the spread, after it, after the forward planes of the fused transform) or
+ * runs it on the context's stream,
  * `reps` launches each, and keeps the fastest.  report (host, 12 doubles): layout, lanes, spread
  * tile width per axis (3), row-field ms, spread + transform + gather ms, layouts tried, fused
- * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0..3,
+ * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0..4,
  * pgm_set_row_fork), whole field-evaluation ms.  Synchronises. */
 int pgm_autotune(pgm_ctx *ctx, const double *pos, int reps, double *report);
 

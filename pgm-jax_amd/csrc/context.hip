@@ -2,6 +2,7 @@
 // induced-dipole solve and the energy/force pass, dispatched to the float (mixed) or double
 // instantiation of the kernels.
 #include <hipcub/hipcub.hpp>
+#include <hip/hip_ext.h>
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
@@ -857,23 +858,33 @@ static int harvest_timing(pgm_ctx *c) {
 
 static constexpr int TIMED_CAP = 1024;
 
-// Record the start event of kernel slot `which` on `st`; returns the pair index, or -1 when off.
-static int timing_begin(pgm_ctx *c, int which, hipStream_t st) {
-    if (!c->timing) return -1;
+// The event pair of the next launch of kernel slot `which` ({nullptr, nullptr} when timing is off).
+// The timed kernels are launched with the pair attached (hipExtLaunchKernel: the kernel's own start
+// and end), so the timing adds no marker packets to the stream: a separate hipEventRecord before and
+// after each timed launch put a wait of several microseconds between the kernels of the CG chain.
+struct TimedPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+static TimedPair timing_pair(pgm_ctx *c, int which) {
+    if (!c->timing) return {};
     auto &t = c->timed[which];
     if (t.ev.empty()) {
         t.ev.resize(2 * TIMED_CAP);
         for (auto &e : t.ev)
-            if (hipEventCreate(&e) != hipSuccess) return -1;
+            if (hipEventCreate(&e) != hipSuccess) return {};
     }
-    if (t.used == TIMED_CAP && harvest_timing(c) < 0) return -1;
+    if (t.used == TIMED_CAP && harvest_timing(c) < 0) return {};
     const int k = t.used++;
-    if (hipEventRecord(t.ev[2 * k], st) != hipSuccess) return -1;
-    return k;
+    return {t.ev[2 * k], t.ev[2 * k + 1]};
 }
 
-static void timing_end(pgm_ctx *c, int which, int k, hipStream_t st) {
-    if (k >= 0) (void)hipEventRecord(c->timed[which].ev[2 * k + 1], st);
+// Launch kernel k, with the event pair attached when it is set.
+template <typename... KArgs, typename... Args>
+static void launch_timed(TimedPair tp, void (*k)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
+    if (tp.e0)
+        hipExtLaunchKernelGGL(k, grid, block, (std::uint32_t)lds, st, tp.e0, tp.e1, 0u, args...);
+    else
+        k<<<grid, block, lds, st>>>(args...);
 }
 
 // ---------------------------------------------------------------------------- PME dispatch
@@ -914,50 +925,49 @@ static int conv_partials(const pgm_ctx *c, bool vir) {
     return use_fused(c, vir) ? nblk((long long)c->pfg.K2 * c->pfg.K3h, PFFT_COLS) : nblk((long long)c->ncplx, 256);
 }
 
-// fork: called once the spread (fork_stage 1) or the forward planes (fork_stage 2) are enqueued.
+// fork: called once the spread (fork_stage 1 and 4) or the forward planes (fork_stage 2) are enqueued.
 template <typename real>
 static int spread_fft(pgm_ctx *c, const real4<real> *dq, bool withq, bool energy, const int *conv, int iter,
                       double *vir_part = nullptr, const std::function<int()> *fork = nullptr) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int P = c->s.order;
-    const int t_sp = energy ? -1 : timing_begin(c, 1, st);
+    const TimedPair tp = energy ? TimedPair{} : timing_pair(c, 1);  // the spread kernel
     const int B = c->nbead;
     if (c->tiled) {
         const PMETiles &t = c->tiles8;
         const size_t lds = spread_lds_bytes(c);
         if (withq) {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<B * t.ntiles, WAVE, lds, st>>>(
-                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
-                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+            ORDER_SWITCH(P, launch_timed(tp, k_spread_tiles<real, PO, true>, dim3(B * t.ntiles), dim3(WAVE), lds, st, t, c->pg,
+                                         c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                         c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter));
         } else {
-            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<B * t.ntiles, WAVE, lds, st>>>(
-                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
-                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+            ORDER_SWITCH(P, launch_timed(tp, k_spread_tiles<real, PO, false>, dim3(B * t.ntiles), dim3(WAVE), lds, st, t, c->pg,
+                                         c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                         c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter));
         }
         LAUNCH_CHECK();
         if (use_fused(c, vir_part != nullptr)) {  // merge + transforms + Gc fused (fft.h)
-            timing_end(c, 1, t_sp, st);
-            if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
+            if (fork && (c->fork_stage == 1 || c->fork_stage == 4)) PGM_TRY((*fork)());
             return fused_transform<real>(c, conv, iter, st, fork, energy ? c->conv_part.as<double>() : nullptr);
         }
-        if (fork && c->fork_stage == 1) PGM_TRY((*fork)());
+        if (fork && (c->fork_stage == 1 || c->fork_stage == 4)) PGM_TRY((*fork)());
         const int *K = c->s.grid;
         k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0] * B), 128, 0, st>>>(
             K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(), conv,
             iter);
     } else {
     k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
+    const dim3 gs(nblk((long long)n * P, 256));
     if (withq) {
-        ORDER_SWITCH(P, (k_spread<real, PO, true><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
-                            n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+        ORDER_SWITCH(P, launch_timed(tp, k_spread<real, PO, true>, gs, dim3(256), 0, st, n, c->pos, dq, c->pg, c->grid.as<real>(),
+                                     conv, iter));
     } else {
-        ORDER_SWITCH(P, (k_spread<real, PO, false><<<nblk((long long)n * PO, 256), 256, 0, st>>>(
-                            n, c->pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+        ORDER_SWITCH(P, launch_timed(tp, k_spread<real, PO, false>, gs, dim3(256), 0, st, n, c->pos, dq, c->pg,
+                                     c->grid.as<real>(), conv, iter));
     }
     }
     LAUNCH_CHECK();
-    timing_end(c, 1, t_sp, st);
     void *in[1] = {c->grid.p}, *out[1] = {c->cgrid.p};
     FFT_TRY(rocfft_execute(c->fwd, in, out, c->info));
     const int K3h = c->s.grid[2] / 2 + 1;
@@ -1007,20 +1017,21 @@ static dim3 bead_grid(const pgm_ctx *c, long long per_bead_threads) { return dim
 // PME gather + combine (tiled through LDS when the grid is tiled).
 template <typename real>
 static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const real4<real> *extra, real4<real> *out, double *part,
-                          const int *conv, int iter) {
+                          const int *conv, int iter, TimedPair tp = {}) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const real cself = (real)c_self(c);
     if (c->tiled) {
         const PMETiles &t = c->tiles;
-        ORDER_SWITCH(c->s.order, (k_gather_tiles<real, PO><<<c->nbead * t.ntiles, GATHER_THREADS, c->rsize * t.bsize, st>>>(
-                                     t, c->pg, c->tstart16.as<int>(), c->tend16.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
-                                     c->loc16.as<int>(), c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
-                                     extra, cself, mode, out, part, conv, iter)));
+        ORDER_SWITCH(c->s.order, launch_timed(tp, k_gather_tiles<real, PO>, dim3(c->nbead * t.ntiles), dim3(GATHER_THREADS),
+                                              c->rsize * t.bsize, st, t, c->pg, c->tstart16.as<int>(), c->tend16.as<int>(),
+                                              c->psidx.as<int>(), c->spl.as<real>(), c->loc16.as<int>(), c->grid.as<real>(), dq,
+                                              c->vrow.as<real4<real>>(), c->alpha_r.as<real>(), extra, cself, mode, out, part,
+                                              conv, iter));
     } else {
-        ORDER_SWITCH(c->s.order, (k_gather_combine<real, PO><<<nblk(n, 256), 256, 0, st>>>(
-                                     n, c->pos, c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
-                                     extra, cself, mode, out, part, conv, iter)));
+        ORDER_SWITCH(c->s.order, launch_timed(tp, k_gather_combine<real, PO>, dim3(nblk(n, 256)), dim3(256), 0, st, n, c->pos,
+                                              c->pg, c->grid.as<real>(), dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                                              extra, cself, mode, out, part, conv, iter));
     }
     LAUNCH_CHECK();
     return PGM_OK;
@@ -1028,22 +1039,24 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
 
 // The CG row field of dq into vrow on stream st, in the context's layout.
 template <typename real, bool WITHQ>
-static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, int iter, hipStream_t st) {
+static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, int iter, hipStream_t st, TimedPair tp = {}) {
     using R4 = real4<real>;
     if (c->layout == 1) {
         const int nmol = c->n / 3, nb = c->nper / 3;
         const int *mj = c->mj.as<int>(), *mc = c->mcount.as<int>();
         const real *mg = c->mg.as<real>();
+        const double4 *xr = c->xr.as<double4>();
         R4 *out = c->vrow.as<R4>();
+        const dim3 b(256);
         switch (c->rlm) {
-            case 4: k_row_field_mol<real, WITHQ, 4><<<nblk((long long)nmol * 4, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
-            case 8: k_row_field_mol<real, WITHQ, 8><<<nblk((long long)nmol * 8, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
-            default: k_row_field_mol<real, WITHQ, 16><<<nblk((long long)nmol * 16, 256), 256, 0, st>>>(nmol, nb, mj, mg, mc, c->xr.as<double4>(), dq, c->box, out, conv, iter); break;
+            case 4: launch_timed(tp, k_row_field_mol<real, WITHQ, 4>, dim3(nblk((long long)nmol * 4, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
+            case 8: launch_timed(tp, k_row_field_mol<real, WITHQ, 8>, dim3(nblk((long long)nmol * 8, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
+            default: launch_timed(tp, k_row_field_mol<real, WITHQ, 16>, dim3(nblk((long long)nmol * 16, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
         }
     } else {
-        k_row_field<real, WITHQ><<<nblk((long long)c->n * RL, 256), 256, 0, st>>>(
-            c->n, c->nper, c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq,
-            c->box, c->vrow.as<R4>(), conv, iter);
+        launch_timed(tp, k_row_field<real, WITHQ>, dim3(nblk((long long)c->n * RL, 256)), dim3(256), 0, st, c->n, c->nper,
+                     c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
+                     c->vrow.as<R4>(), conv, iter);
     }
     LAUNCH_CHECK();
     return PGM_OK;
@@ -1057,47 +1070,48 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     const int *conv = iter >= 0 ? &c->st.as<CGState>()->conv_it : nullptr;
     // fork: the row field on the side stream, concurrently with spread -> FFT -> convolve -> FFT
     // (from the start, or after the spread / the forward planes: fork_stage)
+    // stage 4: no side stream; the row field runs on st after the spread, and no events are recorded
+    const bool serial = c->fork_stage == 4 && !c->ewald;
     const std::function<int()> fork = [&]() -> int {
+        if (serial) {
+            if (withq) return row_field_launch<real, true>(c, dq, conv, iter, st, timing_pair(c, 0));
+            return row_field_launch<real, false>(c, dq, conv, iter, st, timing_pair(c, 0));
+        }
         HIP_TRY(hipEventRecord(c->ev_fork, st));
         HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        const int t_row = timing_begin(c, 0, c->side);
+        const TimedPair tp = timing_pair(c, 0);
         if (withq)
-            PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side)));
+            PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side, tp)));
         else
-            PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side)));
-        timing_end(c, 0, t_row, c->side);
+            PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side, tp)));
         HIP_TRY(hipEventRecord(c->ev_join, c->side));
         return PGM_OK;
     };
     // stage 3: forked at the start, joined before the forward planes (the row field beside the spread)
     const bool fused = c->tiled && !c->ewald && c->pme_fused && c->fused_ok;
-    const bool late = c->tiled && !c->ewald && (c->fork_stage == 1 || (c->fork_stage == 2 && fused));
+    const bool late = c->tiled && !c->ewald && (c->fork_stage == 1 || c->fork_stage == 4 || (c->fork_stage == 2 && fused));
     if (!late) PGM_TRY(fork());
     const real cself = (real)(4.0 * std::pow(c->s.ewald_beta, 3) / (3.0 * SQRT_PI));
     if (c->ewald) {
-        const int t_sp = timing_begin(c, 1, st);
+        const dim3 gk(nblk(c->nk, EW_TILE)), bk(EW_TILE);
+        const TimedPair tsk = timing_pair(c, 1);
         if (withq)
-            k_ewald_sk<real, true><<<nblk(c->nk, EW_TILE), EW_TILE, 0, st>>>(c->nk, c->kvec.as<double4>(), n, c->pos, dq,
-                                                                             c->sk.as<double2>(), nullptr, conv, iter);
+            launch_timed(tsk, k_ewald_sk<real, true>, gk, bk, 0, st, c->nk, c->kvec.as<double4>(), n, c->pos, dq,
+                         c->sk.as<double2>(), (double *)nullptr, conv, iter);
         else
-            k_ewald_sk<real, false><<<nblk(c->nk, EW_TILE), EW_TILE, 0, st>>>(c->nk, c->kvec.as<double4>(), n, c->pos, dq,
-                                                                              c->sk.as<double2>(), nullptr, conv, iter);
+            launch_timed(tsk, k_ewald_sk<real, false>, gk, bk, 0, st, c->nk, c->kvec.as<double4>(), n, c->pos, dq,
+                         c->sk.as<double2>(), (double *)nullptr, conv, iter);
         LAUNCH_CHECK();
-        timing_end(c, 1, t_sp, st);
         HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
-        const int t_ga = timing_begin(c, 2, st);
-        k_ewald_gather_combine<real><<<nblk(n, EW_TILE), EW_TILE, 0, st>>>(
-            n, c->nk, c->kvec.as<double4>(), c->sk.as<double2>(), c->pos, dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
-            extra, cself, mode, out, part, conv, iter);
+        launch_timed(timing_pair(c, 2), k_ewald_gather_combine<real>, dim3(nblk(n, EW_TILE)), bk, 0, st, n, c->nk,
+                     c->kvec.as<double4>(), c->sk.as<double2>(), c->pos, dq, c->vrow.as<real4<real>>(), c->alpha_r.as<real>(),
+                     extra, cself, mode, out, part, conv, iter);
         LAUNCH_CHECK();
-        timing_end(c, 2, t_ga, st);
         return PGM_OK;
     }
     PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter, nullptr, late || (fused && c->fork_stage == 3) ? &fork : nullptr));
-    HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
-    const int t_ga = timing_begin(c, 2, st);
-    PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter));
-    timing_end(c, 2, t_ga, st);
+    if (!serial) HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+    PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter, timing_pair(c, 2)));
     return PGM_OK;
 }
 
@@ -3367,8 +3381,9 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
     if (c->tiled && !c->ewald) {
         PGM_TRY(prepare(c, pos));
         PGM_TRY(dipoles());
-        const int stages = c->pme_fused && c->fused_ok ? 4 : 2;
-        for (int fs = 0; fs < stages; ++fs) {
+        const bool fz = c->pme_fused && c->fused_ok;
+        for (int fs = 0; fs < 5; ++fs) {
+            if (!fz && (fs == 2 || fs == 3)) continue;
             c->fork_stage = fs;
             double ms = 0;
             PGM_TRY(time_ms([&] {
@@ -3406,13 +3421,13 @@ extern "C" int pgm_set_pme_transform(pgm_ctx *c, int fused) {
     }
     drop_graphs(c);
     c->pme_fused = fused != 0;
-    if (!c->pme_fused && c->fork_stage >= 2) c->fork_stage = c->fork_stage == 2 ? 1 : 0;
+    if (!c->pme_fused && (c->fork_stage == 2 || c->fork_stage == 3)) c->fork_stage = c->fork_stage == 2 ? 1 : 0;
     return PGM_OK;
 }
 
 extern "C" int pgm_set_row_fork(pgm_ctx *c, int stage) {
-    if (!c || stage < 0 || stage > 3) return PGM_ERR_ARG;
-    if (stage >= 2 && !(c->pme_fused && c->fused_ok && c->tiled)) {
+    if (!c || stage < 0 || stage > 4) return PGM_ERR_ARG;
+    if ((stage == 2 || stage == 3) && !(c->pme_fused && c->fused_ok && c->tiled)) {
         g_err = "pgm_set_row_fork: stages 2 and 3 need the fused transform";
         return PGM_ERR_ARG;
     }

@@ -98,7 +98,7 @@ def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nb
     return ctx, stream
 
 
-FORKS = ("start", "after_spread", "after_forward_planes", "start_join_before_planes")
+FORKS = ("start", "after_spread", "after_forward_planes", "start_join_before_planes", "none")
 
 
 def _autotune(ctx, pos, reps: int) -> dict:
@@ -479,7 +479,8 @@ class PGMForceField:
 
     def set_row_fork(self, stage: str = "start") -> None:
         """Start the CG row field beside the PME chain at "start", "after_spread", "after_forward_planes" or
-        "start_join_before_planes" (the forward planes wait for it)."""
+        "start_join_before_planes" (the forward planes wait for it), or run it after the spread on the
+        context's own stream ("none": no side stream, no fork and join events)."""
         code = FORKS.index(stage)
         _lib.check(_lib.lib().pgm_set_row_fork(self._ctx, code), "pgm_set_row_fork")
 

@@ -98,7 +98,8 @@ def test_fused_transform_on_beads():
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_row_fork_point_changes_nothing(precision):
     """The row field's start on the side stream (before / after the spread, after the forward planes,
-    joined before them) only moves the overlap: fields, iterations and energies are bitwise the same."""
+    joined before them, or no side stream) only moves the overlap: fields, iterations and energies are
+    bitwise the same."""
     import pgm_jax_amd as P
 
     top, pos, H = systems.water_box(512, 2.483, seed=9)
@@ -106,7 +107,7 @@ def test_row_fork_point_changes_nothing(precision):
                                                        pme_grid=(32, 32, 32), pme_order=6, dipole_tol=1e-7))
     d = np.random.default_rng(5).normal(scale=0.01, size=pos.shape)
     out = []
-    for stage in ("start", "after_spread", "after_forward_planes", "start_join_before_planes"):
+    for stage in ("start", "after_spread", "after_forward_planes", "start_join_before_planes", "none"):
         ff.set_row_fork(stage)
         r = ff.compute(pos, H)
         out.append((ff.field(pos, d).cpu().numpy(), r.iterations, r.energy["total"]))
@@ -115,3 +116,6 @@ def test_row_fork_point_changes_nothing(precision):
     ff.set_pme_transform("library")
     with pytest.raises(P.PGMError):
         ff.set_row_fork("after_forward_planes")
+    ff.set_row_fork("none")  # the library transform without a side stream: the same field again
+    assert np.allclose(ff.field(pos, d).cpu().numpy(), out[0][0], rtol=1e-5 if precision == "mixed" else 1e-12,
+                       atol=1e-6 * np.abs(out[0][0]).max())
