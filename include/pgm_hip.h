@@ -195,7 +195,8 @@ int pgm_set_row_fork(pgm_ctx *ctx, int stage);
  * every valid PME spread-tile width (8 down to order - 1 points per axis; gather tiles twice that),
  * both reciprocal transforms and the point where a CG field evaluation starts the row field on its
  * side stream (before the spread, after it, after the forward planes of the fused transform) or
- * runs it on the context's stream,
+ * runs it on the context's stream (jointly with the row layouts within 10 % of the fastest row
+ * field, by the time of a whole field evaluation),
  * `reps` launches each, and keeps the fastest.  report (host, 12 doubles): layout, lanes, spread
  * tile width per axis (3), row-field ms, spread + transform + gather ms, layouts tried, fused
  * transform chosen (1) or not (0), the other transform's ms (0 if not timed), fork point (0..4,

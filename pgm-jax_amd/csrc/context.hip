@@ -790,21 +790,29 @@ static int prepare(pgm_ctx *c, const double *pos, bool defer_records = false) {
     if (c->layout == 1) {
         const int nmol = n / 3;
         const double rc2 = c->rc_pair * c->rc_pair, rce2 = c->rc_e * c->rc_e;
-        // FP32: the same pass compacts the atom rows of the force pass (k_compact_rows not needed)
+        // The force pass runs on these records unless LJ reaches beyond the electrostatic cutoff; then
+        // (FP32) the same pass compacts the atom rows of the force pass (k_compact_rows not needed).
+        // Other users of the atom rows (strain derivative, parameter gradients) compact them when asked.
         const double beta2 = c->s.ewald_beta * c->s.ewald_beta;
         hipStream_t rs = side_records ? c->side : st;
-        if (c->mixed)
+        const bool rows_too = c->mixed && c->rc_v > c->rc_e;
+        if (rows_too)
             k_mol_tensors<float, true><<<nblk(nmol, 4), 256, 0, rs>>>(nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full,
                                                                c->mcap, c->rlm, c->xr.as<double4>(), c->box, rc2, rce2,
                                                                (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(),
                                                                c->mcount.as<int>(), c->rows.as<int>(), c->cap,
                                                                c->count.as<int>(), c->count_near.as<int>(), beta2);
+        else if (c->mixed)
+            k_mol_tensors<float, false><<<nblk(nmol, 4), 256, 0, rs>>>(
+                nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full, c->mcap, c->rlm, c->xr.as<double4>(), c->box,
+                rc2, rce2, (float)c->s.ewald_beta, c->mj.as<int>(), c->mg.as<float>(), c->mcount.as<int>(), nullptr, 0, nullptr,
+                nullptr, beta2);
         else
             k_mol_tensors<double, false><<<nblk(nmol, 4), 256, 0, rs>>>(
                 nmol, c->mrows.as<int>(), c->mcount_full.as<int>(), c->mcap_full, c->mcap, c->rlm, c->xr.as<double4>(), c->box,
                 rc2, rce2, c->s.ewald_beta, c->mj.as<int>(), c->mg.as<double>(), c->mcount.as<int>(), nullptr, 0, nullptr, nullptr,
                 beta2);
-        c->atom_rows_fresh = c->mixed;
+        c->atom_rows_fresh = rows_too;
         if (side_records) {
             LAUNCH_CHECK();
             HIP_TRY(hipEventRecord(c->ev_rec, c->side));
@@ -1074,6 +1082,10 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
     const bool serial = c->fork_stage == 4 && !c->ewald;
     const std::function<int()> fork = [&]() -> int {
         if (serial) {
+            if (c->rec_pending) {  // the records of this geometry come from the side stream
+                HIP_TRY(hipStreamWaitEvent(st, c->ev_rec, 0));
+                c->rec_pending = false;
+            }
             if (withq) return row_field_launch<real, true>(c, dq, conv, iter, st, timing_pair(c, 0));
             return row_field_launch<real, false>(c, dq, conv, iter, st, timing_pair(c, 0));
         }
@@ -1604,7 +1616,7 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     const int nb = nblk(n, 256);
     const int npb = nblk(nper, 256);  // pack partials per bead
     R4 *dq = c->vdq.as<R4>();
-    if (c->rec_pending) {  // the force pass reads the atom rows the record pass wrote
+    if (c->rec_pending) {  // the force pass reads the records (or atom rows) the record pass wrote
         HIP_TRY(hipStreamWaitEvent(st, c->ev_rec, 0));
         c->rec_pending = false;
     }
@@ -1612,10 +1624,14 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                                             c->alpha64.as<double>(), c->dd.as<double>(), dq,
                                                             c->pack_part.as<double>());
     LAUNCH_CHECK();
-    if (!c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));  // the force pass runs on the atom rows
-    const int npair = nblk((long long)nper * rowl<real>(), 256);  // pair partials per bead
-    const dim3 gpair(npair, B);
     const bool vir = strain != nullptr;
+    // the force pass on the molecule-block records (no strain derivative; every LJ pair in a record)
+    // or on the atom rows
+    const bool mol_forces = c->layout == 1 && !vir && c->rc_v <= c->rc_e;
+    if (!mol_forces && !c->atom_rows_fresh) PGM_TRY(compact_atom_rows(c));
+    const int npair = mol_forces ? nblk((long long)nper * MOL_FORCE_LANES, 256)
+                                 : nblk((long long)nper * rowl<real>(), 256);  // pair partials per bead
+    const dim3 gpair(npair, B);
     const int nconv = conv_partials(c, vir);
     if (vir) {
         PGM_TRY(c->com.ensure(sizeof(double) * 3 * (size_t)c->nmol_all));
@@ -1649,10 +1665,16 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
             HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
             ps = c->side;
         }
-        k_pair_forces<real, false><<<gpair, 256, 0, ps>>>(
-            nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
-            (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
-            c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
+        if (mol_forces)
+            k_pair_forces_mol<real, MOL_FORCE_LANES><<<gpair, 256, 0, ps>>>(
+                nper, n / 3, c->rlm, c->mj.as<int>(), c->mg.as<real>(), c->mcount.as<int>(), c->xr.as<double4>(), dq, c->box,
+                (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->rh.as<double>(), c->se.as<double>(),
+                c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>());
+        else
+            k_pair_forces<real, false><<<gpair, 256, 0, ps>>>(
+                nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
+                (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
+                c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
         if (!c->ewald) HIP_TRY(hipEventRecord(c->ev_join, c->side));
     }
     LAUNCH_CHECK();
@@ -3314,25 +3336,26 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
     lays.push_back({0, RL});
     double best_row = 1e300;
     Lay best_lay = lays.front();
-    for (const Lay &L : lays) {
+    std::vector<double> row_ms;
+    auto use_layout = [&](const Lay &L) -> int {
         c->layout = L.layout;
         if (L.layout == 1) {
             c->rlm = L.lanes;
             PGM_TRY(alloc_mol(c, std::max(c->mcap_full, 64)));
         }
         c->rows_valid = false;
+        return PGM_OK;
+    };
+    for (const Lay &L : lays) {
+        PGM_TRY(use_layout(L));
         PGM_TRY(prepare(c, pos));
         PGM_TRY(dipoles());
         double ms = 0;
         PGM_TRY(time_ms([&] { (void)row_field_launch<real, false>(c, c->vp.as<R4>(), nullptr, -1, st); }, ms));
+        row_ms.push_back(ms);
         if (ms < best_row) best_row = ms, best_lay = L;
     }
-    c->layout = best_lay.layout;
-    if (best_lay.layout == 1) {
-        c->rlm = best_lay.lanes;
-        PGM_TRY(alloc_mol(c, std::max(c->mcap_full, 64)));
-    }
-    c->rows_valid = false;
+    PGM_TRY(use_layout(best_lay));
     // spread-tile widths
     double best_pme = 1e300;
     int best_cap = c->tile_cap;
@@ -3375,24 +3398,34 @@ template <typename real> static int autotune_impl(pgm_ctx *c, const double *pos,
         other_pme = ms[c->pme_fused ? 0 : 1];
         drop_graphs(c);
     }
-    // fork point of the row field in a whole CG field evaluation (row field beside the PME chain)
+    // fork point of the row field in a whole CG field evaluation (row field beside the PME chain),
+    // jointly with the row layout among those within 10 % of the fastest row field alone: the
+    // layouts load the GPU differently beside the PME chain
     double best_fork = 1e300;
     int fork_best = 0;
     if (c->tiled && !c->ewald) {
-        PGM_TRY(prepare(c, pos));
-        PGM_TRY(dipoles());
         const bool fz = c->pme_fused && c->fused_ok;
-        for (int fs = 0; fs < 5; ++fs) {
-            if (!fz && (fs == 2 || fs == 3)) continue;
-            c->fork_stage = fs;
-            double ms = 0;
-            PGM_TRY(time_ms([&] {
-                (void)field_eval<real>(c, c->vp.as<R4>(), false, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), -1);
-            }, ms));
-            if (ms < best_fork) best_fork = ms, fork_best = fs;
+        Lay fork_lay = best_lay;
+        for (size_t li = 0; li < lays.size(); ++li) {
+            if (row_ms[li] > 1.1 * best_row) continue;
+            PGM_TRY(use_layout(lays[li]));
+            PGM_TRY(prepare(c, pos));
+            PGM_TRY(dipoles());
+            for (int fs = 0; fs < 5; ++fs) {
+                if (!fz && (fs == 2 || fs == 3)) continue;
+                c->fork_stage = fs;
+                double ms = 0;
+                PGM_TRY(time_ms([&] {
+                    (void)field_eval<real>(c, c->vp.as<R4>(), false, MODE_AV, nullptr, c->vAp.as<R4>(), c->partA.as<double>(), -1);
+                }, ms));
+                if (ms < best_fork) best_fork = ms, fork_best = fs, fork_lay = lays[li];
+            }
+            drop_graphs(c);
         }
+        for (size_t li = 0; li < lays.size(); ++li)
+            if (lays[li].layout == fork_lay.layout && lays[li].lanes == fork_lay.lanes) best_row = row_ms[li];
+        PGM_TRY(use_layout(fork_lay));
         c->fork_stage = fork_best;
-        drop_graphs(c);
     }
     c->rows_valid = false;
     PGM_TRY(prepare(c, pos));

@@ -885,4 +885,198 @@ __global__ __launch_bounds__(256) void k_row_field_mol(int nmol, int nmol_bead, 
         }
 }
 
+// ---------------------------------------------------------------------------- molecule-blocked force pass
+// The force pass of k_pair_forces (the same per-atom sums g_el, g_lj, rowf and per-block energy
+// partials, each pair again counted from both sides) on the molecule-block records, for 3-site
+// molecules when the van der Waals cutoff does not exceed the electrostatic one (every LJ pair then
+// lies in a record) and no strain derivative is asked for.  FL lanes per atom i = 3I + p walk the
+// records of its molecule I and take the three pairs (p, k) of each: J's positions, dipoles, charges
+// and LJ parameters are the same loads for the three atoms of I (one cache line for the lanes of
+// a wave), and the record's G1, G2 are reused.  On the closed-form branch ((a r)^2, (b r)^2 >= 1)
+// the other two radial functions follow from them and the two Gaussians c_a = 2a/sqrt(pi)
+// exp(-a^2 r^2), c_b likewise: G0 = r^2 G1 + (c_a - c_b) and G3 = (5 G2 - ((2a^2)^2 c_a -
+// (2b^2)^2 c_b)) / r^2, the recursion B_n = ((2n-1) B_(n-1) - (2a^2)^(n-1) c) / r^2 of the closed
+// form read down and up (no erfc).  Pairs off it (a few per atom: its own molecule, hydrogen bonds)
+// are queued per lane in LDS and take pair_G (series) after the record loop, one per lane per round,
+// so that the wave runs the series branch a few times per atom and not in most record rounds.  The
+// terms of a record are summed in `real`, then added to FP64 sums.  Grid (blocks per bead, beads).
+#define MOL_FORCE_TERMS(G0_, G1_, G2_, G3_)                                                            \
+    {                                                                                                  \
+        const real dix = di.x * x + di.y * y + di.z * z;                                               \
+        const real dkx = dk.x * x + dk.y * y + dk.z * z;                                               \
+        const real didk = di.x * dk.x + di.y * dk.y + di.z * dk.z;                                     \
+        const real qk = dk.w;                                                                          \
+        const real t = qi * dkx - qk * dix;                                                            \
+        pe += qi * qk * (G0_) + t * (G1_) - (G2_) * dix * dkx + (G1_) * didk;                          \
+        const real radial = -qi * qk * (G1_) - t * (G2_) + (G3_) * dix * dkx - (G2_) * didk;           \
+        const real qiG1 = qi * (G1_), qkG1 = qk * (G1_);                                               \
+        pg0 += radial * x + qiG1 * dk.x - qkG1 * di.x - (G2_) * (di.x * dkx + dk.x * dix);             \
+        pg1 += radial * y + qiG1 * dk.y - qkG1 * di.y - (G2_) * (di.y * dkx + dk.y * dix);             \
+        pg2 += radial * z + qiG1 * dk.z - qkG1 * di.z - (G2_) * (di.z * dkx + dk.z * dix);             \
+        const real cfield = -qk * (G1_) - (G2_) * dkx;                                                 \
+        pf0 += cfield * x + (G1_) * dk.x;                                                              \
+        pf1 += cfield * y + (G1_) * dk.y;                                                              \
+        pf2 += cfield * z + (G1_) * dk.z;                                                              \
+    }
+
+constexpr int MOL_FORCE_LANES = 8;
+constexpr int MOL_NEAR_QUEUE = 8;  // queued series pairs per lane (more are evaluated at once)
+// Launched with 256 threads per block (the LDS queue has 256 columns).  FP32: at most 128 VGPRs
+// (four waves per SIMD); FP64 unbounded.
+template <typename real, int FL>
+__global__ __launch_bounds__(256, sizeof(real) == 4 ? 4 : 1) void k_pair_forces_mol(int nper, int nmol, int rlm, const int *__restrict__ mj,
+                                                         const real *__restrict__ mg, const int *__restrict__ mcount,
+                                                         const double4 *__restrict__ xr, const real4<real> *__restrict__ dq,
+                                                         Box box, real beta, double rce2, double rcv2,
+                                                         const double *__restrict__ rh, const double *__restrict__ se,
+                                                         double *__restrict__ g_el, double *__restrict__ g_lj,
+                                                         double *__restrict__ rowf, double *__restrict__ epart) {
+    __shared__ double lds[2 * 16];
+    __shared__ int nq[MOL_NEAR_QUEUE * 256];  // (record index << 2 | k) of the lane's series pairs
+    const int lane = threadIdx.x % FL;
+    const int li = blockIdx.x * (blockDim.x / FL) + threadIdx.x / FL;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
+    int *queue = nq + threadIdx.x;  // entry u at queue[256 u]
+    double esum[2] = {0.0, 0.0};
+    if (li < nper) {
+        const int I = i / 3, p = i - 3 * I;
+        const double4 a = xr[i];
+        const real4<real> di = dq[i];
+        const real qi = di.w;
+        const double rhi = rh[i], sei = se[i];
+        const real fb = real(2) * beta * beta;
+        double gx = 0, gy = 0, gz = 0, lx = 0, ly = 0, lz = 0, fx = 0, fy = 0, fz = 0, ee = 0, el = 0;
+        const int c = mcount[I];
+        int qlen = 0;
+        // the pair (p, k) of record e on the series branch
+        auto near_pair = [&](int e, int k, real &pe, real &pg0, real &pg1, real &pg2, real &pf0, real &pf1, real &pf2) {
+            const unsigned word = (unsigned)mj[mell(e, I, nmol, rlm)];
+            const int J = (int)(word & ((1u << 26) - 1u)), code = (int)(word >> 26);
+            const int n[3] = {(code & 3) - 1, ((code >> 2) & 3) - 1, ((code >> 4) & 3) - 1};
+            const double4 b = xr[3 * J + k];
+            const real4<real> dk = dq[3 * J + k];
+            double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+            if (code == MOL_OWN_IMAGES)
+                min_image(dx, dy, dz, box);
+            else
+                apply_shift(dx, dy, dz, box, n);
+            const real x = (real)dx, y = (real)dy, z = (real)dz;
+            const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
+            real G[4];
+            pair_G<real, 4>(aij, beta, x * x + y * y + z * z, G);
+            MOL_FORCE_TERMS(G[0], G[1], G[2], G[3])
+        };
+        for (int e = lane; e < c; e += FL) {
+            const size_t q = mell(e, I, nmol, rlm);
+            const unsigned word = (unsigned)mj[q];
+            const int J = (int)(word & ((1u << 26) - 1u)), code = (int)(word >> 26);
+            const bool own = J == I;
+            const real *gr = mg + q * MOL_REC + 3 * p;
+            const real G1[3] = {gr[0], gr[1], gr[2]}, G2[3] = {gr[9], gr[10], gr[11]};
+            const int n[3] = {(code & 3) - 1, ((code >> 2) & 3) - 1, ((code >> 4) & 3) - 1};
+            real pg0 = 0, pg1 = 0, pg2 = 0, pf0 = 0, pf1 = 0, pf2 = 0, pe = 0;
+            unsigned nearm = 0u;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (own && k == p) continue;
+                const double4 b = xr[3 * J + k];
+                const real4<real> dk = dq[3 * J + k];
+                double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+                if (code == MOL_OWN_IMAGES)
+                    min_image(dx, dy, dz, box);
+                else
+                    apply_shift(dx, dy, dz, box, n);
+                const double r2d = dx * dx + dy * dy + dz * dz;
+                const double eps = sei * se[3 * J + k];
+                if (!own && eps != 0.0 && r2d < rcv2) {  // LJ, FP64 as in k_pair_forces
+                    const double rmin = rhi + rh[3 * J + k];
+                    const double ir2 = 1.0 / r2d;
+                    const double s2 = rmin * rmin * ir2;
+                    const double s6 = s2 * s2 * s2;
+                    el += eps * (s6 * s6 - 2.0 * s6);
+                    const double dl = eps * 12.0 * (s6 - s6 * s6) * ir2;
+                    lx += dl * dx;
+                    ly += dl * dy;
+                    lz += dl * dz;
+                }
+                if (r2d >= rce2) continue;
+                const real x = (real)dx, y = (real)dy, z = (real)dz;
+                const real r2 = x * x + y * y + z * z;
+                const real aij = rsqrt_(real(2) * (real)(a.w + b.w));
+                const real sa = aij * aij * r2, sb = beta * beta * r2;
+                if (sa >= real(1) && sb >= real(1)) {
+                    const real ir = rsqrt_(r2), ir2 = ir * ir;
+                    const real ca = real(TWO_OVER_SQRTPI) * aij * exp_(-sa);
+                    const real cb = real(TWO_OVER_SQRTPI) * beta * exp_(-sb);
+                    const real fa = real(2) * aij * aij;
+                    const real g1 = G1[k], g2 = G2[k];
+                    const real g0 = r2 * g1 + (ca - cb);
+                    const real g3 = (real(5) * g2 - (ca * fa * fa - cb * fb * fb)) * ir2;
+                    MOL_FORCE_TERMS(g0, g1, g2, g3)
+                } else {
+                    nearm |= 1u << k;
+                }
+            }
+            while (nearm) {  // queue the series pairs (evaluate at once when the queue is full)
+                const int k = __ffs(nearm) - 1;
+                nearm &= nearm - 1u;
+                if (qlen < MOL_NEAR_QUEUE)
+                    queue[256 * qlen++] = (e << 2) | k;
+                else
+                    near_pair(e, k, pe, pg0, pg1, pg2, pf0, pf1, pf2);
+            }
+            gx += (double)pg0;
+            gy += (double)pg1;
+            gz += (double)pg2;
+            fx += (double)pf0;
+            fy += (double)pf1;
+            fz += (double)pf2;
+            ee += (double)pe;
+        }
+        for (int u = 0; __ballot(u < qlen); ++u) {  // the queued series pairs, one per lane per round
+            if (u < qlen) {
+                const int v = queue[256 * u];
+                real pg0 = 0, pg1 = 0, pg2 = 0, pf0 = 0, pf1 = 0, pf2 = 0, pe = 0;
+                near_pair(v >> 2, v & 3, pe, pg0, pg1, pg2, pf0, pf1, pf2);
+                gx += (double)pg0;
+                gy += (double)pg1;
+                gz += (double)pg2;
+                fx += (double)pf0;
+                fy += (double)pf1;
+                fz += (double)pf2;
+                ee += (double)pe;
+            }
+        }
+        gx = group_sum<FL>(gx);
+        gy = group_sum<FL>(gy);
+        gz = group_sum<FL>(gz);
+        lx = group_sum<FL>(lx);
+        ly = group_sum<FL>(ly);
+        lz = group_sum<FL>(lz);
+        fx = group_sum<FL>(fx);
+        fy = group_sum<FL>(fy);
+        fz = group_sum<FL>(fz);
+        if (lane == 0) {
+            g_el[3 * i] = gx;
+            g_el[3 * i + 1] = gy;
+            g_el[3 * i + 2] = gz;
+            g_lj[3 * i] = lx;
+            g_lj[3 * i + 1] = ly;
+            g_lj[3 * i + 2] = lz;
+            rowf[3 * i] = fx;
+            rowf[3 * i + 1] = fy;
+            rowf[3 * i + 2] = fz;
+        }
+        esum[0] = ee;
+        esum[1] = el;
+    }
+    block_sum<2>(esum, lds);
+    if (threadIdx.x == 0) {
+        epart[2 * pb] = esum[0];
+        epart[2 * pb + 1] = esum[1];
+    }
+}
+#undef MOL_FORCE_TERMS
+
 }  // namespace pgm
