@@ -25,6 +25,15 @@ struct CGState {
 };
 static_assert(sizeof(CGState) == CG_STRIDE * sizeof(int), "CGState is CG_STRIDE ints");
 
+// The CG state after iteration k - 1 of bead b for the host: one 64-bit store into pinned host
+// memory (slot k % 4) of the solve's sequence number, k and conv_it (clamped to 0xffff, which stands
+// for "not converged"), so the host learns of convergence without a copy and an event per check.
+PGM_DEV void cg_mirror(unsigned long long *mirror, int nbead, int b, unsigned seq, int k, int conv) {
+    if (!mirror) return;
+    const unsigned long long c16 = conv < 0xffff ? (unsigned long long)conv : 0xffffull;
+    mirror[(size_t)(k & 3) * nbead + b] = ((unsigned long long)seq << 32) | ((unsigned long long)(k & 0xffff) << 16) | c16;
+}
+
 // Sum of np partials (stride `stride`, component c) by the whole block, fixed order; result in all threads.
 PGM_DEV double block_reduce_partials(const double *__restrict__ part, int np, int stride, int c, double *lds, bool is_max) {
     double v = 0.0;
@@ -151,11 +160,12 @@ __global__ void k_set_norm_value(CGState *st, double norm) {
 }
 
 // ---------------------------------------------------------------------------- CG
-// Start: z = alpha r, p = z, partials (r . z, max |alpha r|).
+// Start: z = alpha r, p = z, x = x0 (0 when x0 is null), partials (r . z, max |alpha r|).
 template <typename real>
 __global__ __launch_bounds__(256) void k_cg_start(int nper, const real4<real> *__restrict__ r, const real *__restrict__ alpha,
                                                   real4<real> *__restrict__ z, real4<real> *__restrict__ p,
-                                                  double *__restrict__ part) {
+                                                  double *__restrict__ part, const real4<real> *__restrict__ x0,
+                                                  real4<real> *__restrict__ x) {
     __shared__ double lds[32];
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * nper + li;
@@ -172,6 +182,7 @@ __global__ __launch_bounds__(256) void k_cg_start(int nper, const real4<real> *_
         zz.w = 0;
         z[i] = zz;
         p[i] = zz;
+        x[i] = x0 ? x0[i] : real4<real>{};
         s[0] = (double)(rr.x * zz.x) + (double)(rr.y * zz.y) + (double)(rr.z * zz.z);
         m = fmax(fmax(fabs((double)zz.x), fabs((double)zz.y)), fabs((double)zz.z));
     }
@@ -185,7 +196,8 @@ __global__ __launch_bounds__(256) void k_cg_start(int nper, const real4<real> *_
 
 // rz[0], err[0], conv_it (one block per bead; np partials per bead, rows of rzs).
 __global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGState *st, double *__restrict__ rz,
-                                  double *__restrict__ err, int rzs, double tol, int max_iter) {
+                                  double *__restrict__ err, int rzs, double tol, int max_iter,
+                                  unsigned long long *mirror, unsigned seq) {
     __shared__ double lds[32];
     const int b = blockIdx.x;
     part += (size_t)b * np * 2;
@@ -199,6 +211,7 @@ __global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGSta
         const double e = m / st->norm;
         err[0] = e;
         st->conv_it = (e > tol && 0 < max_iter) ? 0x7fffffff : 0;
+        cg_mirror(mirror, gridDim.x, b, seq, 0, st->conv_it);
     }
 }
 
@@ -256,13 +269,18 @@ __global__ __launch_bounds__(256) void k_cg_update1(int nper, const double *__re
 
 // Iteration k, second half: beta = z'.(r' - r) / max(rz_k, 1e-300) (rounded to real); p = z' + beta p.
 // Block 0 of each bead records rz[k+1], err[k+1] and, when converged or out of iterations,
-// conv_it = k + 1.
+// conv_it = k + 1, and mirrors the state to the host (slot k + 1; also when the iteration is a no-op).
 template <typename real>
 __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__restrict__ part, int np, double *__restrict__ rz,
                                                     double *__restrict__ err, int rzs, int k, const real4<real> *__restrict__ z,
-                                                    real4<real> *__restrict__ p, CGState *st, double tol, int max_iter) {
+                                                    real4<real> *__restrict__ p, CGState *st, double tol, int max_iter,
+                                                    unsigned long long *mirror, unsigned seq) {
     const int b = blockIdx.y;
-    if (k >= st[b].conv_it) return;
+    const int conv0 = st[b].conv_it;
+    if (k >= conv0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mirror(mirror, gridDim.y, b, seq, k + 1, conv0);
+        return;
+    }
     __shared__ double lds[32];
     part += (size_t)b * np * 3;
     rz += (size_t)b * rzs;
@@ -286,7 +304,9 @@ __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__re
         rz[k + 1] = s1;
         const double e = mx / st[b].norm;
         err[k + 1] = e;
-        if (!(e > tol && k + 1 < max_iter)) st[b].conv_it = k + 1;
+        const int conv = (e > tol && k + 1 < max_iter) ? conv0 : k + 1;
+        if (conv != conv0) st[b].conv_it = conv;
+        cg_mirror(mirror, gridDim.y, b, seq, k + 1, conv);
     }
 }
 
@@ -379,12 +399,14 @@ PGM_DEV real4<real> mol_apply(const real *__restrict__ Mi, int p, const real *sr
 
 constexpr int MOLP_BLOCK = 192;  // 64 whole molecules per workgroup
 
-// k_cg_start with the molecule blocks: z = M^-1 r, p = z, partials (r . z, max |alpha r|).
+// k_cg_start with the molecule blocks: z = M^-1 r, p = z, x = x0 (0 when x0 is null), partials
+// (r . z, max |alpha r|).
 template <typename real>
 __global__ __launch_bounds__(MOLP_BLOCK) void k_cg_start_mol(int nper, const real4<real> *__restrict__ r,
                                                              const real *__restrict__ alpha, const real *__restrict__ minv,
                                                              real4<real> *__restrict__ z, real4<real> *__restrict__ p,
-                                                             double *__restrict__ part) {
+                                                             double *__restrict__ part, const real4<real> *__restrict__ x0,
+                                                             real4<real> *__restrict__ x) {
     __shared__ real sr[3 * MOLP_BLOCK];
     __shared__ double lds[32];
     const int li = blockIdx.x * MOLP_BLOCK + threadIdx.x;
@@ -405,6 +427,7 @@ __global__ __launch_bounds__(MOLP_BLOCK) void k_cg_start_mol(int nper, const rea
         const real4<real> zz = mol_apply<real>(minv + (size_t)81 * (i / 3), pa, sr + 9 * q);
         z[i] = zz;
         p[i] = zz;
+        x[i] = x0 ? x0[i] : real4<real>{};
         const real a = alpha[i];
         s[0] = (double)(rr.x * zz.x) + (double)(rr.y * zz.y) + (double)(rr.z * zz.z);
         m = fmax(fmax(fabs((double)(a * rr.x)), fabs((double)(a * rr.y))), fabs((double)(a * rr.z)));

@@ -198,6 +198,8 @@ struct pgm_ctx {
     DevBuf bead_it, bead_res;      // iterations and final residual of every bead's last solve
     int *h_pin_int = nullptr;      // 16 + nbead ints (bead iterations from 16)
     CGState *h_pin_conv = nullptr; // CG state read-backs of the pipelined CG (4 slots of nbead states)
+    unsigned long long *h_mirror = nullptr;  // CG states written by the update kernels (cg_mirror), 4 slots of nbead
+    unsigned cg_seq = 0;                     // sequence number of the current CG solve (the mirror's stamp)
     double *h_pin_dbl = nullptr;   // 32 + 5 nbead doubles (bead energies from 32, residuals after them)
     int last_iters = 4;
     int last_adj_iters = 4;  // iterations of the last adjoint solve (read-back hint)
@@ -1384,15 +1386,49 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
     const int max_iter = c->s.max_iter;
     int launched = 0;
     int conv = 0x7fffffff;  // last conv_it seen on the host (largest over the beads)
+    // Eager iterations mirror the CG states to pinned host memory from the update kernel (cg_mirror,
+    // stamped with the solve's sequence number and the slot); captured iterations keep their kernel
+    // arguments from the capture, so with graphs the states are copied back after an event instead.
+    const bool mirror = !graphs && max_iter < 0xffff;
+    const unsigned seq = c->cg_seq;
     auto read_conv = [&](int k) -> int {  // queue the read-back of the CG states after iteration k-1 (slot k)
+        if (mirror) return PGM_OK;
         HIP_TRY(hipMemcpyAsync(c->h_pin_conv + (size_t)(k & 3) * B, S, sizeof(CGState) * B, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipEventRecord(c->ev_read[k & 3], st));
         return PGM_OK;
     };
-    auto seen_conv = [&](int k) -> int {  // the slowest bead's conv_it in read slot k (after it completed)
+    auto wait_conv = [&](int k, int &out) -> int {  // the slowest bead's conv_it after iteration k-1 (slot k)
         int m = 0;
-        for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
-        return m;
+        if (!mirror) {
+            HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
+            for (int b2 = 0; b2 < B; ++b2) m = std::max(m, c->h_pin_conv[(size_t)(k & 3) * B + b2].conv_it);
+            out = m;
+            return PGM_OK;
+        }
+        const unsigned long long want = ((unsigned long long)seq << 32) | ((unsigned long long)(k & 0xffff) << 16);
+        for (int b2 = 0; b2 < B; ++b2) {
+            volatile unsigned long long *w = c->h_mirror + (size_t)(k & 3) * B + b2;
+            unsigned long long v = *w;
+            for (long polls = 1; (v & ~0xffffull) != want; ++polls) {
+                if (polls % 4096 == 0) {  // the stream idle or failed with the state still missing: stop
+                    const hipError_t q = hipStreamQuery(st);
+                    if (q != hipSuccess && q != hipErrorNotReady) {
+                        g_err = std::string("CG state wait: ") + hipGetErrorString(q);
+                        return PGM_ERR_HIP;
+                    }
+                    if (q == hipSuccess && ((v = *w) & ~0xffffull) != want) {
+                        g_err = "CG state wait: the stream finished without writing the state";
+                        return PGM_ERR_HIP;
+                    }
+                }
+                __builtin_ia32_pause();
+                v = *w;
+            }
+            const int cv = (int)(v & 0xffffull);
+            m = std::max(m, cv == 0xffff ? 0x7fffffff : cv);
+        }
+        out = m;
+        return PGM_OK;
     };
     // iteration k on c->stream: the field of p, then the two update kernels (all early-exit once converged)
     auto iteration = [&](int k) -> int {
@@ -1406,7 +1442,7 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
             k_cg_update1<real><<<gB, 256, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
                                                    c->alpha_r.as<real>(), c->partB.as<double>(), S);
         k_cg_update2<real><<<gB, 256, 0, cs>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, tol,
-                                               max_iter);
+                                               max_iter, mirror ? c->h_mirror : nullptr, seq);
         LAUNCH_CHECK();
         return PGM_OK;
     };
@@ -1464,14 +1500,12 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
         if (launched >= expect - 2 || launched == max_iter) PGM_TRY(read_conv(launched));
         if (launched >= expect - 1 || launched == max_iter) {
             // wait for the state after iteration k-1 (read slot k) while iteration k runs
-            HIP_TRY(hipEventSynchronize(c->ev_read[k & 3]));
-            conv = seen_conv(k);
+            PGM_TRY(wait_conv(k, conv));
             if (conv <= k) break;  // every bead converged before iteration k: k was a no-op
         }
     }
     if (conv > launched) {  // not yet seen converged: the state after the last launched iteration decides
-        HIP_TRY(hipEventSynchronize(c->ev_read[launched & 3]));
-        conv = seen_conv(launched);
+        PGM_TRY(wait_conv(launched, conv));
     }
     *launched_out = launched;
     return std::min(conv, launched);
@@ -1550,13 +1584,13 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     // A trial solve tests convergence against its own geometry's normaliser but leaves the state's
     // as it was (forcefield.py:2612-2624): the next solve reloads ind->norm.
     c->norm_on_device = !trial;
-    HIP_TRY(hipMemcpyAsync(x, x0, sizeof(R4) * n, hipMemcpyDeviceToDevice, st));
-    if (c->molp)
+    if (c->molp)  // x = x0 in the start kernel
         k_cg_start_mol<real><<<gM, MOLP_BLOCK, 0, st>>>(nper, r, c->alpha_r.as<real>(), c->pinv.as<real>(), z, p,
-                                                         c->partB.as<double>());
+                                                         c->partB.as<double>(), x0, x);
     else
-        k_cg_start<real><<<gB, 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
-    k_cg_start_finish<<<B, 256, 0, st>>>(c->partB.as<double>(), npb, S, rz, err, c->rzs, c->s.dipole_tol, c->s.max_iter);
+        k_cg_start<real><<<gB, 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>(), x0, x);
+    k_cg_start_finish<<<B, 256, 0, st>>>(c->partB.as<double>(), npb, S, rz, err, c->rzs, c->s.dipole_tol, c->s.max_iter,
+                                         c->h_mirror, ++c->cg_seq);
     LAUNCH_CHECK();
     const int max_iter = c->s.max_iter;
     int launched = 0;
@@ -1959,7 +1993,8 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
         }
     if (hipHostMalloc((void **)&c->h_pin_int, (16 + nbead) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_pin_dbl, (32 + 5 * (size_t)nbead) * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_pin_conv, 4 * (size_t)nbead * sizeof(CGState), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_pin_conv, 4 * (size_t)nbead * sizeof(CGState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_mirror, 4 * (size_t)nbead * sizeof(unsigned long long), hipHostMallocCoherent) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         return fail(PGM_ERR_HIP);
     }
@@ -2059,6 +2094,7 @@ extern "C" int pgm_ctx_destroy(pgm_ctx *c) {
     for (DevBuf *b : all) b->release();
     if (c->h_pin_int) (void)hipHostFree(c->h_pin_int);
     if (c->h_pin_conv) (void)hipHostFree(c->h_pin_conv);
+    if (c->h_mirror) (void)hipHostFree(c->h_mirror);
     if (c->h_pin_dbl) (void)hipHostFree(c->h_pin_dbl);
     delete c;
     return PGM_OK;
@@ -2327,14 +2363,14 @@ static int dipole_vjp_impl(pgm_ctx *c, const double *pos, const double *mu, cons
     k_adjoint_norm<<<1, 256, 0, st>>>(n, mu_bar, c->alpha64.as<double>(), S);
     k_mu_to_x0<real><<<nb, 256, 0, st>>>(n, mu_bar, r);
     LAUNCH_CHECK();
-    HIP_TRY(hipMemsetAsync(x, 0, sizeof(R4) * n, st));
-    if (c->molp)
-        k_cg_start_mol<real><<<dim3(nblk(nper, MOLP_BLOCK), 1), MOLP_BLOCK, 0, st>>>(nper, r, c->alpha_r.as<real>(),
-                                                                                c->pinv.as<real>(), z, p, c->partB.as<double>());
+    if (c->molp)  // x = 0 in the start kernel
+        k_cg_start_mol<real><<<dim3(nblk(nper, MOLP_BLOCK), 1), MOLP_BLOCK, 0, st>>>(
+            nper, r, c->alpha_r.as<real>(), c->pinv.as<real>(), z, p, c->partB.as<double>(), nullptr, x);
     else
-        k_cg_start<real><<<bead_grid(c, nper), 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>());
+        k_cg_start<real><<<bead_grid(c, nper), 256, 0, st>>>(nper, r, c->alpha_r.as<real>(), z, p, c->partB.as<double>(),
+                                                              nullptr, x);
     k_cg_start_finish<<<1, 256, 0, st>>>(c->partB.as<double>(), npb, S, c->rz.as<double>(), c->err.as<double>(), c->rzs, tol,
-                                         c->s.max_iter);
+                                         c->s.max_iter, c->h_mirror, ++c->cg_seq);
     LAUNCH_CHECK();
     c->norm_on_device = false;  // CGState.norm now holds the adjoint's normaliser
     int launched = 0;
