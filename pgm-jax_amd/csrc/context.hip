@@ -198,7 +198,8 @@ struct pgm_ctx {
     DevBuf bead_it, bead_res;      // iterations and final residual of every bead's last solve
     int *h_pin_int = nullptr;      // 16 + nbead ints (bead iterations from 16)
     CGState *h_pin_conv = nullptr; // CG state read-backs of the pipelined CG (4 slots of nbead states)
-    unsigned long long *h_mirror = nullptr;  // CG states written by the update kernels (cg_mirror), 4 slots of nbead
+    unsigned long long *h_mirror = nullptr;  // CG states written by the update kernels (cg_mirror), 4 slots of nbead,
+                                             // then the MD step's largest displacement (a double, h_disp)
     unsigned cg_seq = 0;                     // sequence number of the current CG solve (the mirror's stamp)
     double *h_pin_dbl = nullptr;   // 32 + 5 nbead doubles (bead energies from 32, residuals after them)
     int last_iters = 4;
@@ -251,6 +252,8 @@ struct pgm_ctx {
 // Device slots after the per-bead energies: [0] sum m v^2 of the MD step, [1] largest displacement
 // since the row build.
 static double *aux_slots(const pgm_ctx *c) { return c->energy.as<double>() + 4 * c->nbead; }
+// The largest displacement since the row build, written by the MD step's reduction into pinned host memory.
+static double *h_disp(const pgm_ctx *c) { return reinterpret_cast<double *>(c->h_mirror + 4 * (size_t)c->nbead); }
 
 static void drop_graphs(pgm_ctx *c) {
     for (hipGraphExec_t &g : c->it_graphs)
@@ -1361,9 +1364,10 @@ static int guess_harvest(pgm_ctx *c, const pgm_induction *ind) {
 // ---------------------------------------------------------------------------- induced dipoles
 // Predictor, initial residual (fused or plain) and CG (pgm_jax/md/forcefield.py:1917-2008, :1714-1821).
 // The host queues iterations without waiting: up to the iteration count of the last solve it does
-// not look at the device at all; from there on, after queueing iteration k it waits for the
-// convergence flag written by iteration k-1 (iteration k keeps the GPU busy meanwhile, and turns
-// into no-ops if k-1 converged).  With `quiet` (the MD loop) nothing else is read back: the
+// not look at the device at all; for the last two before that count, after queueing iteration k it
+// waits for the convergence state written by iteration k-1 (iteration k keeps the GPU busy
+// meanwhile, and turns into no-ops if k-1 converged); past the count, iteration k is queued only
+// once k-1 has reported that it did not converge.  With `quiet` (the MD loop) nothing else is read back: the
 // residual and the normaliser stay on the device (stats slot `slot`).  With beads every bead runs
 // its own CG in the same launches; the host waits for the slowest, and iterations / residual are
 // the largest over the beads (per bead: bead_it, bead_res).  Bead normalisers stay on the device.
@@ -1450,6 +1454,12 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
     PGM_TRY(read_conv(0));  // converged before any iteration?
     while (launched < max_iter) {
         const int k = launched;
+        if (k >= expect && k > 0) {
+            // past the expected count: queue iteration k only once iteration k-1 has reported (a
+            // few microseconds of idle GPU against a whole iteration of no-op launches)
+            PGM_TRY(wait_conv(k, conv));
+            if (conv <= k) break;
+        }
         if (graphs) {
             if ((int)c->it_graphs.size() <= k) c->it_graphs.resize(k + 1, nullptr);
             if (gs != st) {
@@ -1498,7 +1508,7 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
         // slot `launched` is waited on at the next pass (launched >= expect - 2) or after the loop;
         // earlier reads would only add a copy and its gap to the queue
         if (launched >= expect - 2 || launched == max_iter) PGM_TRY(read_conv(launched));
-        if (launched >= expect - 1 || launched == max_iter) {
+        if ((launched >= expect - 1 && launched <= expect) || launched == max_iter) {
             // wait for the state after iteration k-1 (read slot k) while iteration k runs
             PGM_TRY(wait_conv(k, conv));
             if (conv <= k) break;  // every bead converged before iteration k: k was a no-op
@@ -1994,7 +2004,8 @@ extern "C" int pgm_ctx_create(pgm_ctx **out, const pgm_topology *top, const pgm_
     if (hipHostMalloc((void **)&c->h_pin_int, (16 + nbead) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_pin_dbl, (32 + 5 * (size_t)nbead) * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_pin_conv, 4 * (size_t)nbead * sizeof(CGState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_mirror, 4 * (size_t)nbead * sizeof(unsigned long long), hipHostMallocCoherent) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_mirror, (4 * (size_t)nbead + 1) * sizeof(unsigned long long), hipHostMallocCoherent) !=
+            hipSuccess) {
         g_err = "hipHostMalloc failed";
         return fail(PGM_ERR_HIP);
     }
@@ -3105,8 +3116,9 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         disp_pending = true;
     }
     for (int k = 0; k < nsteps; ++k) {
-        if (disp_pending) {  // the read-back queued last step is complete: the CG waited on later work
-            if (!c->rows_valid || c->h_pin_dbl[9] > trigger) {
+        if (disp_pending) {  // the value written last step is in: the CG waited on later work of the stream
+            const double disp = k == 0 ? c->h_pin_dbl[9] : *(volatile double *)h_disp(c);
+            if (!c->rows_valid || disp > trigger) {
                 PGM_TRY(md_first(c, s, dt));
                 PGM_TRY(build_rows(c, s->pos));
                 goto forces;
@@ -3115,9 +3127,8 @@ extern "C" int pgm_md_run(pgm_ctx *c, pgm_md_state *s, int nsteps, double dt, pg
         PGM_TRY(md_first(c, s, dt));
     forces:
         k_max_disp<<<nblk(n, 256), 256, 0, st>>>(n, s->pos, c->pos_build.as<double>(), c->md_part.as<double>());
-        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, aux_slots(c) + 1);
+        k_reduce_partials<<<1, 256, 0, st>>>(c->md_part.as<double>(), nblk(n, 256), 1, aux_slots(c) + 1, h_disp(c));
         LAUNCH_CHECK();
-        HIP_TRY(hipMemcpyAsync(c->h_pin_dbl + 9, aux_slots(c) + 1, sizeof(double), hipMemcpyDeviceToHost, st));
         disp_pending = true;
         {
             const int rebuilds_before = c->rebuilds;

@@ -342,7 +342,9 @@ __global__ void k_accumulate_partials(const double *__restrict__ part, int np, d
 }
 
 // One block: out = reduction of np partials (sum or max).
-__global__ void k_reduce_partials(const double *__restrict__ part, int np, int is_max, double *__restrict__ out) {
+// host_out (pinned, coherent host memory; may be null): the result also written there.
+__global__ void k_reduce_partials(const double *__restrict__ part, int np, int is_max, double *__restrict__ out,
+                                  double *host_out = nullptr) {
     __shared__ double lds[32];
     double v = 0.0;
     for (int b = threadIdx.x; b < np; b += blockDim.x) v = is_max ? fmax(v, part[b]) : v + part[b];
@@ -353,7 +355,10 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int np, int i
         block_sum<1>(a, lds);
         v = a[0];
     }
-    if (threadIdx.x == 0) out[0] = v;
+    if (threadIdx.x == 0) {
+        out[0] = v;
+        if (host_out) *host_out = v;
+    }
 }
 
 }  // namespace pgm
