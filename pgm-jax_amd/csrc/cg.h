@@ -539,6 +539,60 @@ __global__ void k_cg_finish(int n, const real4<real> *__restrict__ x, const real
     }
 }
 
+// The end of a solve in one launch (k_cg_finish, then the pushes of mu into the learned guess's
+// ring, slot `ring_slot`, and into the predictor history, each skipped when its pointer is null,
+// then k_record_residual by block 0).
+template <typename real>
+__global__ __launch_bounds__(256) void k_cg_finish_all(int n, const real4<real> *__restrict__ x, const real4<real> *__restrict__ r,
+                                                       const real *__restrict__ alpha, double peek, double *__restrict__ mu,
+                                                       double *__restrict__ ring, int ring_slot, double *__restrict__ hist,
+                                                       int nbead, const double *__restrict__ err, int rzs,
+                                                       const CGState *__restrict__ st, int launched, int *__restrict__ bead_it,
+                                                       double *__restrict__ bead_res, double *__restrict__ slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const real4<real> xx = x[i], rr = r[i];
+        const real a = alpha[i];
+        const real pk = (real)peek;
+        double m[3];
+        if (peek != 0.0) {
+            m[0] = (double)(xx.x + pk * rr.x * a);
+            m[1] = (double)(xx.y + pk * rr.y * a);
+            m[2] = (double)(xx.z + pk * rr.z * a);
+        } else {
+            m[0] = (double)xx.x;
+            m[1] = (double)xx.y;
+            m[2] = (double)xx.z;
+        }
+        const size_t s3 = 3 * (size_t)n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = 3 * (size_t)i + c;
+            mu[o] = m[c];
+            if (ring) ring[(size_t)ring_slot * s3 + o] = m[c];
+            if (hist) {
+                hist[3 * s3 + o] = hist[2 * s3 + o];
+                hist[2 * s3 + o] = hist[s3 + o];
+                hist[s3 + o] = hist[o];
+                hist[o] = m[c];
+            }
+        }
+    }
+    if (blockIdx.x == 0) {  // k_record_residual
+        __shared__ double lds[32];
+        double mx = 0.0;
+        for (int b = threadIdx.x; b < nbead; b += blockDim.x) {
+            const int it = min(st[b].conv_it, launched);
+            const double e = err[(size_t)b * rzs + it];
+            bead_it[b] = it;
+            bead_res[b] = e;
+            mx = fmax(mx, e);
+        }
+        mx = block_max(mx, lds);
+        if (threadIdx.x == 0) slot[0] = mx;
+    }
+}
+
 // hist = [mu, hist[0], hist[1], hist[2]] (in place, newest first).
 __global__ void k_hist_push(int n3, const double *__restrict__ mu, double *__restrict__ hist) {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;

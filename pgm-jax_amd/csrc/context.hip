@@ -1127,7 +1127,10 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
         return PGM_OK;
     }
     PGM_TRY(spread_fft<real>(c, dq, withq, false, conv, iter, nullptr, late || (fused && c->fork_stage == 3) ? &fork : nullptr));
-    if (!serial) HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+    if (!serial) {
+        HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+        c->rec_pending = false;  // the row field followed the record pass on the side stream
+    }
     PGM_TRY(gather_combine<real>(c, dq, mode, extra, out, part, conv, iter, timing_pair(c, 2)));
     return PGM_OK;
 }
@@ -1608,18 +1611,25 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
                                     c->use_graphs && !c->timing, &launched);
     if (it < 0) return it;
     c->last_iters = std::max(1, it);
-    k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
-    if (use_net && ind->count % c->guess.every == 0) PGM_TRY(guess_harvest(c, ind));
+    double *slot = c->stats.as<double>() + (c->stats_slot % c->stats_cap);
+    auto &G = c->guess;
+    if (use_net && ind->count % c->guess.every == 0) {  // a harvest reads the ring before mu joins it
+        k_cg_finish<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu);
+        PGM_TRY(guess_harvest(c, ind));
+        if (learned) k_ring_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, G.ring.as<double>(), G.head);
+        if (!trial) k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
+        k_record_residual<<<1, 256, 0, st>>>(B, err, c->rzs, S, launched, c->bead_it.as<int>(), c->bead_res.as<double>(), slot);
+    } else {
+        k_cg_finish_all<real><<<nb, 256, 0, st>>>(n, x, r, c->alpha_r.as<real>(), c->s.peek, ind->mu,
+                                                  learned ? G.ring.as<double>() : nullptr, G.head, trial ? nullptr : ind->hist, B,
+                                                  err, c->rzs, S, launched, c->bead_it.as<int>(), c->bead_res.as<double>(),
+                                                  slot);
+    }
+    LAUNCH_CHECK();
     if (learned) {  // the guess's own history ring
-        auto &G = c->guess;
-        k_ring_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, G.ring.as<double>(), G.head);
         G.head = (G.head + 1) % GUESS_K;
         G.nring = std::min(G.nring + 1, GUESS_K);
     }
-    if (!trial) k_hist_push<<<nblk(3 * n, 256), 256, 0, st>>>(3 * n, ind->mu, ind->hist);
-    double *slot = c->stats.as<double>() + (c->stats_slot % c->stats_cap);
-    k_record_residual<<<1, 256, 0, st>>>(B, err, c->rzs, S, launched, c->bead_it.as<int>(), c->bead_res.as<double>(), slot);
-    LAUNCH_CHECK();
     c->stats_slot += 1;
     if (!trial) ind->count += 1;
     if (iterations) *iterations = it;
