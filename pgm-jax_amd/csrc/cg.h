@@ -20,7 +20,7 @@ namespace pgm {
 
 struct CGState {
     int conv_it;  // first iteration index that must not run (INT_MAX while unconverged)
-    int pad;
+    unsigned seq; // sequence number of the solve (set by k_cg_start_finish; stamps the host mirror)
     double norm;  // mean |alpha b| (convergence normaliser)
 };
 static_assert(sizeof(CGState) == CG_STRIDE * sizeof(int), "CGState is CG_STRIDE ints");
@@ -211,6 +211,7 @@ __global__ void k_cg_start_finish(const double *__restrict__ part, int np, CGSta
         const double e = m / st->norm;
         err[0] = e;
         st->conv_it = (e > tol && 0 < max_iter) ? 0x7fffffff : 0;
+        st->seq = seq;
         cg_mirror(mirror, gridDim.x, b, seq, 0, st->conv_it);
     }
 }
@@ -274,11 +275,11 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__restrict__ part, int np, double *__restrict__ rz,
                                                     double *__restrict__ err, int rzs, int k, const real4<real> *__restrict__ z,
                                                     real4<real> *__restrict__ p, CGState *st, double tol, int max_iter,
-                                                    unsigned long long *mirror, unsigned seq) {
+                                                    unsigned long long *mirror) {
     const int b = blockIdx.y;
     const int conv0 = st[b].conv_it;
     if (k >= conv0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mirror(mirror, gridDim.y, b, seq, k + 1, conv0);
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mirror(mirror, gridDim.y, b, st[b].seq, k + 1, conv0);
         return;
     }
     __shared__ double lds[32];
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void k_cg_update2(int nper, const double *__re
         err[k + 1] = e;
         const int conv = (e > tol && k + 1 < max_iter) ? conv0 : k + 1;
         if (conv != conv0) st[b].conv_it = conv;
-        cg_mirror(mirror, gridDim.y, b, seq, k + 1, conv);
+        cg_mirror(mirror, gridDim.y, b, st[b].seq, k + 1, conv);
     }
 }
 

@@ -1393,10 +1393,11 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
     const int max_iter = c->s.max_iter;
     int launched = 0;
     int conv = 0x7fffffff;  // last conv_it seen on the host (largest over the beads)
-    // Eager iterations mirror the CG states to pinned host memory from the update kernel (cg_mirror,
-    // stamped with the solve's sequence number and the slot); captured iterations keep their kernel
-    // arguments from the capture, so with graphs the states are copied back after an event instead.
-    const bool mirror = !graphs && max_iter < 0xffff;
+    // The update kernel mirrors the CG states to pinned host memory (cg_mirror, stamped with the
+    // solve's sequence number, which k_cg_start_finish keeps in the device state, so that captured
+    // iterations stamp the current solve too, and the slot); iteration counts beyond 16 bits take
+    // a copy and an event per check instead.
+    const bool mirror = max_iter < 0xffff;
     const unsigned seq = c->cg_seq;
     auto read_conv = [&](int k) -> int {  // queue the read-back of the CG states after iteration k-1 (slot k)
         if (mirror) return PGM_OK;
@@ -1449,7 +1450,7 @@ static int cg_iterate(pgm_ctx *c, double tol, int expect, bool graphs, int *laun
             k_cg_update1<real><<<gB, 256, 0, cs>>>(nper, c->partA.as<double>(), gather_parts(c), rz, c->rzs, k, x, r, z, p, Ap,
                                                    c->alpha_r.as<real>(), c->partB.as<double>(), S);
         k_cg_update2<real><<<gB, 256, 0, cs>>>(nper, c->partB.as<double>(), npb, rz, err, c->rzs, k, z, p, S, tol,
-                                               max_iter, mirror ? c->h_mirror : nullptr, seq);
+                                               max_iter, mirror ? c->h_mirror : nullptr);
         LAUNCH_CHECK();
         return PGM_OK;
     };
