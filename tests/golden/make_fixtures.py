@@ -16,6 +16,10 @@ Every output is data (inputs and expected outputs), never reference source:
 * ``pgm3p25_512.prmtop``, ``pgm3p25_512.rst7`` -- byte copies of the two input files of
   ``examples/water512`` (the reference's install check, tests/test_water512.py), read by the
   product's own Amber readers in the tests of the MD runner.
+* ``md_rigid_points.npz`` -- the snapshots of two of the reference's rigid-water trajectory goldens
+  (md_rigid_nve, md_rigid_npt: 64 pGM3P-25 waters, float64, CG to 1e-10, tests/regression/
+  regression_cases.py:356-411): per snapshot the positions, box, induced dipoles and the recorded
+  elec / vdw / potential energies, as single points at new geometries (the NPT ones in changed boxes).
 * ``pmemd_pbc.npz`` -- the independent Fortran (pmemd-pgm) single point of
   data/validation/amber_ref/pmemd_pbc: EELEC / VDWAALS (kcal/mol), forces (kcal/mol/A) and induced
   dipoles (e A).
@@ -129,6 +133,18 @@ def pmemd_pbc(ref: str) -> dict:
     return {"eelec_kcal": np.array(eelec), "vdw_kcal": np.array(vdw), "forces_kcal_A": frc, "mu_eA": np.array(mu)}
 
 
+def md_rigid_points(gold: str) -> dict:
+    """Snapshots (positions, box, induced dipoles, energies) of the md_rigid_nve / md_rigid_npt goldens."""
+    out = {}
+    for case, short in (("md_rigid_nve", "nve"), ("md_rigid_npt", "npt")):
+        with np.load(os.path.join(gold, case + ".npz"), allow_pickle=False) as z:
+            tags = sorted({k.split(".")[0] for k in z.files if k.startswith("t")})
+            for t in tags:
+                for k in ("pos", "box", "mu", "obs.epot", "obs.elec", "obs.vdw", "obs.cg_resid_max"):
+                    out[f"{short}.{t}.{k}"] = z[f"{t}.{k}"]
+    return out
+
+
 def copy_water512_inputs(ref: str) -> None:
     """Copy the prmtop and NetCDF restart of examples/water512 into tests/golden/."""
     for name in ("pgm3p25_512.prmtop", "pgm3p25_512.rst7"):
@@ -143,6 +159,7 @@ def main(ref: str) -> None:
     for name in ("ff_small_box", "ff_pgm3p25_512", "periodic_model"):
         with np.load(os.path.join(gold, name + ".npz"), allow_pickle=False) as z:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **{k: z[k] for k in z.files})
+    np.savez_compressed(os.path.join(HERE, "md_rigid_points.npz"), **md_rigid_points(gold))
     try:
         np.savez_compressed(os.path.join(HERE, "pmemd_pbc.npz"), **pmemd_pbc(ref))
     except (OSError, IndexError, KeyError) as exc:  # optional independent check
