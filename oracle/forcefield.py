@@ -46,6 +46,8 @@ from .pairs import min_image, pairs
 from .pme import PME, grid_size
 
 KE = 138.935458  # Coulomb constant, kJ/mol nm / e^2 (pgm_jax/units.py:31)
+FARADAY_KJ = 96.48533212331002  # kJ/mol per (e V) (pgm_jax/units.py:45)
+VNM_TO_INTERNAL = FARADAY_KJ / KE  # a field of 1 V/nm in e/nm^2 (pgm_jax/md/efield.py:104)
 SQRT_PI = math.sqrt(math.pi)
 PRED = {"mu3": (3.0, -3.0, 1.0), "mu4": (4.0, -6.0, 4.0, -1.0)}  # forcefield.py:605
 
@@ -97,6 +99,7 @@ class Result:
     iterations: int
     residual: float
     induction: InductionState
+    dipole: np.ndarray | None = None  # M = sum q r + sum (p + mu) + offset with an external field
 
 
 def lj_long_range(rh: np.ndarray, se: np.ndarray, volume: float, rc: float) -> float:
@@ -222,15 +225,19 @@ class ForceField:
         n = self.top.n
         return InductionState(np.zeros((n, 3)), np.zeros((4, n, 3)), 0, 1.0)
 
-    def solve(self, g: dict, S: dict, Gk: np.ndarray, p: np.ndarray, ind: InductionState) -> tuple:
-        """Predictor + (fused) initial residual + CG; returns (mu, it, err, new state) (forcefield.py:1917-2008)."""
+    def solve(self, g: dict, S: dict, Gk: np.ndarray, p: np.ndarray, ind: InductionState, F0=None) -> tuple:
+        """Predictor + (fused) initial residual + CG; returns (mu, it, err, new state) (forcefield.py:1917-2008).
+
+        F0: a uniform external field (3,) [e/nm^2] added to the permanent field of every atom (the
+        right-hand side of the induction equations, forcefield.py:1966-1997 with _ext_at :1823-1832)."""
         t, s = self.top, self.s
         a64 = t.alpha[:, None]
         A = self.operator(g, S, Gk)
         q = t.q
+        ext = np.zeros(3) if F0 is None else np.asarray(F0, float)
 
         def plain(x0h, have):
-            b = self.field(g, S, Gk, q, p)
+            b = self.field(g, S, Gk, q, p) + ext
             ab = a64 * b
             x0 = x0h if have else ab
             return x0, b - A(x0), float(np.mean(np.abs(ab))) + 1e-300
@@ -241,7 +248,7 @@ class ForceField:
             x0h = sum(ci * ind.hist[k] for k, ci in enumerate(c))
             if s.fused and have and ind.count % s.norm_refresh != 0:
                 x0 = x0h
-                r0 = self.field(g, S, Gk, q, p + x0h) - div_alpha(x0h, a64)
+                r0 = self.field(g, S, Gk, q, p + x0h) + ext - div_alpha(x0h, a64)
                 norm = ind.norm
             else:
                 x0, r0, norm = plain(x0h, have)
@@ -296,23 +303,40 @@ class ForceField:
         u_pol = np.sum(div_alpha(mu * mu, 2.0 * t.alpha[:, None]))
         return KE * (u_rec + u_self + u_bg + u_pol)
 
-    def energy_forces(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, g: dict, S: dict, Gk: np.ndarray) -> tuple:
-        """Return ({"elec", "vdw", "total"}, forces (N, 3)) at fixed mu (forcefield.py:2362-2430)."""
+    def energy_forces(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, g: dict, S: dict, Gk: np.ndarray,
+                      F0=None, offset=None) -> tuple:
+        """Return ({"elec", "vdw", "total"}, forces (N, 3)) at fixed mu (forcefield.py:2362-2430).
+
+        With a uniform external field F0 (3,) [e/nm^2] (forcefield.py:2417-2430, _ext_terms :2075-2088,
+        field_dipole :2090-2097): the energy gains "field" = -KE F0 . M, M = sum q r + sum (p + mu)
+        + offset (molecules whole), the total includes it, dE/dd of every atom gains -KE F0 (the
+        torque on the covalent dipoles) and every atom the force KE q F0; also returns M."""
         t = self.top
         p = self.perm_dipoles(pos, H)
         d = p + mu
         se, sl, g_el, g_lj = self.pair_terms(g, t.q, d)
         dEdd = KE * self.row_field(g, t.q, d)
+        if F0 is not None:
+            dEdd = dEdd - KE * np.asarray(F0, float)[None, :]
         rec_dd, rec_dr = self.pme.gradients(S, Gk, t.q, d)
         gd_np = KE * (rec_dd - self.c_self * d)
         gpos_np = KE * rec_dr
         forces = -(KE * g_el + g_lj + gpos_np + self.perm_dipoles_vjp(pos, H, dEdd + gd_np))
         e_elec = KE * se + self.nonpair(S, Gk, d, mu, H)
         e_lj = sl + self.vdw_tail(H)
-        return {"elec": e_elec, "vdw": e_lj, "total": e_elec + e_lj}, forces
+        if F0 is None:
+            return {"elec": e_elec, "vdw": e_lj, "total": e_elec + e_lj}, forces
+        F0 = np.asarray(F0, float)
+        M = np.sum(t.q[:, None] * pos + d, axis=0) + (np.zeros(3) if offset is None else np.asarray(offset, float))
+        e_f = -KE * float(np.dot(F0, M))
+        forces = forces + KE * t.q[:, None] * F0[None, :]
+        return {"elec": e_elec, "vdw": e_lj, "field": e_f, "total": e_elec + e_lj + e_f}, forces, M
 
-    def compute(self, pos: np.ndarray, H: np.ndarray, ind: InductionState | None = None) -> Result:
-        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560)."""
+    def compute(self, pos: np.ndarray, H: np.ndarray, ind: InductionState | None = None, efield=None) -> Result:
+        """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560).
+
+        efield = (E (3,) [V/nm], offset (3,) [e nm] or None): a uniform static external field (the
+        constant-field kind of forcefield.py:2048-2073, md/efield.py); Result.dipole = M."""
         pos = np.asarray(pos, float)
         H = np.asarray(H, float)
         ind = self.init_induction() if ind is None else ind
@@ -320,9 +344,13 @@ class ForceField:
         p = self.perm_dipoles(pos, H)
         S = self.pme.setup(pos, H)
         Gk = self.pme.influence(H)
-        mu, it, err, ind = self.solve(g, S, Gk, p, ind)
-        energy, forces = self.energy_forces(pos, H, mu, g, S, Gk)
-        return Result(energy, forces, mu, it, err, ind)
+        F0 = None if efield is None else VNM_TO_INTERNAL * np.asarray(efield[0], float)
+        mu, it, err, ind = self.solve(g, S, Gk, p, ind, F0)
+        if F0 is None:
+            energy, forces = self.energy_forces(pos, H, mu, g, S, Gk)
+            return Result(energy, forces, mu, it, err, ind)
+        energy, forces, M = self.energy_forces(pos, H, mu, g, S, Gk, F0, efield[1])
+        return Result(energy, forces, mu, it, err, ind, M)
 
     def strain_derivative(self, pos: np.ndarray, H: np.ndarray, mu: np.ndarray, molecular: bool = True) -> np.ndarray:
         """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, minus the tail impulse E_lrc I.

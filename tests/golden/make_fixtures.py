@@ -11,8 +11,9 @@ Every output is data (inputs and expected outputs), never reference source:
   read from the POL_GAUSS_* / LENNARD_JONES_* / MASS sections of the prmtop (converted to nm, e,
   e nm, nm^3, R*/2 nm, sqrt(kJ/mol) the way the reference's ``param.read_prmtop_pgm``
   (pgm_jax/param.py:113-233, :281-307) defines them).
-* ``ff_small_box.npz``, ``ff_pgm3p25_512.npz``, ``periodic_model.npz`` -- copies of the golden
-  outputs of the reference's regression harness (tests/regression/golden/), recorded with jax 0.11.2.
+* ``ff_small_box.npz``, ``ff_pgm3p25_512.npz``, ``periodic_model.npz``, ``efield_point.npz`` -- copies
+  of the golden outputs of the reference's regression harness (tests/regression/golden/), recorded
+  with jax 0.11.2.
 * ``pgm3p25_512.prmtop``, ``pgm3p25_512.rst7`` -- byte copies of the two input files of
   ``examples/water512`` (the reference's install check, tests/test_water512.py), read by the
   product's own Amber readers in the tests of the MD runner.
@@ -156,7 +157,7 @@ def main(ref: str) -> None:
     copy_water512_inputs(ref)
     np.savez_compressed(os.path.join(HERE, "water512.npz"), **water512(ref))
     gold = os.path.join(ref, "tests/regression/golden")
-    for name in ("ff_small_box", "ff_pgm3p25_512", "periodic_model"):
+    for name in ("ff_small_box", "ff_pgm3p25_512", "periodic_model", "efield_point"):
         with np.load(os.path.join(gold, name + ".npz"), allow_pickle=False) as z:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **{k: z[k] for k in z.files})
     np.savez_compressed(os.path.join(HERE, "md_rigid_points.npz"), **md_rigid_points(gold))
