@@ -121,6 +121,21 @@ long long pgm_row_blocks(pgm_ctx *ctx);
 int pgm_set_params(pgm_ctx *ctx, const double *q, const double *radius, const double *alpha, const double *lj_rmin_half,
                    const double *lj_sqrt_eps, const double *cov_c);
 
+/* A uniform static external field for the next evaluations: the constant-field kind of the
+ * efield=(E, offset) argument of PGMForceField.compute / energy / energy_fixed_mu
+ * (forcefield.py:2048-2097, :2489-2560; md/efield.py:372-404).  E (3, V/nm, host) joins the
+ * permanent field of every atom in the induction equations (as F0 = E FARADAY / KE), every charge
+ * feels KE q F0 and every covalent dipole its torque, and the energy gains
+ * field = -E . M FARADAY with M = sum q r + sum (p + mu) + offset [e nm] (offset: 3 host doubles,
+ * NULL: zero); pgm_energy.total includes it.  E NULL: no field (the field-free kernels' results,
+ * bit for bit).  Strain derivatives and the VJPs refuse a context with a field (PGM_ERR_STATE). */
+int pgm_set_efield(pgm_ctx *ctx, const double *E_vnm, const double *offset);
+
+/* The field's part of the last pgm_compute / pgm_trial_energy / pgm_energy_forces: e_field
+ * (nbead, kJ/mol; Result.energy["field"]) and the total dipole M (nbead x 3, e nm; Result.dipole),
+ * host arrays.  PGM_ERR_ARG without a field.  Synchronises. */
+int pgm_field_result(pgm_ctx *ctx, double *e_field, double *dipole);
+
 /* PGMForceField.perm_dipoles (forcefield.py:1032-1042): p (N,3) f64 device. */
 int pgm_perm_dipoles(pgm_ctx *ctx, const double *pos, double *p);
 

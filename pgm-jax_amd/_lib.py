@@ -46,6 +46,8 @@ EXPORTS = (
     "pgm_dipole_vjp",
     "pgm_compute_vjp",
     "pgm_set_params",
+    "pgm_set_efield",
+    "pgm_field_result",
     "pgm_solve_linear_induction",
     "pgm_solve_linear_induction_noalloc",
     "pgm_md_setup",
@@ -202,6 +204,8 @@ def lib() -> C.CDLL:
     L.pgm_dipole_vjp.argtypes = [P, P, P, P, D, P, P, P, P, P, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_compute_vjp.argtypes = [P, P, P, D, P, P, D, P, P, P, P, P, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_set_params.argtypes = [P, P, P, P, P, P, P]
+    L.pgm_set_efield.argtypes = [P, P, P]
+    L.pgm_field_result.argtypes = [P, P, P]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_solve_linear_induction_noalloc.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_md_setup.argtypes = [P, C.POINTER(D), P]

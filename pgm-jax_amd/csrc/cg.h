@@ -18,6 +18,11 @@
 
 namespace pgm {
 
+// A uniform external field in internal units (e/nm^2; zero when off), or KE times it, by value.
+struct Ext3 {
+    double x, y, z;
+};
+
 struct CGState {
     int conv_it;  // first iteration index that must not run (INT_MAX while unconverged)
     unsigned seq; // sequence number of the solve (set by k_cg_start_finish; stamps the host mirror)
@@ -670,12 +675,18 @@ __global__ void k_forces_final(int n, const double *__restrict__ pos, Box box, c
                                const double *__restrict__ g_lj, const double *__restrict__ rec_dr,
                                const double *__restrict__ gp, const int *__restrict__ cstart, const int *__restrict__ ci,
                                const int *__restrict__ cj, const double *__restrict__ cc, const int *__restrict__ jstart,
-                               const int *__restrict__ jlist, double *__restrict__ forces) {
+                               const int *__restrict__ jlist, double *__restrict__ forces, const double *__restrict__ q = nullptr,
+                               Ext3 kf = {}) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     double v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = KE * g_el[3 * a + c] + g_lj[3 * a + c] + KE * rec_dr[3 * a + c];
+    if (q) {  // a uniform external field pulls every charge: force KE q F0 (kf = KE F0)
+        v[0] -= q[a] * kf.x;
+        v[1] -= q[a] * kf.y;
+        v[2] -= q[a] * kf.z;
+    }
     for (int pass = 0; pass < 2; ++pass) {
         const int m0 = pass == 0 ? cstart[a] : jstart[a];
         const int m1 = pass == 0 ? cstart[a + 1] : jstart[a + 1];
@@ -704,10 +715,15 @@ __global__ void k_forces_final(int n, const double *__restrict__ pos, Box box, c
 // Energy assembly (one block per bead, out[4 b .. 4 b + 3] = elec, vdw, total, u_rec): parts =
 // {pair rows (e_el, e_lj) x npair, (sum|d|^2, u_pol) x npack, u_rec x nconv} per bead; consts =
 // {KE * (u_self_q + u_bg), e_tail, c_self}.
+//
+// With a uniform external field (mpart not null): the field's dipole M = sum of the nm partials of
+// k_field_dipole per bead + offset, its energy -KE F0 . M (kf = KE F0) added to the total, and
+// (e_field, M) in fout[4 b .. 4 b + 3] (forcefield.py:2075-2097).
 __global__ void k_energy_final(const double *__restrict__ pair_part, int npair, const double *__restrict__ pack_part, int npack,
                                const double *__restrict__ conv_part, int nconv, double ke_const, double tail, double cself,
-                               double *__restrict__ out) {
-    __shared__ double lds[5 * 16];
+                               double *__restrict__ out, const double *__restrict__ mpart = nullptr, int nm = 0,
+                               Ext3 kf = {}, Ext3 off = {}, double *__restrict__ fout = nullptr) {
+    __shared__ double lds[8 * 16];
     const int b = blockIdx.x;
     const double2 *pp = reinterpret_cast<const double2 *>(pair_part) + (size_t)b * npair;
     const double2 *pk = reinterpret_cast<const double2 *>(pack_part) + (size_t)b * npack;
@@ -728,16 +744,49 @@ __global__ void k_energy_final(const double *__restrict__ pair_part, int npair, 
         v[2] += x.x, v[3] += x.y;
     }
     for (int k = threadIdx.x; k < nconv; k += nt) v[4] += conv_part[k];
-    block_sum<5>(v, lds);
-    const double e_el = v[0], e_lj = v[1], d2 = v[2], upol = v[3], urec = v[4];
+    double m[3] = {0.0, 0.0, 0.0};
+    if (mpart)
+        for (int k = threadIdx.x; k < nm; k += nt)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) m[c] += mpart[3 * ((size_t)b * nm + k) + c];
+    double w[8] = {v[0], v[1], v[2], v[3], v[4], m[0], m[1], m[2]};
+    block_sum<8>(w, lds);
+    const double e_el = w[0], e_lj = w[1], d2 = w[2], upol = w[3], urec = w[4];
     if (threadIdx.x == 0) {
         const double elec = KE * 0.5 * e_el + KE * (urec - 0.5 * cself * d2 + upol) + ke_const;
         const double vdw = 0.5 * e_lj + tail;
+        double efield = 0.0;
+        if (mpart) {
+            const double M[3] = {w[5] + off.x, w[6] + off.y, w[7] + off.z};
+            efield = -(kf.x * M[0] + kf.y * M[1] + kf.z * M[2]);
+            fout[4 * b] = efield;
+            fout[4 * b + 1] = M[0];
+            fout[4 * b + 2] = M[1];
+            fout[4 * b + 3] = M[2];
+        }
         out[0] = elec;
         out[1] = vdw;
-        out[2] = elec + vdw;
+        out[2] = elec + vdw + efield;
         out[3] = urec;
     }
+}
+
+// Per-block partials (grid: blocks per bead, beads) of the dipole a uniform external field acts on,
+// sum q r + sum d with d = p + mu (FP64; molecules whole, forcefield.py:2090-2097).
+__global__ __launch_bounds__(256) void k_field_dipole(int nper, const double *__restrict__ pos, const double *__restrict__ q,
+                                                      const double *__restrict__ dd, double *__restrict__ part) {
+    __shared__ double lds[3 * 16];
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * nper + li;
+    const int pb = blockIdx.y * gridDim.x + blockIdx.x;
+    double m[3] = {0.0, 0.0, 0.0};
+    if (li < nper)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m[c] = q[i] * pos[3 * i + c] + dd[3 * i + c];
+    block_sum<3>(m, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) part[3 * pb + c] = m[c];
 }
 
 }  // namespace pgm

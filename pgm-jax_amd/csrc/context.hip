@@ -201,6 +201,11 @@ struct pgm_ctx {
     unsigned long long *h_mirror = nullptr;  // CG states written by the update kernels (cg_mirror), 4 slots of nbead,
                                              // then the MD step's largest displacement (a double, h_disp)
     unsigned cg_seq = 0;                     // sequence number of the current CG solve (the mirror's stamp)
+    // uniform static external field (pgm_set_efield): F0 in e/nm^2, the dipole offset in e nm; the
+    // field's dipole partials and (e_field, M) per bead of the last evaluation
+    bool efield_on = false;
+    Ext3 F0{}, foff{};
+    DevBuf fpart, fout;
     double *h_pin_dbl = nullptr;   // 32 + 5 nbead doubles (bead energies from 32, residuals after them)
     int last_iters = 4;
     int last_adj_iters = 4;  // iterations of the last adjoint solve (read-back hint)
@@ -1052,7 +1057,8 @@ static int gather_combine(pgm_ctx *c, const real4<real> *dq, int mode, const rea
 
 // The CG row field of dq into vrow on stream st, in the context's layout.
 template <typename real, bool WITHQ>
-static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, int iter, hipStream_t st, TimedPair tp = {}) {
+static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, int iter, hipStream_t st, TimedPair tp = {},
+                            Ext3 ext = {}) {
     using R4 = real4<real>;
     if (c->layout == 1) {
         const int nmol = c->n / 3, nb = c->nper / 3;
@@ -1062,22 +1068,23 @@ static int row_field_launch(pgm_ctx *c, const real4<real> *dq, const int *conv, 
         R4 *out = c->vrow.as<R4>();
         const dim3 b(256);
         switch (c->rlm) {
-            case 4: launch_timed(tp, k_row_field_mol<real, WITHQ, 4>, dim3(nblk((long long)nmol * 4, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
-            case 8: launch_timed(tp, k_row_field_mol<real, WITHQ, 8>, dim3(nblk((long long)nmol * 8, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
-            default: launch_timed(tp, k_row_field_mol<real, WITHQ, 16>, dim3(nblk((long long)nmol * 16, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter); break;
+            case 4: launch_timed(tp, k_row_field_mol<real, WITHQ, 4>, dim3(nblk((long long)nmol * 4, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter, ext); break;
+            case 8: launch_timed(tp, k_row_field_mol<real, WITHQ, 8>, dim3(nblk((long long)nmol * 8, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter, ext); break;
+            default: launch_timed(tp, k_row_field_mol<real, WITHQ, 16>, dim3(nblk((long long)nmol * 16, 256)), b, 0, st, nmol, nb, mj, mg, mc, xr, dq, c->box, out, conv, iter, ext); break;
         }
     } else {
         launch_timed(tp, k_row_field<real, WITHQ>, dim3(nblk((long long)c->n * RL, 256)), dim3(256), 0, st, c->n, c->nper,
                      c->jell.as<int>(), c->count.as<int>(), c->G1.as<real>(), c->G2.as<real>(), c->xr.as<double4>(), dq, c->box,
-                     c->vrow.as<R4>(), conv, iter);
+                     c->vrow.as<R4>(), conv, iter, ext);
     }
     LAUNCH_CHECK();
     return PGM_OK;
 }
 
+// ext: a uniform external field added to the field of every atom (evaluations with charges only).
 template <typename real>
 static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, const real4<real> *extra, real4<real> *out,
-                      double *part, int iter) {
+                      double *part, int iter, Ext3 ext = {}) {
     const int n = c->n;
     hipStream_t st = c->stream;
     const int *conv = iter >= 0 ? &c->st.as<CGState>()->conv_it : nullptr;
@@ -1091,14 +1098,14 @@ static int field_eval(pgm_ctx *c, const real4<real> *dq, bool withq, int mode, c
                 HIP_TRY(hipStreamWaitEvent(st, c->ev_rec, 0));
                 c->rec_pending = false;
             }
-            if (withq) return row_field_launch<real, true>(c, dq, conv, iter, st, timing_pair(c, 0));
+            if (withq) return row_field_launch<real, true>(c, dq, conv, iter, st, timing_pair(c, 0), ext);
             return row_field_launch<real, false>(c, dq, conv, iter, st, timing_pair(c, 0));
         }
         HIP_TRY(hipEventRecord(c->ev_fork, st));
         HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         const TimedPair tp = timing_pair(c, 0);
         if (withq)
-            PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side, tp)));
+            PGM_TRY((row_field_launch<real, true>(c, dq, conv, iter, c->side, tp, ext)));
         else
             PGM_TRY((row_field_launch<real, false>(c, dq, conv, iter, c->side, tp)));
         HIP_TRY(hipEventRecord(c->ev_join, c->side));
@@ -1584,12 +1591,12 @@ static int solve_impl(pgm_ctx *c, pgm_induction *ind, int *iterations, double *r
     if (fused) {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), x0, q, dq);
         LAUNCH_CHECK();
-        PGM_TRY(field_eval<real>(c, dq, true, MODE_FUSED, x0, r, nullptr, -1));
+        PGM_TRY(field_eval<real>(c, dq, true, MODE_FUSED, x0, r, nullptr, -1, c->F0));
         if (B == 1 && (!quiet || !c->norm_on_device)) k_set_norm_value<<<1, 64, 0, st>>>(S, ind->norm);
     } else {
         k_pack_dq<real><<<nb, 256, 0, st>>>(n, c->pperm.as<double>(), nullptr, q, dq);
         LAUNCH_CHECK();
-        PGM_TRY(field_eval<real>(c, dq, true, MODE_FIELD, nullptr, b, c->partA.as<double>(), -1));
+        PGM_TRY(field_eval<real>(c, dq, true, MODE_FIELD, nullptr, b, c->partA.as<double>(), -1, c->F0));
         k_set_norm<<<B, 256, 0, st>>>(c->partA.as<double>(), gather_parts(c), 1.0 / (3.0 * nper), S);
         if (!have) k_scale_alpha<real><<<nb, 256, 0, st>>>(n, b, c->alpha_r.as<real>(), x0);
         LAUNCH_CHECK();
@@ -1680,6 +1687,17 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
                                                             c->pack_part.as<double>());
     LAUNCH_CHECK();
     const bool vir = strain != nullptr;
+    if (vir && c->efield_on) {
+        g_err = "the strain derivative with an external field is not built";
+        return PGM_ERR_STATE;
+    }
+    const Ext3 ext = c->F0;  // zero without a field
+    const Ext3 kf{KE * c->F0.x, KE * c->F0.y, KE * c->F0.z};
+    if (c->efield_on) {  // the field's dipole sum q r + sum (p + mu), partials per bead
+        k_field_dipole<<<bead_grid(c, nper), 256, 0, st>>>(nper, c->pos, c->q.as<double>(), c->dd.as<double>(),
+                                                           c->fpart.as<double>());
+        LAUNCH_CHECK();
+    }
     // the force pass on the molecule-block records (no strain derivative; every LJ pair in a record)
     // or on the atom rows
     const bool mol_forces = c->layout == 1 && !vir && c->rc_v <= c->rc_e;
@@ -1724,12 +1742,12 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
             k_pair_forces_mol<real, MOL_FORCE_LANES><<<gpair, 256, 0, ps>>>(
                 nper, n / 3, c->rlm, c->mj.as<int>(), c->mg.as<real>(), c->mcount.as<int>(), c->xr.as<double4>(), dq, c->box,
                 (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->rh.as<double>(), c->se.as<double>(),
-                c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>());
+                c->g_el.as<double>(), c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), ext);
         else
             k_pair_forces<real, false><<<gpair, 256, 0, ps>>>(
                 nper, c->rows.as<int>(), c->count.as<int>(), c->count_near.as<int>(), c->cap, c->xr.as<double4>(), dq, c->box,
                 (real)c->s.ewald_beta, c->rc_e * c->rc_e, c->rc_v * c->rc_v, c->ljp.as<double4>(), c->g_el.as<double>(),
-                c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr);
+                c->g_lj.as<double>(), c->rowf.as<double>(), c->pair_part.as<double>(), nullptr, nullptr, ext);
         if (!c->ewald) HIP_TRY(hipEventRecord(c->ev_join, c->side));
     }
     LAUNCH_CHECK();
@@ -1776,14 +1794,16 @@ static int energy_forces_impl(pgm_ctx *c, const double *mu, pgm_energy *energy, 
     if (forces) {
         k_forces_final<<<nb, 256, 0, st>>>(n, c->pos, c->box, c->g_el.as<double>(), c->g_lj.as<double>(), c->rec_dr.as<double>(),
                                            c->gp.as<double>(), c->cstart.as<int>(), c->ci.as<int>(), c->cj.as<int>(),
-                                           c->cc.as<double>(), c->jstart.as<int>(), c->jlist.as<int>(), forces);
+                                           c->cc.as<double>(), c->jstart.as<int>(), c->jlist.as<int>(), forces,
+                                           c->efield_on ? c->q.as<double>() : nullptr, kf);
         LAUNCH_CHECK();
     }
     const double b0 = c->s.ewald_beta;
     const double ke_const = KE * (-(b0 / SQRT_PI) * c->sum_q2 - M_PI * c->sum_q * c->sum_q / (2.0 * c->volume * b0 * b0));
     const double tail = c->s.lj_lrc ? lj_tail(c) : 0.0;
     k_energy_final<<<B, 256, 0, st>>>(c->pair_part.as<double>(), npair, c->pack_part.as<double>(), npb, c->conv_part.as<double>(),
-                                      nconv, ke_const, tail, c_self(c), c->energy.as<double>());
+                                      nconv, ke_const, tail, c_self(c), c->energy.as<double>(),
+                                      c->efield_on ? c->fpart.as<double>() : nullptr, npb, kf, c->foff, c->fout.as<double>());
     LAUNCH_CHECK();
     if (quiet) return PGM_OK;  // energies stay in c->energy
     PGM_TRY(read_energy(c, energy));
@@ -2439,6 +2459,10 @@ extern "C" int pgm_dipole_vjp(pgm_ctx *c, const double *pos, const double *mu, c
         return PGM_ERR_ARG;
     }
     PGM_TRY(single_system(c, "pgm_dipole_vjp"));
+    if (c->efield_on) {
+        g_err = "pgm_dipole_vjp: the derivatives with an external field are not built";
+        return PGM_ERR_STATE;
+    }
     if (c->ewald) {
         g_err = "pgm_dipole_vjp: PME reciprocal space only (this context holds the exact Ewald sum)";
         return PGM_ERR_STATE;
@@ -2589,6 +2613,10 @@ extern "C" int pgm_compute_vjp(pgm_ctx *c, const double *pos, const double *mu, 
         return PGM_ERR_ARG;
     }
     PGM_TRY(single_system(c, "pgm_compute_vjp"));
+    if (c->efield_on) {
+        g_err = "pgm_compute_vjp: the derivatives with an external field are not built";
+        return PGM_ERR_STATE;
+    }
     if (c->ewald) {
         g_err = "pgm_compute_vjp: PME reciprocal space only (this context holds the exact Ewald sum)";
         return PGM_ERR_STATE;
@@ -2597,6 +2625,57 @@ extern "C" int pgm_compute_vjp(pgm_ctx *c, const double *pos, const double *mu, 
                                               sqrt_eps_bar, cov_bar, iterations, residual)
                     : compute_vjp_impl<double>(c, pos, mu, e_bar, f_bar, mu_bar, tol, q_bar, radius_bar, alpha_bar, rmin_half_bar,
                                                sqrt_eps_bar, cov_bar, iterations, residual);
+}
+
+// ---------------------------------------------------------------------------- external field
+// The uniform static field of forcefield.py:2048-2097 (md/efield.py, the constant-field kind): F0 =
+// E [V/nm] x FARADAY / KE in the model's e/nm^2 joins the permanent field of every atom in the
+// induction equations, the energy gains -KE F0 . M with M = sum q r + sum (p + mu) + offset, every
+// charge feels KE q F0 and every covalent dipole the torque of the field.
+constexpr double FARADAY_KJ = 96.48533212331002;  // kJ/mol per (e V)
+
+extern "C" int pgm_set_efield(pgm_ctx *c, const double *E_vnm, const double *offset) {
+    if (!c) return PGM_ERR_ARG;
+    if (c->ewald && E_vnm) {
+        g_err = "pgm_set_efield: PME reciprocal space only (this context holds the exact Ewald sum)";
+        return PGM_ERR_STATE;
+    }
+    if (!E_vnm) {
+        c->efield_on = false;
+        c->F0 = Ext3{};
+        c->foff = Ext3{};
+        return PGM_OK;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(E_vnm[k]) || (offset && !std::isfinite(offset[k]))) {
+            g_err = "pgm_set_efield: non-finite field or offset";
+            return PGM_ERR_ARG;
+        }
+    const double s = FARADAY_KJ / KE;
+    PGM_TRY(c->fpart.ensure(sizeof(double) * 3 * (size_t)c->nbead * nblk(c->nper, 256)));
+    PGM_TRY(c->fout.ensure(sizeof(double) * 4 * (size_t)c->nbead));
+    HIP_TRY(hipMemsetAsync(c->fout.p, 0, sizeof(double) * 4 * (size_t)c->nbead, c->stream));
+    c->F0 = Ext3{s * E_vnm[0], s * E_vnm[1], s * E_vnm[2]};
+    c->foff = offset ? Ext3{offset[0], offset[1], offset[2]} : Ext3{};
+    c->efield_on = true;
+    return PGM_OK;
+}
+
+extern "C" int pgm_field_result(pgm_ctx *c, double *e_field, double *dipole) {
+    if (!c || !e_field || !dipole) return PGM_ERR_ARG;
+    if (!c->efield_on) {
+        g_err = "pgm_field_result: no external field is set";
+        return PGM_ERR_ARG;
+    }
+    const int B = c->nbead;
+    std::vector<double> h(4 * (size_t)B);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->fout.p, sizeof(double) * 4 * B, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b) {
+        e_field[b] = h[4 * b];
+        for (int k = 0; k < 3; ++k) dipole[3 * b + k] = h[4 * b + 1 + k];
+    }
+    return PGM_OK;
 }
 
 // ---------------------------------------------------------------------------- parameters

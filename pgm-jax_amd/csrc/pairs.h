@@ -349,7 +349,8 @@ template <typename real, bool WITHQ>
 __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *__restrict__ jell, const int *__restrict__ count,
                                                    const real *__restrict__ G1, const real *__restrict__ G2,
                                                    const double4 *__restrict__ xr, const real4<real> *__restrict__ dq, Box box,
-                                                   real4<real> *__restrict__ out, const int *__restrict__ conv_it, int iter) {
+                                                   real4<real> *__restrict__ out, const int *__restrict__ conv_it, int iter,
+                                                   Ext3 ext = {}) {
     const int lane = threadIdx.x % RL;
     const int i = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
     if (i >= n || cg_done(conv_it, i / nper, iter)) return;
@@ -392,6 +393,11 @@ __global__ __launch_bounds__(256) void k_row_field(int n, int nper, const int *_
         v.y = ey;
         v.z = ez;
         v.w = 0;
+        if (WITHQ) {  // a uniform external field: its dU/dd is -ext
+            v.x -= (real)ext.x;
+            v.y -= (real)ext.y;
+            v.z -= (real)ext.z;
+        }
         out[i] = v;
     }
 }
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__rest
                                                      double *__restrict__ g_el,
                                                      double *__restrict__ g_lj, double *__restrict__ rowf,
                                                      double *__restrict__ epart, const double *__restrict__ off,
-                                                     double *__restrict__ wpart) {
+                                                     double *__restrict__ wpart, Ext3 ext = {}) {
     __shared__ double lds[9 * 16];
     constexpr int RL_ = rowl<real>();
     const int lane = threadIdx.x % RL_;
@@ -523,9 +529,9 @@ __global__ __launch_bounds__(256) void k_pair_forces(int nper, const int *__rest
             g_lj[3 * i] = lx;
             g_lj[3 * i + 1] = ly;
             g_lj[3 * i + 2] = lz;
-            rowf[3 * i] = fx;
-            rowf[3 * i + 1] = fy;
-            rowf[3 * i + 2] = fz;
+            rowf[3 * i] = fx - ext.x;  // a uniform external field: its dU/dd is -ext
+            rowf[3 * i + 1] = fy - ext.y;
+            rowf[3 * i + 2] = fz - ext.z;
         }
         esum[0] = ee;
         esum[1] = el;
@@ -813,7 +819,7 @@ __global__ __launch_bounds__(256) void k_row_field_mol(int nmol, int nmol_bead, 
                                                        const real *__restrict__ mg, const int *__restrict__ mcount,
                                                        const double4 *__restrict__ xr, const real4<real> *__restrict__ dq,
                                                        Box box, real4<real> *__restrict__ out, const int *__restrict__ conv_it,
-                                                       int iter) {
+                                                       int iter, Ext3 ext = {}) {
     const int lane = threadIdx.x % RLM;
     const int I = blockIdx.x * (blockDim.x / RLM) + threadIdx.x / RLM;
     if (I >= nmol || cg_done(conv_it, I / nmol_bead, iter)) return;
@@ -881,6 +887,11 @@ __global__ __launch_bounds__(256) void k_row_field_mol(int nmol, int nmol_bead, 
             v.y = acc[3 * p + 1];
             v.z = acc[3 * p + 2];
             v.w = 0;
+            if (WITHQ) {  // a uniform external field: its dU/dd is -ext
+                v.x -= (real)ext.x;
+                v.y -= (real)ext.y;
+                v.z -= (real)ext.z;
+            }
             out[3 * I + p] = v;
         }
 }
@@ -930,7 +941,7 @@ __global__ __launch_bounds__(256, sizeof(real) == 4 ? 4 : 1) void k_pair_forces_
                                                          Box box, real beta, double rce2, double rcv2,
                                                          const double *__restrict__ rh, const double *__restrict__ se,
                                                          double *__restrict__ g_el, double *__restrict__ g_lj,
-                                                         double *__restrict__ rowf, double *__restrict__ epart) {
+                                                         double *__restrict__ rowf, double *__restrict__ epart, Ext3 ext = {}) {
     __shared__ double lds[2 * 16];
     __shared__ int nq[MOL_NEAR_QUEUE * 256];  // (record index << 2 | k) of the lane's series pairs
     const int lane = threadIdx.x % FL;
@@ -1064,9 +1075,9 @@ __global__ __launch_bounds__(256, sizeof(real) == 4 ? 4 : 1) void k_pair_forces_
             g_lj[3 * i] = lx;
             g_lj[3 * i + 1] = ly;
             g_lj[3 * i + 2] = lz;
-            rowf[3 * i] = fx;
-            rowf[3 * i + 1] = fy;
-            rowf[3 * i + 2] = fz;
+            rowf[3 * i] = fx - ext.x;  // a uniform external field: its dU/dd is -ext
+            rowf[3 * i + 1] = fy - ext.y;
+            rowf[3 * i + 2] = fz - ext.z;
         }
         esum[0] = ee;
         esum[1] = el;

@@ -58,7 +58,8 @@ class InductionState:
 
 @dataclasses.dataclass
 class Result:
-    """Result of compute (forcefield.py:716-746): energy {"elec","vdw","total"} [kJ/mol], forces (N,3) [kJ/mol/nm]."""
+    """Result of compute (forcefield.py:716-746): energy {"elec","vdw","total"} (and "field" with an
+    external field) [kJ/mol], forces (N,3) [kJ/mol/nm]."""
 
     energy: dict
     forces: object
@@ -66,6 +67,7 @@ class Result:
     iterations: int
     residual: float
     overflow: bool = False
+    dipole: object = None  # with an external field: M = sum q r + sum (p + mu) + offset (3,) [e nm]
 
 
 def _make_context(sys: System, H: np.ndarray, settings: MDSettings, K: tuple, nbead: int = 1):
@@ -175,15 +177,48 @@ class PGMForceField:
         if disp > 0.5 * self.s.skin:
             self.build_rows(pos)
 
+    def _efield(self, efield) -> bool:
+        """Put the external field of an efield argument on the context (pgm_set_efield): None, or
+        (E [V/nm] (3,), offset [e nm] (3,) or None[, "E"]), the constant-field kind of
+        PGMForceField._ext (forcefield.py:2048-2073).  Returns whether a field is on."""
+        if efield is None:
+            key = None
+        else:
+            kind = efield[2] if len(efield) > 2 else "E"
+            if kind == "D":
+                raise NotImplementedError("the constant-displacement field (efield kind 'D') is not built")
+            if kind != "E":
+                raise ValueError(f"field kind {kind!r}: 'E' (constant field) or 'D' (constant displacement)")
+            E = np.ascontiguousarray(np.asarray(efield[0], float).reshape(3))
+            off = None if efield[1] is None else np.ascontiguousarray(np.asarray(efield[1], float).reshape(3))
+            key = (tuple(E), None if off is None else tuple(off))
+        if key != getattr(self, "_efield_now", None):
+            if key is None:
+                _lib.check(_lib.lib().pgm_set_efield(self._ctx, None, None), "pgm_set_efield")
+            else:
+                _lib.check(_lib.lib().pgm_set_efield(self._ctx, _ptr(E, C.c_double),
+                                                     None if off is None else _ptr(off, C.c_double)), "pgm_set_efield")
+            self._efield_now = key
+        return key is not None
+
+    def _field_result(self) -> tuple[float, np.ndarray]:
+        """(energy["field"] [kJ/mol], M (3,) [e nm]) of the last evaluation (pgm_field_result)."""
+        e, M = (C.c_double * 1)(), (C.c_double * 3)()
+        _lib.check(_lib.lib().pgm_field_result(self._ctx, e, M), "pgm_field_result")
+        return e[0], np.array(list(M), float)
+
     # ------------------------------------------------------------------ entry points
-    def compute(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None) -> Result:
+    def compute(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None, efield=None) -> Result:
         """Solve the induced dipoles from the predictor, then return energy and forces (forcefield.py:2489-2560).
 
         params: a parameter pytree {quantity: (K,) array} of the system's tied table (System.expand;
         None: the parameters in use), set on the device before the evaluation (set_params).  Its
-        gradient for a loss of the outputs is grad_params."""
+        gradient for a loss of the outputs is grad_params.  efield: None or (E [V/nm], offset [e nm]
+        or None), a uniform static field (md/efield.py): energy["field"] = -E . M and Result.dipole =
+        M; molecules whole in pos."""
         if params is not None:
             self.set_params(params)
+        on = self._efield(efield)
         pos = _dev(pos, (self.n, 3))
         self._box(H)
         self._ensure_rows(pos)
@@ -198,14 +233,19 @@ class PGMForceField:
                                                C.c_void_p(forces.data_ptr()), C.byref(it), C.byref(res)), "pgm_compute")
         new = InductionState(ind.mu, ind.hist, cind.count, cind.norm)
         energy = {"elec": e.elec, "vdw": e.vdw, "total": e.total}
-        return Result(energy, forces, new, it.value, res.value, st == _lib.PGM_ROW_OVERFLOW)
+        M = None
+        if on:
+            energy["field"], M = self._field_result()
+        return Result(energy, forces, new, it.value, res.value, st == _lib.PGM_ROW_OVERFLOW, M)
 
-    def energy(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None) -> tuple:
+    def energy(self, pos, H=None, idx=None, ind: InductionState | None = None, params=None, efield=None) -> tuple:
         """Return (total energy [kJ/mol], state, CG iterations, overflow) for Monte Carlo barostat trials
         (forcefield.py:2562-2626): the dipoles are solved from ind.mu (no predictor) into a new mu;
-        the state's history, count and norm are those passed in."""
+        the state's history, count and norm are those passed in.  efield as in compute (the total
+        includes the field's energy)."""
         if params is not None:
             self.set_params(params)
+        self._efield(efield)
         pos = _dev(pos, (self.n, 3))
         self._box(H)
         self._ensure_rows(pos)
@@ -219,9 +259,11 @@ class PGMForceField:
                                               C.byref(res)), "pgm_trial_energy")
         return e.total, InductionState(mu, ind.hist, ind.count, ind.norm), it.value, st == _lib.PGM_ROW_OVERFLOW
 
-    def energy_forces(self, pos, mu, H=None) -> tuple[dict, object]:
-        """Return (energy parts, forces) at fixed induced dipoles mu (forcefield.py:2249-2292, :2362-2430)."""
+    def energy_forces(self, pos, mu, H=None, efield=None) -> tuple[dict, object]:
+        """Return (energy parts, forces) at fixed induced dipoles mu (forcefield.py:2249-2292, :2362-2430);
+        efield as in compute (energy["field"])."""
         torch = _torch()
+        on = self._efield(efield)
         pos = _dev(pos, (self.n, 3))
         mu = _dev(mu, (self.n, 3))
         self._box(H)
@@ -230,20 +272,26 @@ class PGMForceField:
         e = _lib.Energy()
         _lib.check(_lib.lib().pgm_energy_forces(self._ctx, C.c_void_p(pos.data_ptr()), C.c_void_p(mu.data_ptr()), C.byref(e),
                                                 C.c_void_p(forces.data_ptr())), "pgm_energy_forces")
-        return {"elec": e.elec, "vdw": e.vdw, "total": e.total}, forces
+        energy = {"elec": e.elec, "vdw": e.vdw, "total": e.total}
+        if on:
+            energy["field"] = self._field_result()[0]
+        return energy, forces
 
-    def energy_fixed_mu(self, pos, H, mu) -> float:
+    def energy_fixed_mu(self, pos, H, mu, efield=None) -> float:
         """Return the total energy [kJ/mol] with the induced dipoles held at mu (forcefield.py:2249-2292)."""
-        return self.energy_forces(pos, mu, H)[0]["total"]
+        return self.energy_forces(pos, mu, H, efield)[0]["total"]
 
-    def strain_derivative(self, pos, H, mu, molecular: bool = True):
+    def strain_derivative(self, pos, H, mu, molecular: bool = True, efield=None):
         """Return dE/d eps (3, 3) [kJ/mol] at fixed mu, plus the tail impulse -E_lrc I with lj_lrc
         (forcefield.py:2628-2695), as a float64 numpy array.
 
         molecular=True: molecules are translated with their centres of mass (the system's masses;
         molecules whole in `pos`).  molecular=False: atomic scaling, every atom r -> (1 + eps) r (the
-        covalent dipole frames strain with them).
+        covalent dipole frames strain with them).  An external field (efield) is not built here.
         """
+        if efield is not None:
+            raise NotImplementedError("strain_derivative with an external field is not built")
+        self._efield(None)
         torch = _torch()
         pos = _dev(pos, (self.n, 3))
         mu = _dev(mu, (self.n, 3))
@@ -301,6 +349,7 @@ class PGMForceField:
         self._box(H)
         self._ensure_rows(pos)
         tol = float(self.s.adjoint_tol if tol is None else tol)
+        self._efield(None)  # the derivatives are those of the field-free force field
         f64 = dict(dtype=torch.float64, device="cuda")
         out = {"lam": torch.empty((self.n, 3), **f64), "q": torch.empty(self.n, **f64), "radius": torch.empty(self.n, **f64),
                "alpha": torch.empty(self.n, **f64), "p": torch.empty((self.n, 3), **f64),
@@ -352,6 +401,7 @@ class PGMForceField:
         self._box(H)
         self._ensure_rows(pos)
         tol = float(self.s.adjoint_tol if tol is None else tol)
+        self._efield(None)  # the derivatives are those of the field-free force field
         f64 = dict(dtype=torch.float64, device="cuda")
         keys = ("q", "radius", "alpha", "lj_rmin_half", "lj_sqrt_eps")
         out = {k: torch.empty(self.n, **f64) for k in keys}

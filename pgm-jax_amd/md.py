@@ -37,13 +37,15 @@ class RigidWaterMD:
     """MD of consecutive 3-site rigid molecules (constraint lengths from the initial geometry).
 
     NVE without a thermostat; with `thermostat=Langevin(gamma)` and `temperature` [K], the g-BAOAB
-    step (noise from Philox4x32-10 keyed by `seed`).
+    step (noise from Philox4x32-10 keyed by `seed`).  efield: None or (E [V/nm], offset [e nm] or
+    None), a uniform static field on every step (the constant-field kind of md/efield.py).
     """
 
     def __init__(self, ff: PGMForceField, pos, vel, masses=None, thermostat: Langevin | None = None,
-                 temperature: float = 298.0, seed: int = 0) -> None:
+                 temperature: float = 298.0, seed: int = 0, efield=None) -> None:
         torch = _torch()
         self.ff = ff
+        self.efield = efield
         n = ff.n
         self.pos = _dev(pos, (n, 3)).clone()
         self.vel = _dev(vel, (n, 3)).clone()
@@ -71,6 +73,7 @@ class RigidWaterMD:
     def run(self, nsteps: int, dt_fs: float = 1.0) -> dict:
         """Advance nsteps of dt_fs femtoseconds; return the report of the last step (energies in kJ/mol)."""
         rep = _lib.MDReport()
+        self.ff._efield(self.efield)
         st = _lib.check(_lib.lib().pgm_md_run(self.ff._ctx, C.byref(self._state), int(nsteps), dt_fs * 1e-3, C.byref(rep)),
                         "pgm_md_run")
         self.cg_total += rep.cg_iterations_total
