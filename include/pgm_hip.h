@@ -148,6 +148,16 @@ int pgm_field(pgm_ctx *ctx, const double *pos, int with_charges, const double *d
 int pgm_pme_energy(pgm_ctx *ctx, const double *pos, int with_charges, const double *d, double *e_rec,
                    double *grad_d);
 
+/* PME.spread (pgm_jax/md/pme.py:306-337): the grid Q (K1, K2, K3) f64 device of the topology
+ * charges times q_scale (with_charges 0 or 1) and the dipoles d (N,3) f64 device at pos, in the
+ * working precision (FP32 mixed) converted to f64.  One system per context, PME reciprocal space. */
+int pgm_pme_spread(pgm_ctx *ctx, const double *pos, int with_charges, const double *d, double *Q);
+
+/* PME.influence (pgm_jax/md/pme.py:243-267) of the context's box: G (K1, K2, K3/2 + 1) f64 host,
+ * exp(-pi^2 |m|^2 / beta^2) / (2 pi V |m|^2) times the spline moduli and the half-space weights
+ * (1 on the planes m3 = 0 and m3 = K3/2, 2 elsewhere), 0 at m = 0 [1/nm]. */
+int pgm_pme_influence(pgm_ctx *ctx, double *G);
+
 /* PGMForceField.compute (forcefield.py:2489-2560): rows compaction, permanent dipoles, PME setup,
  * predictor + CG induced dipoles (updates ind), energies and forces (N,3) f64 device.
  * iterations and residual as Result.iterations / Result.residual. */

@@ -17,7 +17,8 @@ __all__ = ["MDSettings", "elec_cutoff_settings", "ewald_beta_for", "grid_size", 
 from .md import Langevin, RigidWaterMD  # noqa: E402
 from .periodic import PeriodicModel, PeriodicPGM, pressure_bar  # noqa: E402
 from .beads import PGMBeads, PGMBeadState  # noqa: E402
+from .pme import PME  # noqa: E402
 from . import amber, driver, models, solver  # noqa: E402
 
-__all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "PGMBeads", "PGMBeadState",
+__all__ += ["Langevin", "RigidWaterMD", "PeriodicModel", "PeriodicPGM", "pressure_bar", "PGMBeads", "PGMBeadState", "PME",
             "amber", "driver", "models", "solver"]

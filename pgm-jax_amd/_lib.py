@@ -47,6 +47,8 @@ EXPORTS = (
     "pgm_compute_vjp",
     "pgm_set_params",
     "pgm_set_efield",
+    "pgm_pme_spread",
+    "pgm_pme_influence",
     "pgm_field_result",
     "pgm_solve_linear_induction",
     "pgm_solve_linear_induction_noalloc",
@@ -205,6 +207,8 @@ def lib() -> C.CDLL:
     L.pgm_compute_vjp.argtypes = [P, P, P, D, P, P, D, P, P, P, P, P, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_set_params.argtypes = [P, P, P, P, P, P, P]
     L.pgm_set_efield.argtypes = [P, P, P]
+    L.pgm_pme_spread.argtypes = [P, P, I, P, P]
+    L.pgm_pme_influence.argtypes = [P, P]
     L.pgm_field_result.argtypes = [P, P, P]
     L.pgm_solve_linear_induction.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]
     L.pgm_solve_linear_induction_noalloc.argtypes = [I, P, P, P, P, D, I, P, C.POINTER(I), C.POINTER(D)]

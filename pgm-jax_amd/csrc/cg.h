@@ -142,6 +142,12 @@ __global__ void k_scale_alpha(int n, const real4<real> *__restrict__ b, const re
     x[i] = v;
 }
 
+// real (n) -> double (n)
+template <typename real> __global__ void k_to_double(size_t n, const real *__restrict__ v, double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)v[i];
+}
+
 // real4 (N) -> double (N, 3)
 template <typename real>
 __global__ void k_to_double3(int n, const real4<real> *__restrict__ v, double *__restrict__ out) {

@@ -344,6 +344,32 @@ template <typename T> static int upload(DevBuf &b, const T *h, size_t count, hip
 }
 
 // ---------------------------------------------------------------------------- box
+// The influence function of the convolution, Gc = 2 G_raw B(m) on the half spectrum (K1, K2,
+// K3/2 + 1) without the half-space weights (pgm_jax/md/pme.py:243-267), for the box whose inverse
+// is Hi (row-major) and volume c->volume.
+static void influence_host(const pgm_ctx *c, const double *Hi, std::vector<double> &G) {
+    const int K1 = c->s.grid[0], K2 = c->s.grid[1], K3 = c->s.grid[2], K3h = K3 / 2 + 1;
+    std::vector<double> m1 = moduli(K1, c->s.order), m2 = moduli(K2, c->s.order), m3 = moduli(K3, c->s.order);
+    const double R[9] = {Hi[0], Hi[3], Hi[6], Hi[1], Hi[4], Hi[7], Hi[2], Hi[5], Hi[8]};  // rows: reciprocal vectors (H^-T)
+    const double beta = c->s.ewald_beta;
+    G.assign((size_t)K1 * K2 * K3h, 0.0);
+    for (int a = 0; a < K1; ++a) {
+        const double fa = a <= (K1 - 1) / 2 ? a : a - K1;  // fftfreq
+        for (int b = 0; b < K2; ++b) {
+            const double fb = b <= (K2 - 1) / 2 ? b : b - K2;
+            for (int k = 0; k < K3h; ++k) {
+                const double mx = fa * R[0] + fb * R[3] + k * R[6];
+                const double my = fa * R[1] + fb * R[4] + k * R[7];
+                const double mz = fa * R[2] + fb * R[5] + k * R[8];
+                const double msq = mx * mx + my * my + mz * mz;
+                double g = 0.0;
+                if (msq > 0) g = std::exp(-M_PI * M_PI * msq / (beta * beta)) / (2.0 * M_PI * c->volume * msq);
+                G[((size_t)a * K2 + b) * K3h + k] = 2.0 * g / (m1[a] * m2[b] * m3[k]);
+            }
+        }
+    }
+}
+
 static int set_box(pgm_ctx *c, const double *H) {
     drop_graphs(c);
     for (int k = 0; k < 9; ++k) c->H[k] = H[k];
@@ -382,27 +408,8 @@ static int set_box(pgm_ctx *c, const double *H) {
         ncs[a] = std::max(1, (int)std::floor(h / rl));
     }
     c->nc = make_int3(ncs[0], ncs[1], ncs[2]);
-    // influence function Gc = 2 G_raw B(m) on the half spectrum (pgm_jax/md/pme.py:243-267)
-    const int K1 = c->s.grid[0], K2 = c->s.grid[1], K3 = c->s.grid[2], K3h = K3 / 2 + 1;
-    std::vector<double> m1 = moduli(K1, c->s.order), m2 = moduli(K2, c->s.order), m3 = moduli(K3, c->s.order);
-    const double R[9] = {Hi[0], Hi[3], Hi[6], Hi[1], Hi[4], Hi[7], Hi[2], Hi[5], Hi[8]};  // rows: reciprocal vectors (H^-T)
-    const double beta = c->s.ewald_beta;
-    std::vector<double> G((size_t)K1 * K2 * K3h);
-    for (int a = 0; a < K1; ++a) {
-        const double fa = a <= (K1 - 1) / 2 ? a : a - K1;  // fftfreq
-        for (int b = 0; b < K2; ++b) {
-            const double fb = b <= (K2 - 1) / 2 ? b : b - K2;
-            for (int k = 0; k < K3h; ++k) {
-                const double mx = fa * R[0] + fb * R[3] + k * R[6];
-                const double my = fa * R[1] + fb * R[4] + k * R[7];
-                const double mz = fa * R[2] + fb * R[5] + k * R[8];
-                const double msq = mx * mx + my * my + mz * mz;
-                double g = 0.0;
-                if (msq > 0) g = std::exp(-M_PI * M_PI * msq / (beta * beta)) / (2.0 * M_PI * c->volume * msq);
-                G[((size_t)a * K2 + b) * K3h + k] = 2.0 * g / (m1[a] * m2[b] * m3[k]);
-            }
-        }
-    }
+    std::vector<double> G;
+    influence_host(c, Hi, G);
     if (c->mixed) {
         std::vector<float> Gf(G.begin(), G.end());
         PGM_TRY(upload(c->Gc, Gf.data(), Gf.size(), c->stream));
@@ -410,6 +417,7 @@ static int set_box(pgm_ctx *c, const double *H) {
         PGM_TRY(upload(c->Gc, G.data(), G.size(), c->stream));
     }
     if (c->ewald) {  // k = 2 pi H^-1 m, f_k = (4 pi / V) exp(-k^2 / (4 beta^2)) / k^2 (pgm_jax/ewald.py:317-333)
+        const double beta = c->s.ewald_beta;
         std::vector<double4> kv((size_t)std::max(c->nk, 1));
         for (int k = 0; k < c->nk; ++k) {
             const double m0 = c->h_m[3 * k], m1 = c->h_m[3 * k + 1], m2 = c->h_m[3 * k + 2];
@@ -2246,6 +2254,81 @@ extern "C" int pgm_pme_energy(pgm_ctx *c, const double *pos, int with_charges, c
     if (!c || !pos || !d) return PGM_ERR_ARG;
     PGM_TRY(single_system(c, "pgm_pme_energy"));
     return c->mixed ? pme_api<float>(c, pos, with_charges, d, e_rec, grad_d) : pme_api<double>(c, pos, with_charges, d, e_rec, grad_d);
+}
+
+// PME.spread (pgm_jax/md/pme.py:306-337): the grid Q of the topology charges (with_charges) and
+// the dipoles d, merged from the spread tiles (or spread per atom on untiled grids).
+template <typename real> static int spread_api(pgm_ctx *c, const double *pos, int withq, const double *d, double *Q) {
+    using R4 = real4<real>;
+    if (pos != c->pos) drop_graphs(c);
+    c->pos = pos;
+    PGM_TRY(pme_bin<real>(c, pos));
+    hipStream_t st = c->stream;
+    const int n = c->n, P = c->s.order;
+    k_pack_dq<real><<<nblk(n, 256), 256, 0, st>>>(n, d, nullptr, withq ? c->q.as<double>() : nullptr, c->vdq.as<R4>());
+    LAUNCH_CHECK();
+    const R4 *dq = c->vdq.as<R4>();
+    const int *conv = nullptr;
+    const int iter = -1;
+    if (c->tiled) {
+        const PMETiles &t = c->tiles8;
+        const size_t lds = spread_lds_bytes(c);
+        if (withq) {
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, true><<<t.ntiles, WAVE, lds, st>>>(
+                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+        } else {
+            ORDER_SWITCH(P, (k_spread_tiles<real, PO, false><<<t.ntiles, WAVE, lds, st>>>(
+                                t, c->pg, c->tstart.as<int>(), c->tend.as<int>(), c->psidx.as<int>(), c->spl.as<real>(),
+                                c->loc.as<int>(), dq, c->blocks.as<real>(), conv, iter)));
+        }
+        LAUNCH_CHECK();
+        const int *K = c->s.grid;
+        k_merge_table<real><<<dim3(nblk(K[2], 128), K[1], K[0]), 128, 0, st>>>(
+            K[0], K[1], K[2], (size_t)t.ntiles * t.bsize, c->merge_tab.as<int>(), c->blocks.as<real>(), c->grid.as<real>(), conv,
+            iter);
+    } else {
+        k_fill<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), real(0));
+        const dim3 gs(nblk((long long)n * P, 256));
+        if (withq) {
+            ORDER_SWITCH(P, (k_spread<real, PO, true><<<gs, 256, 0, st>>>(n, pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+        } else {
+            ORDER_SWITCH(P, (k_spread<real, PO, false><<<gs, 256, 0, st>>>(n, pos, dq, c->pg, c->grid.as<real>(), conv, iter)));
+        }
+    }
+    LAUNCH_CHECK();
+    k_to_double<real><<<nblk((long long)c->ngrid, 256), 256, 0, st>>>(c->ngrid, c->grid.as<real>(), Q);
+    LAUNCH_CHECK();
+    return PGM_OK;
+}
+
+extern "C" int pgm_pme_spread(pgm_ctx *c, const double *pos, int with_charges, const double *d, double *Q) {
+    if (!c || !pos || !d || !Q) return PGM_ERR_ARG;
+    PGM_TRY(single_system(c, "pgm_pme_spread"));
+    if (c->ewald) {
+        g_err = "pgm_pme_spread: PME reciprocal space only (this context holds the exact Ewald sum)";
+        return PGM_ERR_STATE;
+    }
+    return c->mixed ? spread_api<float>(c, pos, with_charges, d, Q) : spread_api<double>(c, pos, with_charges, d, Q);
+}
+
+// PME.influence (pgm_jax/md/pme.py:243-267) of the context's box: G (K1, K2, K3/2 + 1) with the
+// half-space weights (1 on the planes m3 = 0 and m3 = K3/2, 2 elsewhere), float64, host.
+extern "C" int pgm_pme_influence(pgm_ctx *c, double *G) {
+    if (!c || !G) return PGM_ERR_ARG;
+    if (c->ewald) {
+        g_err = "pgm_pme_influence: PME reciprocal space only (this context holds the exact Ewald sum)";
+        return PGM_ERR_STATE;
+    }
+    std::vector<double> Gc;
+    influence_host(c, c->box.hinv, Gc);
+    const int K3 = c->s.grid[2], K3h = K3 / 2 + 1;
+    for (size_t i = 0; i < Gc.size(); ++i) {
+        const int k = (int)(i % K3h);
+        const bool single = k == 0 || (K3 % 2 == 0 && k == K3 / 2);
+        G[i] = single ? 0.5 * Gc[i] : Gc[i];
+    }
+    return PGM_OK;
 }
 
 template <typename real>
