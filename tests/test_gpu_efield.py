@@ -15,7 +15,9 @@ pgm_set_efield / pgm_field_result behind PGMForceField.compute(..., efield=(E, o
   * E = 0 gives the field-free results bit for bit, and switching the field off again restores
     them; the trial energy with the field from the converged dipoles equals compute's total;
   * MD in a field (RigidWaterMD(efield=...)): one FP64 step against the numpy SHAKE/RATTLE step
-    with the oracle's forces in the same field, and NVE conservation of the energy with -E . M.
+    with the oracle's forces in the same field, and NVE conservation of the energy with -E . M;
+  * energy_forces / energy_fixed_mu in the field at the golden's dipoles: the golden energies to
+    1e-10 (stationary in the dipoles) and forces to 1e-8.
 """
 
 import os
@@ -171,3 +173,20 @@ def test_md_with_field():
     r1 = md.run(200, 1.0)
     print(r0, r1, flush=True)
     assert abs(r1["etot"] - r0["etot"]) < 2e-3 * r1["ekin"], (r0["etot"], r1["etot"], r1["ekin"])
+
+
+def test_energy_at_fixed_dipoles_in_the_field():
+    """energy_forces / energy_fixed_mu with efield at the golden's own dipoles (energy_fixed_mu of
+    forcefield.py:2249-2292 with an external field): the total is stationary in the dipoles, so it
+    equals the golden total to 1e-10 relative; forces within 1e-8 of the largest (FP64)."""
+    import pgm_jax_amd as P
+
+    top, pos, H = systems.small_box(13, nm=0)
+    ff = P.PGMForceField(_system(top), H, P.MDSettings(precision="double", predictor="none", skin=0.05, **EFIELD))
+    with np.load(GOLD) as g:
+        e, F = ff.energy_forces(pos, g["E.mu"], H, efield=(E_VNM, None))
+        for k in ("elec", "vdw", "field", "total"):
+            ref = float(g[f"E.energy.{k}"])
+            assert abs(e[k] - ref) <= 1e-10 * abs(ref), (k, e[k], ref)
+        assert np.abs(F.cpu().numpy() - g["E.forces"]).max() <= 1e-8 * np.abs(g["E.forces"]).max()
+        assert ff.energy_fixed_mu(pos, H, g["E.mu"], efield=(E_VNM, None)) == e["total"]
