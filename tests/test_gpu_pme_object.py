@@ -2,8 +2,9 @@
 
 PME(grid, order, beta, dtype).influence / setup / spread / grad_dipoles / energy through
 pgm_pme_influence, pgm_pme_spread and pgm_pme_energy, on the skewed toy box (138 atoms) with its
-charges and seeded dipoles of 0.02 e nm: a 32^3 grid of order 6 and a 12^3 grid of
-order 4; charges switched between calls and a call without charges.  FP64: influence to 1e-13 of
+charges and seeded dipoles of 0.02 e nm: a 32^3 grid of order 6, a 12^3 grid of order 4 and a
+24 x 20 x 18 grid of odd order 5 (axes that are not powers of two: the library transform);
+charges switched between calls and a call without charges.  FP64: influence to 1e-13 of
 its largest value, Q to 1e-12, E_rec to 1e-11 relative, dU/dd to 1e-10 of its largest entry;
 float32 (FP32 kernels, FP64 sums): Q to 1e-6, E_rec to 1e-6, dU/dd to 1e-5 (influence is FP64
 on the host in both).
@@ -17,7 +18,7 @@ from oracle import systems
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((32, 32, 32), 6, 6.0), ((12, 12, 12), 4, 3.5)]
+CASES = [((32, 32, 32), 6, 6.0), ((12, 12, 12), 4, 3.5), ((24, 20, 18), 5, 4.0)]
 
 
 def _rel(a, b):
